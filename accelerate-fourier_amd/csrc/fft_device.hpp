@@ -1,0 +1,496 @@
+// Device side of the transform path: complex helpers, register butterflies,
+// LDS Stockham stages and the generic tile pass.
+//
+// Written for gfx950 (CDNA4): 64-wide wavefronts, 160 KiB LDS per CU.  A tile
+// of B lines lives in LDS; every stage reads R inputs per butterfly into
+// registers, applies the stage twiddles, runs the radix-R DFT and writes the
+// R outputs back in Stockham (self-sorting) order, so the result of the last
+// stage is in natural order with no bit reversal.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "af_pass.hpp"
+
+namespace af {
+
+template <typename T> struct Cx;
+template <> struct Cx<float> { using C = float2; using W = float4; };
+template <> struct Cx<double> { using C = double2; using W = double2; };
+
+__device__ __forceinline__ float2 mk(float a, float b) { return make_float2(a, b); }
+__device__ __forceinline__ double2 mk(double a, double b) { return make_double2(a, b); }
+template <class C> __device__ __forceinline__ C cadd(C a, C b) { return mk(a.x + b.x, a.y + b.y); }
+template <class C> __device__ __forceinline__ C csub(C a, C b) { return mk(a.x - b.x, a.y - b.y); }
+template <class C> __device__ __forceinline__ C cmul(C a, C b) {
+    return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <class C> __device__ __forceinline__ C cconj(C a) { return mk(a.x, -a.y); }
+template <class C, class T> __device__ __forceinline__ C cscale(C a, T s) { return mk(a.x * s, a.y * s); }
+// a * (s i)
+template <class C, class T> __device__ __forceinline__ C mul_si(C a, T s) { return mk(-s * a.y, s * a.x); }
+
+// Twiddle-table entries hold exp(-2 pi i a / N) (the forward sign).  For FP32
+// an entry is a compensated pair hi + lo (float4: hi.re, hi.im, lo.re, lo.im)
+// generated from FP64; for FP64 it is a double2.
+//
+// twmul(x, w, s) multiplies x by exp(s 2 pi i a / N): by the table value
+// itself for s = -1 and by its conjugate for s = +1.
+__device__ __forceinline__ float2 twmul(float2 x, float4 w, float s, bool plain) {
+    const float hy = -s * w.y;
+    float re = x.x * w.x - x.y * hy;
+    float im = x.x * hy + x.y * w.x;
+    if (!plain) {
+        const float ly = -s * w.w;
+        re = fmaf(x.x, w.z, fmaf(-x.y, ly, re));
+        im = fmaf(x.x, ly, fmaf(x.y, w.z, im));
+    }
+    return make_float2(re, im);
+}
+__device__ __forceinline__ double2 twmul(double2 x, double2 w, double s, bool) {
+    const double wy = -s * w.y;
+    return make_double2(x.x * w.x - x.y * wy, x.x * wy + x.y * w.x);
+}
+// multiply by a stored vector entry as it is
+template <class C, class W> __device__ __forceinline__ C vmul(C x, W w, bool plain) {
+    return twmul(x, w, decltype(x.x)(-1), plain);
+}
+
+// Split an FP64 complex value into the table format of precision T.
+__device__ __forceinline__ void to_w(double2 v, float4 &w) {
+    const float hx = (float)v.x, hy = (float)v.y;
+    w = make_float4(hx, hy, (float)(v.x - (double)hx), (float)(v.y - (double)hy));
+}
+__device__ __forceinline__ void to_w(double2 v, double2 &w) { w = v; }
+__device__ __forceinline__ double2 from_w(float4 w) {
+    return make_double2((double)w.x + (double)w.z, (double)w.y + (double)w.w);
+}
+__device__ __forceinline__ double2 from_w(double2 w) { return w; }
+
+// x * w for an FP64 twiddle w, in precision T (compensated for FP32).
+__device__ __forceinline__ float2 dmul(float2 x, double2 w, bool plain) {
+    float4 ww;
+    to_w(w, ww);
+    return twmul(x, ww, -1.0f, plain);
+}
+__device__ __forceinline__ double2 dmul(double2 x, double2 w, bool) { return cmul(x, w); }
+
+// ---------------------------------------------------------------------------
+// Register butterflies: in-place DFT of R values, exp(s 2 pi i j k / R).
+// ---------------------------------------------------------------------------
+
+// x * exp(s 2 pi i k / R) for a compile-time-foldable k in [0, R/2)
+template <int R, class C, class T>
+__device__ __forceinline__ C mul_wr(C x, int k, T s) {
+    using R_ = decltype(x.x);
+    constexpr double SQ = 0.707106781186547524401;
+    if (k == 0) return x;
+    if (4 * k == R) return mul_si(x, s);
+    if (8 * k == R) return mk(R_(SQ) * (x.x - s * x.y), R_(SQ) * (x.y + s * x.x));
+    if (8 * k == 3 * R) return mk(R_(SQ) * (-x.x - s * x.y), R_(SQ) * (-x.y + s * x.x));
+    // remaining cases occur for R = 16: angles k pi / 8 with k odd
+    constexpr double C8 = 0.923879532511286756128, S8 = 0.382683432365089771728;
+    double c = 0, sn = 0;
+    const int k16 = k * (16 / R);
+    if (k16 == 1) { c = C8; sn = S8; }
+    else if (k16 == 3) { c = S8; sn = C8; }
+    else if (k16 == 5) { c = -S8; sn = C8; }
+    else { c = -C8; sn = S8; }
+    const R_ cc = R_(c), ss = R_(sn) * s;
+    return mk(x.x * cc - x.y * ss, x.x * ss + x.y * cc);
+}
+
+template <int R, class C, class T>
+__device__ __forceinline__ void dft_p2(C *v, T s) {
+    if constexpr (R == 1) {
+        return;
+    } else if constexpr (R == 2) {
+        const C a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else {
+        constexpr int M = R / 2;
+        C e[M], o[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) { e[j] = v[2 * j]; o[j] = v[2 * j + 1]; }
+        dft_p2<M>(e, s);
+        dft_p2<M>(o, s);
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const C t = mul_wr<R>(o[k], k, s);
+            v[k] = cadd(e[k], t);
+            v[k + M] = csub(e[k], t);
+        }
+    }
+}
+
+template <int R> struct OddC;
+template <> struct OddC<3> {
+    static constexpr double c[3] = {1.0, -0.5, -0.5};
+    static constexpr double s[3] = {0.0, 0.866025403784438646764, -0.866025403784438646764};
+};
+template <> struct OddC<5> {
+    static constexpr double c[5] = {1.0, 0.309016994374947424102, -0.809016994374947424102,
+                                    -0.809016994374947424102, 0.309016994374947424102};
+    static constexpr double s[5] = {0.0, 0.951056516295153572116, 0.587785252292473129169,
+                                    -0.587785252292473129169, -0.951056516295153572116};
+};
+template <> struct OddC<7> {
+    static constexpr double c[7] = {1.0, 0.623489801858733530525, -0.222520933956314404289,
+                                    -0.900968867902419126236, -0.900968867902419126236,
+                                    -0.222520933956314404289, 0.623489801858733530525};
+    static constexpr double s[7] = {0.0, 0.781831482468029808708, 0.974927912181823607018,
+                                    0.433883739117558120476, -0.433883739117558120476,
+                                    -0.974927912181823607018, -0.781831482468029808708};
+};
+template <> struct OddC<11> {
+    static constexpr double c[11] = {1.0, 0.841253532831181168862, 0.415415013001886425529,
+                                     -0.142314838273285140444, -0.654860733945285064057,
+                                     -0.95949297361449738989, -0.95949297361449738989,
+                                     -0.654860733945285064057, -0.142314838273285140444,
+                                     0.415415013001886425529, 0.841253532831181168862};
+    static constexpr double s[11] = {0.0, 0.540640817455597582108, 0.909631995354518371412,
+                                     0.989821441880932732376, 0.755749574354258283774,
+                                     0.281732556841429697711, -0.281732556841429697711,
+                                     -0.755749574354258283774, -0.989821441880932732376,
+                                     -0.909631995354518371412, -0.540640817455597582108};
+};
+template <> struct OddC<13> {
+    static constexpr double c[13] = {1.0, 0.8854560256532098959, 0.568064746731155802512,
+                                     0.120536680255323053349, -0.35460488704253562597,
+                                     -0.748510748171101098635, -0.970941817426052027157,
+                                     -0.970941817426052027157, -0.748510748171101098635,
+                                     -0.35460488704253562597, 0.120536680255323053349,
+                                     0.568064746731155802512, 0.8854560256532098959};
+    static constexpr double s[13] = {0.0, 0.464723172043768545656, 0.82298386589365639458,
+                                     0.992708874098053992801, 0.93501624268541482344,
+                                     0.663122658240795202377, 0.239315664287557767149,
+                                     -0.239315664287557767149, -0.663122658240795202377,
+                                     -0.93501624268541482344, -0.992708874098053992801,
+                                     -0.82298386589365639458, -0.464723172043768545656};
+};
+
+// Odd prime R: pair x_j with x_{R-j} so each output pair (k, R-k) shares the
+// cosine and sine sums.
+template <int R, class C, class T>
+__device__ __forceinline__ void dft_odd(C *v, T s) {
+    using R_ = decltype(v[0].x);
+    constexpr int H = (R - 1) / 2;
+    C a[H], b[H];
+    const C x0 = v[0];
+    C sum = x0;
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+        a[j - 1] = cadd(v[j], v[R - j]);
+        b[j - 1] = csub(v[j], v[R - j]);
+        sum = cadd(sum, a[j - 1]);
+    }
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        C t = x0, u = mk(R_(0), R_(0));
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            const int m = (j * k) % R;
+            const R_ c = R_(OddC<R>::c[m]), sn = R_(OddC<R>::s[m]);
+            t = mk(t.x + a[j - 1].x * c, t.y + a[j - 1].y * c);
+            u = mk(u.x + b[j - 1].x * sn, u.y + b[j - 1].y * sn);
+        }
+        v[k] = mk(t.x - s * u.y, t.y + s * u.x);
+        v[R - k] = mk(t.x + s * u.y, t.y - s * u.x);
+    }
+    v[0] = sum;
+}
+
+template <int R, class C, class T>
+__device__ __forceinline__ void dft(C *v, T s) {
+    if constexpr (R == 2 || R == 4 || R == 8 || R == 16) dft_p2<R>(v, s);
+    else dft_odd<R>(v, s);
+}
+
+// LDS index padding: one spare slot every 2^ps elements spreads strided
+// accesses over the banks.
+__device__ __forceinline__ int pidx(int i, int ps) { return ps ? i + (i >> ps) : i; }
+
+// One Stockham stage of radix R over a tile of B lines of length L.
+// Butterfly i of a line reads elements i + j L/R and writes
+// (i - k) R + k + j p with k = i mod p, p = product of the previous radices.
+template <typename T, int R>
+__device__ __forceinline__ void stage(typename Cx<T>::C *lds, const int L, const int ld, const int B,
+                                      const int p, const typename Cx<T>::W *__restrict__ tw, const T s,
+                                      const int ps, const bool plain) {
+    using C = typename Cx<T>::C;
+    constexpr int MB = (16 + R - 1) / R;
+    const int nbl = L / R;
+    const int nbf = B * nbl;
+    const int tws = L / (p * R);
+    const int tid = threadIdx.x, NT = blockDim.x;
+    C v[MB][R];
+    int base[MB], ii[MB];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int bf = tid + b * NT;
+        base[b] = 0;
+        ii[b] = 0;
+        if (bf < nbf) {
+            const int line = bf / nbl;
+            const int i = bf - line * nbl;
+            base[b] = line * ld;
+            ii[b] = i;
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[b][j] = lds[pidx(base[b] + i + j * nbl, ps)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int bf = tid + b * NT;
+        if (bf < nbf) {
+            const int i = ii[b];
+            const int k = i % p;
+            if (p > 1) {
+#pragma unroll
+                for (int j = 1; j < R; ++j) v[b][j] = twmul(v[b][j], tw[j * k * tws], s, plain);
+            }
+            dft<R>(v[b], s);
+            const int o = base[b] + (i - k) * R + k;
+#pragma unroll
+            for (int j = 0; j < R; ++j) lds[pidx(o + j * p, ps)] = v[b][j];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void run_stages(typename Cx<T>::C *lds, const AfPass &d, const T s) {
+    using W = typename Cx<T>::W;
+    const W *tw = (const W *)d.tw;
+    const bool plain = (d.flags & AFP_PLAIN_TW) != 0;
+    for (int st = 0; st < d.nst; ++st) {
+        const int R = d.radix[st], p = d.pst[st];
+        switch (R) {
+        case 2: stage<T, 2>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
+        case 3: stage<T, 3>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
+        case 4: stage<T, 4>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
+        case 5: stage<T, 5>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
+        case 7: stage<T, 7>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
+        case 8: stage<T, 8>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
+        case 11: stage<T, 11>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
+        case 13: stage<T, 13>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
+        default: stage<T, 16>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Element access
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ typename Cx<T>::C ld_c(const void *re, const void *im, int64_t a) {
+    if (im) return mk(((const T *)re)[a], ((const T *)im)[a]);
+    return ((const typename Cx<T>::C *)re)[a];
+}
+template <typename T>
+__device__ __forceinline__ void st_c(void *re, void *im, int64_t a, typename Cx<T>::C v) {
+    if (im) {
+        ((T *)re)[a] = v.x;
+        ((T *)im)[a] = v.y;
+    } else {
+        ((typename Cx<T>::C *)re)[a] = v;
+    }
+}
+
+// FP64 value of the four-step twiddle W_N^(sign * a), a < N.
+__device__ __forceinline__ double2 ft_twiddle(const AfPass &d, int64_t a) {
+    const double2 c = ((const double2 *)d.ft_coarse)[a >> d.ftShift];
+    const double2 f = ((const double2 *)d.ft_fine)[a & ((int64_t(1) << d.ftShift) - 1)];
+    double2 w = cmul(c, f);
+    if (d.ftSign > 0) w.y = -w.y;
+    return w;
+}
+
+struct TileLines {
+    int64_t row[64];
+    int64_t line[64];
+    int valid[64];
+};
+
+// ---------------------------------------------------------------------------
+// One tile of a pass: load -> stages -> [spectrum product -> inverse stages]
+// -> store.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C *lds, TileLines &tl,
+                          typename Cx<T>::C *s_x0, typename Cx<T>::C *s_dc) {
+    using C = typename Cx<T>::C;
+    using W = typename Cx<T>::W;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const int B = d.B, L = d.L, ld = d.ld, ps = d.padshift;
+    const bool plain = (d.flags & AFP_PLAIN_TW) != 0;
+    const int64_t nlines_total = d.nrows * d.nlines;
+
+    if (tid < B) {
+        const int64_t gl = tile * B + tid;
+        const int v = gl < nlines_total;
+        tl.valid[tid] = v;
+        tl.row[tid] = v ? gl / d.nlines : 0;
+        tl.line[tid] = v ? gl - (gl / d.nlines) * d.nlines : 0;
+        if (v && (d.flags & AFP_RADER)) s_x0[tid] = ld_c<T>(d.in_re, d.in_im, tl.row[tid] * d.in_bs);
+    }
+    __syncthreads();
+
+    // ---- load phase ----
+    const int total = B * L;
+    for (int idx = tid; idx < total; idx += NT) {
+        int b, e;
+        if (d.in_lines_fast) { b = idx % B; e = idx / B; }
+        else { b = idx / L; e = idx - b * L; }
+        C v = mk(T(0), T(0));
+        if (tl.valid[b]) {
+            const int64_t r = tl.row[b], l = tl.line[b];
+            const int64_t rb = r * d.in_bs;
+            if (d.flags & AFP_REAL_ENTANGLE) {
+                // fromSpectrum (Real.hs:96-107): z = (S_e + S_{e+L}) + i W_{2L}^{+e} (S_e - S_{e+L})
+                const C p0 = ld_c<T>(d.in_re, d.in_im, rb + e);
+                const C p1 = ld_c<T>(d.in_re, d.in_im, rb + e + L);
+                const C t = twmul(csub(p0, p1), ((const W *)d.aux_tw)[e], T(1), plain);
+                v = cadd(cadd(p0, p1), mk(-t.y, t.x));
+            } else if (d.flags & AFP_UNTANGLE_ADJ) {
+                // Adjoint of toSpectrum: x -> F(x) maps reals to the full spectrum, so
+                // its adjoint is G -> Re(F^H G) = fromSpectrum(H) with H the Hermitian
+                // part (G_k + conj G_{-k}) / 2, which makes F^H H real.  Load H and
+                // entangle it exactly as fromSpectrum does.
+                const int er = (L - e) % L;
+                const C g0 = ld_c<T>(d.in_re, d.in_im, rb + e);
+                const C g1 = ld_c<T>(d.in_re, d.in_im, rb + e + L);
+                const C h0 = ld_c<T>(d.in_re, d.in_im, rb + er + L);
+                const C h1 = ld_c<T>(d.in_re, d.in_im, rb + er);
+                // Hermitian part of G at e and e+L: (G_e + conj G_{-e}) / 2, where -e of e is
+                // (2L - e) mod 2L, i.e. L + er for e in [0, L) (er = (L-e) mod L) unless e = 0.
+                C q0, q1;
+                if (e == 0) {
+                    q0 = mk(g0.x, T(0));            // (G_0 + conj G_0)/2
+                    q1 = mk(g1.x, T(0));            // (G_L + conj G_L)/2
+                } else {
+                    q0 = cscale(cadd(g0, cconj(h0)), T(0.5));
+                    q1 = cscale(cadd(g1, cconj(h1)), T(0.5));
+                }
+                const C t = twmul(csub(q0, q1), ((const W *)d.aux_tw)[e], T(1), plain);
+                v = cadd(cadd(q0, q1), mk(-t.y, t.x));
+            } else {
+                const int64_t pos = l * d.in_ls + (int64_t)e * d.in_es;
+                if (pos < d.in_len) {
+                    const int64_t a = d.in_map ? (int64_t)d.in_map[pos] : pos;
+                    v = ld_c<T>(d.in_re, d.in_im, rb + a);
+                    if (d.in_mul) v = vmul(v, ((const W *)d.in_mul)[pos], plain);
+                }
+            }
+        }
+        lds[pidx(b * ld + e, ps)] = v;
+    }
+    __syncthreads();
+
+    // ---- first stage sequence ----
+    const T s1 = T(d.sign);
+    run_stages<T>(lds, d, s1);
+
+    // ---- convolution: spectrum product and inverse stages ----
+    if (d.flags & AFP_CONV) {
+        for (int idx = tid; idx < total; idx += NT) {
+            const int b = idx / L, e = idx - b * L;
+            if (!tl.valid[b]) continue;
+            const int64_t r = tl.row[b], l = tl.line[b];
+            const int li = pidx(b * ld + e, ps);
+            C v = lds[li];
+            if (l == 0 && e == 0) {
+                if (d.flags & AFP_RADER) s_dc[b] = v;
+                if (d.flags & AFP_DC_CAPTURE) ((C *)d.dc_out)[r] = v;
+            }
+            const int64_t si = r * d.spec_bs + l * d.spec_ls + e;
+            if (d.flags & AFP_SPEC_PLAIN) lds[li] = cmul(v, ((const C *)d.spec)[si]);
+            else lds[li] = vmul(v, ((const W *)d.spec)[si], plain);
+        }
+        __syncthreads();
+        run_stages<T>(lds, d, -s1);
+    }
+
+    // ---- store phase ----
+    if (d.flags & AFP_REAL_UNTANGLE) {
+        // toSpectrum (Real.hs:53-70) on the half-length spectrum Z of this line
+        for (int idx = tid; idx < total; idx += NT) {
+            const int b = idx / L, e = idx - b * L;
+            if (!tl.valid[b]) continue;
+            const int64_t ob = tl.row[b] * d.out_bs;
+            const C zk = lds[pidx(b * ld + e, ps)];
+            const C zr = lds[pidx(b * ld + (L - e) % L, ps)];
+            const C xp = cscale(cadd(zk, zr), T(0.5)), xm = cscale(csub(zk, zr), T(0.5));
+            const C ev = mk(xp.x, xm.y);
+            // odds = i W_{2L}^e (xm.re + i xp.im) with the forward-sign W_{2L}^e
+            const C t = twmul(mk(xm.x, xp.y), ((const W *)d.aux_tw)[e], T(-1), plain);
+            const C od = mk(-t.y, t.x);
+            C lo = csub(ev, od), hi = cadd(ev, od);
+            if (d.flags & AFP_SCALE) { lo = cscale(lo, T(d.scale)); hi = cscale(hi, T(d.scale)); }
+            st_c<T>(d.out_re, d.out_im, ob + e, lo);
+            st_c<T>(d.out_re, d.out_im, ob + e + L, hi);
+        }
+        __syncthreads();
+        return;
+    }
+    if (d.flags & AFP_ENTANGLE_ADJ) {
+        // Adjoint of fromSpectrum's entangling z_e = S_e (1 + c_e) + S_{e+L} (1 - c_e),
+        // c_e = i conj(W_{2L}^e): from the half-length cotangent U (after the
+        // opposite-sign sub-transform) dS_e = U_e + conj(c_e) U_e and
+        // dS_{e+L} = U_e - conj(c_e) U_e, with conj(c_e) = -i W_{2L}^e.
+        for (int idx = tid; idx < total; idx += NT) {
+            const int b = idx / L, e = idx - b * L;
+            if (!tl.valid[b]) continue;
+            const int64_t ob = tl.row[b] * d.out_bs;
+            const C u = lds[pidx(b * ld + e, ps)];
+            // c = conj(i W_{2L}^{+e}) * u = -i W_{2L}^{-e} u
+            const C t = twmul(u, ((const W *)d.aux_tw)[e], T(-1), plain);
+            const C c = mk(t.y, -t.x);
+            st_c<T>(d.out_re, d.out_im, ob + e, cadd(u, c));
+            st_c<T>(d.out_re, d.out_im, ob + e + L, csub(u, c));
+        }
+        __syncthreads();
+        return;
+    }
+    const bool rader_any = (d.flags & (AFP_RADER | AFP_RADER_OUT)) != 0;
+    for (int idx = tid; idx < total; idx += NT) {
+        int b, e;
+        if (d.out_lines_fast) { b = idx % B; e = idx / B; }
+        else { b = idx / L; e = idx - b * L; }
+        if (!tl.valid[b]) continue;
+        const int64_t r = tl.row[b], l = tl.line[b];
+        const int64_t o = l * d.out_ls + (int64_t)e * d.out_es;
+        if (o >= d.out_len) continue;
+        C v = lds[pidx(b * ld + e, ps)];
+        if (d.flags & AFP_POSTTW) {
+            // l * e < 2^62 for every supported N, so the product is exact in 64 bits
+            const uint64_t le = (uint64_t)l * (uint64_t)e, N = (uint64_t)d.ftN;
+            const int64_t a = (int64_t)(((N & (N - 1)) == 0) ? (le & (N - 1)) : (le % N));
+            v = dmul(v, ft_twiddle(d, a), plain);
+        }
+        if (d.out_mul) v = vmul(v, ((const W *)d.out_mul)[o], plain);
+        if (d.flags & AFP_SCALE) v = cscale(v, T(d.scale));
+        const int64_t ob = r * d.out_bs;
+        if (rader_any) {
+            C x0;
+            if (d.flags & AFP_RADER) x0 = s_x0[b];
+            else x0 = ld_c<T>(d.x0_re, d.x0_im, r * d.x0_bs);
+            v = cadd(v, x0);
+        }
+        const int64_t a = d.out_map ? (int64_t)d.out_map[o] : o;
+        st_c<T>(d.out_re, d.out_im, ob + a, v);
+    }
+    if (rader_any && tid < B && tl.valid[tid] && tl.line[tid] == 0) {
+        // X[0] = x0 + sum of the other inputs = x0 + DC of the permuted sequence
+        const int64_t r = tl.row[tid];
+        C x0, dc;
+        if (d.flags & AFP_RADER) { x0 = s_x0[tid]; dc = s_dc[tid]; }
+        else { x0 = ld_c<T>(d.x0_re, d.x0_im, r * d.x0_bs); dc = ((const C *)d.dc_in)[r]; }
+        st_c<T>(d.out_re, d.out_im, r * d.out_bs, cadd(x0, dc));
+    }
+    __syncthreads();
+}
+
+}  // namespace af

@@ -1,0 +1,321 @@
+// HIP kernels of the transform path for gfx950 and their host launchers.
+//
+//  k_pass          one pass (tiles of B lines in LDS) of a transform plan
+//  k_pass_grouped  the same for many pass descriptors in one launch (uneven batches)
+//  k_real_conv     fused real FFT convolution y = fromSpectrum(toSpectrum x * toSpectrum k)
+//  k_real_conv_vjp its adjoint (dx, dk from the cotangent w) in one launch
+//  k_gen_*         FP64 generation of twiddles, chirps and Rader kernels on the GPU
+#include <hip/hip_runtime.h>
+
+#include "fft_device.hpp"
+#include "kernels.hpp"
+
+namespace af {
+
+// ---------------------------------------------------------------------------
+// Pass kernels
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_pass(const AfPass d) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ TileLines tl;
+    __shared__ typename Cx<T>::C s_x0[64], s_dc[64];
+    auto *lds = (typename Cx<T>::C *)smem;
+    for (int64_t tile = blockIdx.x; tile < d.ntiles; tile += gridDim.x)
+        pass_tile<T>(d, tile, lds, tl, s_x0, s_dc);
+}
+
+// tile_prefix[i] = first global tile of descriptor i; tile_prefix[ndesc] = total.
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_pass_grouped(const AfPass *__restrict__ descs, const int64_t *__restrict__ tile_prefix, const int ndesc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ TileLines tl;
+    __shared__ typename Cx<T>::C s_x0[64], s_dc[64];
+    auto *lds = (typename Cx<T>::C *)smem;
+    const int64_t total = tile_prefix[ndesc];
+    for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
+        int lo = 0, hi = ndesc - 1;
+        while (lo < hi) {  // last descriptor whose first tile is <= g
+            const int mid = (lo + hi + 1) >> 1;
+            if (tile_prefix[mid] <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        pass_tile<T>(descs[lo], g - tile_prefix[lo], lds, tl, s_x0, s_dc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused real FFT convolution, one row per workgroup (Real.hs:49-107 composed
+// with the pointwise product of Planned.hs:816-821).
+// ---------------------------------------------------------------------------
+
+// Untangled full spectrum at index e of the half-length spectrum Z:
+// S_e = ev - od, S_{e+L} = ev + od (toSpectrum, Real.hs:53-70).
+template <typename T>
+__device__ __forceinline__ void untangle(typename Cx<T>::C zk, typename Cx<T>::C zr, typename Cx<T>::W w,
+                                         bool plain, typename Cx<T>::C &lo, typename Cx<T>::C &hi) {
+    using C = typename Cx<T>::C;
+    const C xp = cscale(cadd(zk, zr), T(0.5)), xm = cscale(csub(zk, zr), T(0.5));
+    const C ev = mk(xp.x, xm.y);
+    const C t = twmul(mk(xm.x, xp.y), w, T(-1), plain);
+    const C od = mk(-t.y, t.x);
+    lo = csub(ev, od);
+    hi = cadd(ev, od);
+}
+
+// fromSpectrum's entangling (Real.hs:96-107): z_e = (p0 + p1) + i conj(W_{2L}^e) (p0 - p1)
+template <typename T>
+__device__ __forceinline__ typename Cx<T>::C entangle(typename Cx<T>::C p0, typename Cx<T>::C p1,
+                                                      typename Cx<T>::W w, bool plain) {
+    const auto t = twmul(csub(p0, p1), w, T(1), plain);
+    return cadd(cadd(p0, p1), mk(-t.y, t.x));
+}
+
+template <typename T>
+__device__ __forceinline__ void load_real_row(typename Cx<T>::C *buf, const AfPass &d, const T *src) {
+    const auto *s2 = (const typename Cx<T>::C *)src;
+    for (int e = threadIdx.x; e < d.L; e += blockDim.x) buf[pidx(e, d.padshift)] = s2[e];
+}
+template <typename T>
+__device__ __forceinline__ void store_real_row(const typename Cx<T>::C *buf, const AfPass &d, T *dst) {
+    auto *d2 = (typename Cx<T>::C *)dst;
+    for (int e = threadIdx.x; e < d.L; e += blockDim.x) d2[e] = buf[pidx(e, d.padshift)];
+}
+
+// conj_b: multiply by conj of the second spectrum (correlation, for the adjoint).
+// Pairs (e, L-e) are handled by one thread, so the update is in place.
+template <typename T>
+__device__ __forceinline__ void spectral_product(typename Cx<T>::C *A, typename Cx<T>::C *Bb,
+                                                 typename Cx<T>::C *dst, const AfPass &d, bool conj_b) {
+    using C = typename Cx<T>::C;
+    using W = typename Cx<T>::W;
+    const int L = d.L, ps = d.padshift;
+    const bool plain = (d.flags & AFP_PLAIN_TW) != 0;
+    const W *aux = (const W *)d.aux_tw;
+    for (int e = threadIdx.x; e <= L / 2; e += blockDim.x) {
+        const int er = (L - e) % L;
+        const C a0 = A[pidx(e, ps)], a1 = A[pidx(er, ps)];
+        const C b0 = Bb[pidx(e, ps)], b1 = Bb[pidx(er, ps)];
+        C sa_lo, sa_hi, sb_lo, sb_hi;
+        untangle<T>(a0, a1, aux[e], plain, sa_lo, sa_hi);
+        untangle<T>(b0, b1, aux[e], plain, sb_lo, sb_hi);
+        if (conj_b) { sb_lo = cconj(sb_lo); sb_hi = cconj(sb_hi); }
+        const C z0 = entangle<T>(cmul(sa_lo, sb_lo), cmul(sa_hi, sb_hi), aux[e], plain);
+        C z1 = z0;
+        if (er != e) {
+            C ta_lo, ta_hi, tb_lo, tb_hi;
+            untangle<T>(a1, a0, aux[er], plain, ta_lo, ta_hi);
+            untangle<T>(b1, b0, aux[er], plain, tb_lo, tb_hi);
+            if (conj_b) { tb_lo = cconj(tb_lo); tb_hi = cconj(tb_hi); }
+            z1 = entangle<T>(cmul(ta_lo, tb_lo), cmul(ta_hi, tb_hi), aux[er], plain);
+        }
+        dst[pidx(e, ps)] = z0;
+        if (er != e) dst[pidx(er, ps)] = z1;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_real_conv(const AfPass d, const T *__restrict__ x, const T *__restrict__ k, T *__restrict__ y) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    using C = typename Cx<T>::C;
+    const int64_t N = 2 * (int64_t)d.L;
+    C *A = (C *)smem;
+    C *Bb = A + pidx(d.ld, d.padshift);
+    for (int64_t r = blockIdx.x; r < d.nrows; r += gridDim.x) {
+        load_real_row<T>(A, d, x + r * N);
+        load_real_row<T>(Bb, d, k + r * N);
+        __syncthreads();
+        run_stages<T>(A, d, T(-1));
+        run_stages<T>(Bb, d, T(-1));
+        spectral_product<T>(A, Bb, A, d, false);
+        __syncthreads();
+        run_stages<T>(A, d, T(1));
+        store_real_row<T>(A, d, y + r * N);
+        __syncthreads();
+    }
+}
+
+// dk = fromSpectrum(S_w conj S_x), dx = fromSpectrum(S_w conj S_k)
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_real_conv_vjp(const AfPass d, const T *__restrict__ x, const T *__restrict__ k, const T *__restrict__ w,
+                T *__restrict__ dx, T *__restrict__ dk) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    using C = typename Cx<T>::C;
+    const int64_t N = 2 * (int64_t)d.L;
+    C *A = (C *)smem;
+    C *Bb = A + pidx(d.ld, d.padshift);
+    for (int64_t r = blockIdx.x; r < d.nrows; r += gridDim.x) {
+        load_real_row<T>(A, d, w + r * N);
+        load_real_row<T>(Bb, d, x + r * N);
+        __syncthreads();
+        run_stages<T>(A, d, T(-1));
+        run_stages<T>(Bb, d, T(-1));
+        spectral_product<T>(A, Bb, Bb, d, true);
+        __syncthreads();
+        run_stages<T>(Bb, d, T(1));
+        store_real_row<T>(Bb, d, dk + r * N);
+        __syncthreads();
+        load_real_row<T>(Bb, d, k + r * N);
+        __syncthreads();
+        run_stages<T>(Bb, d, T(-1));
+        spectral_product<T>(A, Bb, Bb, d, true);
+        __syncthreads();
+        run_stages<T>(Bb, d, T(1));
+        store_real_row<T>(Bb, d, dx + r * N);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Table generation in FP64 (sincospi of exactly reduced rational angles)
+// ---------------------------------------------------------------------------
+
+// out[i] = exp(-2 pi i (i*mul mod N) / N)
+template <typename WT>
+__global__ void k_gen_twiddles(WT *__restrict__ out, int64_t N, int64_t count, int64_t mul) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int64_t a = (int64_t)(((unsigned __int128)i * (uint64_t)mul) % (uint64_t)N);
+    double s, c;
+    sincospi(2.0 * (double)a / (double)N, &s, &c);
+    to_w(make_double2(c, -s), out[i]);
+}
+
+// Bluestein chirp: out[k] = exp(sign i pi (kt^2 mod 2n) / n), kt = k or k - M
+// (wrapped, as Private.chirp with padLen M, Private.hs:428-436), optionally
+// conjugated.  kt^2 mod 2n is exact in 64-bit integers.
+__global__ void k_gen_chirp(double2 *__restrict__ out, int64_t n, int64_t count, int sign, int64_t M, int conj) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    int64_t kt = k;
+    if (M > 0 && 2 * k >= M) kt = k - M;
+    if (kt < 0) kt = -kt;
+    const uint64_t r = (uint64_t)(((unsigned __int128)kt * (uint64_t)kt) % (uint64_t)(2 * n));
+    double s, c;
+    sincospi((double)r / (double)n, &s, &c);
+    double im = sign * s;
+    if (conj) im = -im;
+    out[k] = make_double2(c, im);
+}
+
+// Rader kernel: out[i] = exp(sign 2 pi i powg[i] / p)  (levelCachePrime, Planned.hs:655-664)
+__global__ void k_gen_rader_kernel(double2 *__restrict__ out, const int32_t *__restrict__ powg, int64_t p,
+                                   int64_t count, int sign) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    double s, c;
+    sincospi(2.0 * (double)powg[i] / (double)p, &s, &c);
+    out[i] = make_double2(c, sign * s);
+}
+
+// out[i] = scale * in[src(i)] in the table format; with N1 > 0 the source is the
+// four-step row order: i = k1*N2 + k2 reads in[k1 + N1*k2].
+template <typename WT>
+__global__ void k_convert(WT *__restrict__ out, const double2 *__restrict__ in, int64_t count, double scale,
+                          int64_t N1, int64_t N2) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    int64_t src = i;
+    if (N1 > 0) src = (i / N2) + N1 * (i % N2);
+    const double2 v = in[src];
+    to_w(make_double2(v.x * scale, v.y * scale), out[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename T>
+static hipError_t launch_pass_t(const AfPass &d, hipStream_t s) {
+    if (d.ntiles <= 0) return hipSuccess;
+    const int64_t g = d.ntiles < (int64_t)1 << 30 ? d.ntiles : (int64_t)1 << 30;
+    hipLaunchKernelGGL(k_pass<T>, dim3((unsigned)g), dim3(d.nthreads), d.lds_bytes, s, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass(int prec, const AfPass &d, hipStream_t s) {
+    return prec == PREC_F64 ? launch_pass_t<double>(d, s) : launch_pass_t<float>(d, s);
+}
+
+hipError_t launch_pass_grouped(int prec, const AfPass *descs, const int64_t *prefix, int ndesc,
+                               int64_t total_tiles, int nthreads, int lds_bytes, hipStream_t s) {
+    if (total_tiles <= 0) return hipSuccess;
+    const int64_t g = total_tiles < (int64_t)1 << 30 ? total_tiles : (int64_t)1 << 30;
+    if (prec == PREC_F64)
+        hipLaunchKernelGGL(k_pass_grouped<double>, dim3((unsigned)g), dim3(nthreads), lds_bytes, s, descs, prefix, ndesc);
+    else
+        hipLaunchKernelGGL(k_pass_grouped<float>, dim3((unsigned)g), dim3(nthreads), lds_bytes, s, descs, prefix, ndesc);
+    return hipGetLastError();
+}
+
+hipError_t launch_real_conv(int prec, const AfPass &d, const void *x, const void *k, void *y, hipStream_t s) {
+    if (d.nrows <= 0) return hipSuccess;
+    if (prec == PREC_F64)
+        hipLaunchKernelGGL(k_real_conv<double>, dim3((unsigned)d.nrows), dim3(d.nthreads), d.lds_bytes, s, d,
+                           (const double *)x, (const double *)k, (double *)y);
+    else
+        hipLaunchKernelGGL(k_real_conv<float>, dim3((unsigned)d.nrows), dim3(d.nthreads), d.lds_bytes, s, d,
+                           (const float *)x, (const float *)k, (float *)y);
+    return hipGetLastError();
+}
+
+hipError_t launch_real_conv_vjp(int prec, const AfPass &d, const void *x, const void *k, const void *w, void *dx,
+                                void *dk, hipStream_t s) {
+    if (d.nrows <= 0) return hipSuccess;
+    if (prec == PREC_F64)
+        hipLaunchKernelGGL(k_real_conv_vjp<double>, dim3((unsigned)d.nrows), dim3(d.nthreads), d.lds_bytes, s, d,
+                           (const double *)x, (const double *)k, (const double *)w, (double *)dx, (double *)dk);
+    else
+        hipLaunchKernelGGL(k_real_conv_vjp<float>, dim3((unsigned)d.nrows), dim3(d.nthreads), d.lds_bytes, s, d,
+                           (const float *)x, (const float *)k, (const float *)w, (float *)dx, (float *)dk);
+    return hipGetLastError();
+}
+
+hipError_t gen_twiddles(int prec, void *out, int64_t N, int64_t count, int64_t mul, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    if (prec == PREC_F64)
+        hipLaunchKernelGGL(k_gen_twiddles<double2>, dim3(grid_for(count, 256)), dim3(256), 0, s, (double2 *)out, N,
+                           count, mul);
+    else
+        hipLaunchKernelGGL(k_gen_twiddles<float4>, dim3(grid_for(count, 256)), dim3(256), 0, s, (float4 *)out, N,
+                           count, mul);
+    return hipGetLastError();
+}
+
+hipError_t gen_twiddles_d(void *out, int64_t N, int64_t count, int64_t mul, hipStream_t s) {
+    return gen_twiddles(PREC_F64, out, N, count, mul, s);
+}
+
+hipError_t gen_chirp_d(void *out, int64_t n, int64_t count, int sign, int64_t M, int conj, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gen_chirp, dim3(grid_for(count, 256)), dim3(256), 0, s, (double2 *)out, n, count, sign, M,
+                       conj);
+    return hipGetLastError();
+}
+
+hipError_t gen_rader_kernel_d(void *out, const int32_t *powg, int64_t p, int64_t count, int sign, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gen_rader_kernel, dim3(grid_for(count, 256)), dim3(256), 0, s, (double2 *)out, powg, p,
+                       count, sign);
+    return hipGetLastError();
+}
+
+hipError_t convert_to_w(int prec, void *out, const void *in, int64_t count, double scale, int64_t N1, int64_t N2,
+                        hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    if (prec == PREC_F64)
+        hipLaunchKernelGGL(k_convert<double2>, dim3(grid_for(count, 256)), dim3(256), 0, s, (double2 *)out,
+                           (const double2 *)in, count, scale, N1, N2);
+    else
+        hipLaunchKernelGGL(k_convert<float4>, dim3(grid_for(count, 256)), dim3(256), 0, s, (float4 *)out,
+                           (const double2 *)in, count, scale, N1, N2);
+    return hipGetLastError();
+}
+
+}  // namespace af

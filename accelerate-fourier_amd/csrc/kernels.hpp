@@ -1,0 +1,31 @@
+// Host-side declarations of the kernel launchers (internal to the library).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "af_pass.hpp"
+
+namespace af {
+
+enum { PREC_F64 = 0, PREC_F32 = 1 };
+
+hipError_t launch_pass(int prec, const AfPass &d, hipStream_t s);
+hipError_t launch_pass_grouped(int prec, const AfPass *descs, const int64_t *prefix, int ndesc,
+                               int64_t total_tiles, int nthreads, int lds_bytes, hipStream_t s);
+hipError_t launch_real_conv(int prec, const AfPass &d, const void *x, const void *k, void *y, hipStream_t s);
+hipError_t launch_real_conv_vjp(int prec, const AfPass &d, const void *x, const void *k, const void *w,
+                                void *dx, void *dk, hipStream_t s);
+
+// Twiddle tables W_N^(i*mul), i < count, forward sign, in the table format of prec.
+hipError_t gen_twiddles(int prec, void *out, int64_t N, int64_t count, int64_t mul, hipStream_t s);
+hipError_t gen_twiddles_d(void *out, int64_t N, int64_t count, int64_t mul, hipStream_t s);
+hipError_t gen_chirp_d(void *out, int64_t n, int64_t count, int sign, int64_t M, int conj, hipStream_t s);
+hipError_t gen_rader_kernel_d(void *out, const int32_t *powg, int64_t p, int64_t count, int sign, hipStream_t s);
+hipError_t convert_to_w(int prec, void *out, const void *in, int64_t count, double scale, int64_t N1, int64_t N2,
+                        hipStream_t s);
+
+// Karatsuba (karatsuba.hip)
+hipError_t launch_karatsuba(const int64_t *x, const int64_t *y, int64_t *z, const int64_t *dev_meta, int64_t npairs,
+                            int max_P, int cyclic, hipStream_t s);
+
+}  // namespace af

@@ -1,0 +1,69 @@
+// Host planner of the MI355X transform path (no HIP calls in here).
+//
+// The reference plans on the host from a Haskell Int length (Planned.plan,
+// Planned.hs:201-293): even lengths by split-radix, odd lengths by Good-Thomas
+// over the first coprime Fermat factor pair, primes by Rader and, when Rader
+// nests three deep, by Bluestein on ceiling5Smooth(2n-1).  The device plan
+// computes the same DFT with the structure that suits gfx950: any length whose
+// prime factors are radices of the register butterflies (2, 3, 5, 7, 11, 13)
+// runs as a mixed-radix Stockham sequence in LDS (one pass up to the LDS
+// limit, two passes of the four-step factorisation N = N1 N2 beyond it);
+// primes p with smooth p-1 run Rader's convolution of length p-1; everything
+// else runs Bluestein with a smooth padded length.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace af {
+
+// NumberTheory.hs:94-101
+int64_t ceil_pow2(int64_t n);
+// NumberTheory.hs:103-124
+int64_t ceil_5smooth(int64_t n);
+// smallest m >= n whose prime factors are all in {2,3,5,7} (Bluestein pad)
+int64_t ceil_7smooth(int64_t n);
+bool is_prime(int64_t n);
+// NumberTheory.hs:32-47: the smallest primitive root modulo a prime
+int64_t multiplicative_generator(int64_t p);
+int64_t mod_pow(int64_t base, int64_t expo, int64_t m);
+
+// Radices of the register butterflies, or false when n has another prime factor.
+bool stockham_radices(int64_t n, std::vector<int> &radices);
+
+// Largest single-pass length and per-tile element budget for a precision.
+int64_t lds_max_len(int prec);
+int lds_tile_bytes_cap();
+int elem_bytes(int prec);
+
+enum CoreKind { CORE_NONE = 0, CORE_SINGLE = 1, CORE_FOUR = 2 };
+struct CoreShape {
+    int kind = CORE_NONE;
+    int64_t N = 0;
+    std::vector<int> r;        // single pass radices
+    int64_t N1 = 0, N2 = 0;    // four-step factors
+    std::vector<int> r1, r2;
+};
+// Shape of an FFT of length N by passes (single or four-step); CORE_NONE if N
+// is not smooth or does not factor within the LDS limit.
+CoreShape core_shape(int prec, int64_t N);
+
+enum PlanAlgo {
+    ALG_EMPTY = 0,     // n = 0: nothing to do
+    ALG_COPY = 1,      // n = 1: identity
+    ALG_STOCKHAM = 2,  // smooth n
+    ALG_RADER = 3,     // prime n, Rader convolution of length n-1
+    ALG_BLUESTEIN = 4  // chirp-z with padded length M
+};
+struct PlanShape {
+    int algo = ALG_EMPTY;
+    int64_t n = 0;
+    int64_t M = 0;     // Bluestein padded length
+    CoreShape core;    // the FFT that runs on the device
+};
+// algorithm: af_algorithm (auto, decompose, chirp2, chirp235)
+bool plan_shape(int prec, int64_t n, int algorithm, PlanShape &out, std::string &why);
+std::string describe_shape(const PlanShape &s);
+
+}  // namespace af

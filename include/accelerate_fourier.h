@@ -1,0 +1,219 @@
+/*
+ * accelerate-fourier on MI355X (gfx950): the C ABI.
+ *
+ * This is the drop-in boundary for the transform path of the Haskell library
+ * accelerate-fourier.  Each entry point replaces one pure `Acc -> Acc`
+ * function of the reference; a Haskell binding calls it through `foreignAcc`
+ * and a `foreign import ccall` (see INTEGRATION.md for the binding source).
+ *
+ * Conventions
+ *  - Plain C types only; device buffers are raw pointers the caller owns.
+ *  - Complex arrays are planar by default, as Accelerate (< 1.3) stores
+ *    `Array sh (Complex a)`: one array of real parts and one of imaginary
+ *    parts.  Passing `im == NULL` selects the interleaved layout (re, im pairs)
+ *    for the same entry point.
+ *  - Arrays are row-major with the transform axis innermost and contiguous;
+ *    the leading `sh` extents are flattened into `batch`.
+ *  - `stream` is a hipStream_t (NULL = the default stream); calls are
+ *    asynchronous with respect to the host.
+ *  - Every precondition that the reference documents but does not check
+ *    returns AF_INVALID_ARG instead of computing garbage.  Nothing throws.
+ *  - Plans are immutable after creation: concurrent af_execute calls on one
+ *    plan from several host threads or streams are allowed.
+ */
+#ifndef ACCELERATE_FOURIER_H
+#define ACCELERATE_FOURIER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    AF_SUCCESS = 0,
+    AF_INVALID_ARG = 1,   /* a documented precondition of the reference is violated */
+    AF_UNSUPPORTED = 2,   /* a valid request this build cannot serve */
+    AF_NO_DEVICE = 3,     /* no gfx950 device is visible */
+    AF_HIP_ERROR = 4,     /* a HIP runtime call failed */
+    AF_OUT_OF_MEMORY = 5,
+    AF_INTERNAL = 6,
+    AF_TIMEOUT = 7        /* a bounded device-side wait gave up (persistent batch) */
+} af_status;
+
+typedef enum { AF_F64 = 0, AF_F32 = 1 } af_precision;
+
+/* Which reference function a plan stands for. */
+typedef enum {
+    AF_C2C = 0,          /* Planned.transform sign n                         (Planned.hs:103-106) */
+    AF_R2C_FULL = 1,     /* Real.toSpectrum (Planned.transform sign (n/2))   (Real.hs:49-70)      */
+    AF_C2R_FULL = 2,     /* Real.fromSpectrum (Planned.transform sign (n/2)) (Real.hs:92-107)     */
+    AF_CONV_CYCLIC = 3,  /* Planned.convolveCyclic n                         (Planned.hs:779-792) */
+    AF_REAL_CONV = 4     /* fromSpectrum t' (toSpectrum t x * toSpectrum t k), t/t' = forward/inverse
+                            transform (n/2): the real FFT convolution (Real.hs:49-107 composed) */
+} af_kind;
+
+/* Algorithm selection, mirroring the reference's entry points. */
+typedef enum {
+    AF_ALG_AUTO = 0,       /* Planned.transform: decompose, Bluestein when Rader nests (Planned.hs:215-220) */
+    AF_ALG_DECOMPOSE = 1,  /* Planned.transformDecompose: never Bluestein       (Planned.hs:114-119) */
+    AF_ALG_CHIRP2 = 2,     /* Planned.transformChirp2: Bluestein, power-of-two pad (Planned.hs:704-708) */
+    AF_ALG_CHIRP235 = 3    /* Planned.transformChirp235: Bluestein, 5-smooth pad   (Planned.hs:716-720) */
+} af_algorithm;
+
+typedef struct {
+    int64_t length;      /* n: the inner extent (for real kinds: the real length, even) */
+    int32_t sign;        /* -1 = Sign.forward, +1 = Sign.inverse (Sign.hs:52-54); for the
+                            real kinds: the sign of the half-length sub-transform */
+    int32_t kind;        /* af_kind */
+    int32_t precision;   /* af_precision */
+    int32_t algorithm;   /* af_algorithm */
+    uint32_t flags;      /* AF_FLAG_* */
+} af_plan_desc;
+
+/* FP32 stage twiddles and vectors as compensated hi+lo float pairs (default on). */
+#define AF_FLAG_PLAIN_TWIDDLES 0x1u   /* use hi parts only (for error studies) */
+
+typedef struct af_plan_s *af_plan;
+typedef struct af_batch_s *af_batch;
+
+const char *af_status_string(af_status s);
+const char *af_version(void);
+
+/*
+ * Plan creation: host-side planning (Planned.plan, Planned.hs:201-220) plus
+ * device tables generated on the GPU in FP64 (twiddles, chirps, Rader and
+ * Bluestein kernel spectra).  Replaces the reference's `cache sign len`
+ * (Planned.hs:322-328), which is what partial application of
+ * `transform sign len` shares between calls.
+ */
+af_status af_plan_create(const af_plan_desc *desc, af_plan *plan);
+af_status af_plan_destroy(af_plan plan);
+/* Text of the device plan (pass structure); host-only. */
+af_status af_plan_describe(const af_plan_desc *desc, char *buf, size_t buflen);
+/* Bytes of device scratch one call with `batch` rows needs (0 for most plans). */
+af_status af_plan_workspace_size(af_plan plan, int64_t batch, size_t *bytes);
+
+/*
+ * Complex transform of `batch` rows of length n (AF_C2C plans).
+ * Replaces: Planned.transform sign n :: Transform (sh:.Int) (Complex a)
+ *           (Planned.hs:103-106).  Unnormalised, as the reference.
+ * in_im == NULL (and then out_im == NULL) selects interleaved complex.
+ * In-place (in == out) is allowed.  workspace may be NULL (stream-ordered
+ * allocation inside the call) or a buffer of af_plan_workspace_size bytes.
+ */
+af_status af_execute(af_plan plan, int64_t batch,
+                     const void *in_re, const void *in_im,
+                     void *out_re, void *out_im,
+                     void *workspace, void *stream);
+
+/*
+ * Real signal to full spectrum (AF_R2C_FULL plans).
+ * Replaces: Real.toSpectrum (Planned.transform sign (n/2)) (Real.hs:49-70).
+ * x: batch x n reals; out: batch x n complex (the full, not halved, spectrum).
+ */
+af_status af_to_spectrum(af_plan plan, int64_t batch, const void *x,
+                         void *out_re, void *out_im, void *workspace, void *stream);
+
+/*
+ * Full self-adjoint spectrum to real signal (AF_C2R_FULL plans), unnormalised
+ * (n times the signal).  Replaces: Real.fromSpectrum (Planned.transform sign
+ * (n/2)) (Real.hs:92-107).
+ */
+af_status af_from_spectrum(af_plan plan, int64_t batch, const void *in_re, const void *in_im,
+                           void *y, void *workspace, void *stream);
+
+/*
+ * Cyclic convolution of complex rows (AF_CONV_CYCLIC plans), normalised.
+ * Replaces: Planned.convolveCyclic n (Planned.hs:779-792).  n >= 1.
+ */
+af_status af_convolve_cyclic(af_plan plan, int64_t batch,
+                             const void *x_re, const void *x_im,
+                             const void *y_re, const void *y_im,
+                             void *out_re, void *out_im, void *workspace, void *stream);
+
+/*
+ * Real FFT convolution (AF_REAL_CONV plans):
+ *   y = fromSpectrum (transform inverse (n/2)) (toSpectrum (transform forward (n/2)) x
+ *                                               * toSpectrum (transform forward (n/2)) k)
+ * = n * (x cyclic-convolved with k), real, batch x n.  Composes Real.hs:49-107
+ * with the pointwise product of convolveSpectrumCyclicCache (Planned.hs:816-821).
+ */
+af_status af_real_convolve(af_plan plan, int64_t batch, const void *x, const void *k,
+                           void *y, void *stream);
+
+/* ---- Vector-Jacobian products (no reference counterpart: the adjoints of the
+ * functions above, for callers that differentiate through them) ---- */
+
+/* Adjoint of af_execute: the conjugate-transposed DFT, i.e. the transform with
+ * the opposite sign applied to the cotangent (gradient convention dL/dconj(z)). */
+af_status af_vjp_execute(af_plan plan, int64_t batch,
+                         const void *g_re, const void *g_im,
+                         void *out_re, void *out_im, void *workspace, void *stream);
+/* Adjoint of af_to_spectrum: real gradient from a complex cotangent
+ * (the transpose of the real-FFT untangling). */
+af_status af_vjp_to_spectrum(af_plan plan, int64_t batch, const void *g_re, const void *g_im,
+                             void *dx, void *workspace, void *stream);
+/* Adjoint of af_from_spectrum: complex cotangent of the spectrum from a real
+ * cotangent (the transpose of the entangling). */
+af_status af_vjp_from_spectrum(af_plan plan, int64_t batch, const void *g,
+                               void *out_re, void *out_im, void *workspace, void *stream);
+/* Adjoint of af_real_convolve with cotangent w: dx = n (w correlated with k),
+ * dk = n (w correlated with x), both in one fused launch. */
+af_status af_vjp_real_convolve(af_plan plan, int64_t batch, const void *x, const void *k,
+                               const void *w, void *dx, void *dk, void *stream);
+
+/* ---- Karatsuba (Int64, bit-exact) ---- */
+
+/*
+ * Linear convolution of `batch` pairs of equal-length Int64 rows, wrap-around
+ * arithmetic.  x, y: batch x len; z: batch x (2 len - 1) (nothing for len = 0).
+ * Replaces: Convolution.Adhoc.karatsuba (Convolution/Adhoc.hs:36-64).
+ */
+af_status af_karatsuba_i64(int64_t batch, int64_t len, const int64_t *x, const int64_t *y,
+                           int64_t *z, void *stream);
+/*
+ * Ragged batch of pairs: pair p has length offsets[p+1]-offsets[p] >= 1 and
+ * its inputs start at offsets[p]; its 2L-1 outputs start at 2*offsets[p]-p.
+ * `offsets` is a host array of npairs+1 entries.  Same function as above.
+ */
+af_status af_karatsuba_ragged_i64(int64_t npairs, const int64_t *offsets,
+                                  const int64_t *x, const int64_t *y, int64_t *z, void *stream);
+/* Convolution.Adhoc.cyclic karatsuba (Convolution/Adhoc.hs:104-111): batch x len outputs. */
+af_status af_karatsuba_cyclic_i64(int64_t batch, int64_t len, const int64_t *x, const int64_t *y,
+                                  int64_t *z, void *stream);
+
+/* ---- Uneven batches: many lengths in one graph launch ---- */
+
+/*
+ * A batch of `count` complex transforms of lengths lengths[i] >= 0, all with
+ * one sign and precision, packed back to back (row i starts at the prefix sum
+ * of the lengths).  Equivalent to Planned.transform sign lengths[i] on each
+ * row.  The work of all rows runs as one captured hipGraph.
+ */
+af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign,
+                          int32_t precision, af_batch *batch);
+af_status af_batch_destroy(af_batch batch);
+af_status af_batch_execute(af_batch batch, const void *in_re, const void *in_im,
+                           void *out_re, void *out_im, void *stream);
+af_status af_batch_describe(af_batch batch, char *buf, size_t buflen);
+
+/* ---- Host-side index maps, for bit-exact checks against the reference ---- */
+
+/* Rader permutations for prime p (Permutation.hs:45-80): perm[i] = g^i mod p - 1,
+ * rev = Permutation.reverse perm, inv = Permutation.inverse perm; each p-1 long. */
+af_status af_rader_permutations(int64_t p, int64_t *perm, int64_t *rev, int64_t *inv);
+/* Good-Thomas maps of the device kernels for n*m with gcd(n,m) = 1:
+ * in_map[k*n + j] = (n*k + m*j) mod nm (permuteSkewGrid m n, Planned.hs:597-606),
+ * out_map[q] = (q mod n)*m + (q mod m) (permuteSkewGridInv, Planned.hs:608-617). */
+af_status af_good_thomas_maps(int64_t n, int64_t m, int64_t *in_map, int64_t *out_map);
+/* NumberTheory.multiplicativeGenerator / ceiling5Smooth / ceilingPowerOfTwo */
+int64_t af_multiplicative_generator(int64_t p);
+int64_t af_ceiling_5smooth(int64_t n);
+int64_t af_ceiling_power_of_two(int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
