@@ -1,0 +1,53 @@
+"""Uneven batches: many transforms of different lengths in one graph launch.
+
+``UnevenBatch(lengths, sign, precision)`` applies ``Planned.transform sign
+lengths[i]`` (Planned.hs:103-106) to each of ``len(lengths)`` rows packed back
+to back in one flat complex array.  All rows run as grouped launches of the
+generic pass kernel captured in one hipGraph.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _abi, _arrays
+from .sign import Sign
+
+
+class UnevenBatch:
+    def __init__(self, lengths, sign: Sign, precision: str = "f32"):
+        if not isinstance(sign, Sign):
+            sign = Sign(int(sign))
+        self.lengths = [int(n) for n in lengths]
+        self.total = sum(self.lengths)
+        self.prec = _abi.F64 if precision == "f64" else _abi.F32
+        arr = _abi.int64_array(self.lengths)
+        h = ctypes.c_void_p()
+        _abi.check(_abi.lib().af_batch_create(len(self.lengths), ctypes.cast(arr, ctypes.POINTER(ctypes.c_int64)),
+                                              sign.value, self.prec, ctypes.byref(h)), "af_batch_create")
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _abi._lib is not None:
+            _abi.lib().af_batch_destroy(h)
+            self.handle = None
+
+    def describe(self) -> str:
+        buf = ctypes.create_string_buffer(4096)
+        _abi.check(_abi.lib().af_batch_describe(self.handle, buf, len(buf)), "af_batch_describe")
+        return buf.value.decode()
+
+    def __call__(self, x, out=None):
+        re, im, shape = _arrays.complex_view(x)
+        if len(shape) != 1 or shape[0] != self.total:
+            raise ValueError(f"expected a flat array of {self.total} elements")
+        if _arrays.precision_of_complex(x) != self.prec:
+            raise TypeError("precision differs from the batch's")
+        if out is None:
+            out = _arrays.empty_like_complex(x)
+        ore, oim = _arrays.ptrs(out)
+        _abi.check(_abi.lib().af_batch_execute(self.handle, re, im, ore, oim, _arrays.stream_handle()),
+                   "af_batch_execute")
+        return out

@@ -1,0 +1,70 @@
+"""Karatsuba convolution (Data.Array.Accelerate.Convolution.Adhoc).
+
+``karatsuba(x, y)`` is ``karatsuba :: Transform2 (sh :. Int) a`` for Int64
+(Convolution/Adhoc.hs:36-64): equal-length rows, result length 2L-1 (0 for
+L = 0), wrap-around Int64 arithmetic, bit-exact on the GPU.
+``cyclic(karatsuba)`` is ``Convolution.cyclic karatsuba`` (Adhoc.hs:104-111).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _abi, _arrays
+
+
+def _check_pair(x, y):
+    for t, nm in ((x, "x"), (y, "y")):
+        _arrays._check_tensor(t, nm)
+        if t.dtype != torch.int64:
+            raise TypeError(f"{nm} must be int64")
+    if x.shape != y.shape:
+        # Adhoc.hs:33-34: both arrays must have the same size
+        raise ValueError("Karatsuba operands must have the same shape")
+
+
+def karatsuba(x, y, out=None):
+    _check_pair(x, y)
+    L = x.shape[-1]
+    batch = _arrays.batch_of(tuple(x.shape))
+    zl = 2 * L - 1 if L > 0 else 0
+    if out is None:
+        out = torch.empty(tuple(x.shape[:-1]) + (zl,), dtype=torch.int64, device=x.device)
+    _abi.check(_abi.lib().af_karatsuba_i64(batch, L, x.data_ptr(), y.data_ptr(), out.data_ptr(),
+                                           _arrays.stream_handle()), "af_karatsuba_i64")
+    return out
+
+
+def karatsuba_ragged(offsets, x, y, out=None):
+    """Pairs of uneven lengths packed back to back; pair p occupies
+    offsets[p]:offsets[p+1] of x and y and 2*offsets[p]-p : +2L-1 of the result."""
+    offsets = [int(o) for o in offsets]
+    npairs = len(offsets) - 1
+    for t, nm in ((x, "x"), (y, "y")):
+        _arrays._check_tensor(t, nm)
+        if t.dtype != torch.int64 or t.dim() != 1:
+            raise TypeError(f"{nm} must be a flat int64 tensor")
+    if out is None:
+        out = torch.empty(2 * offsets[-1] - npairs, dtype=torch.int64, device=x.device)
+    arr = _abi.int64_array(offsets)
+    _abi.check(_abi.lib().af_karatsuba_ragged_i64(npairs, ctypes.cast(arr, ctypes.POINTER(ctypes.c_int64)),
+                                                  x.data_ptr(), y.data_ptr(), out.data_ptr(),
+                                                  _arrays.stream_handle()), "af_karatsuba_ragged_i64")
+    return out
+
+
+def cyclic(conv):
+    """Convolution.cyclic: wrap the tail of a linear convolution (Adhoc.hs:104-111)."""
+    if conv is karatsuba:
+        def cyc(x, y, out=None):
+            _check_pair(x, y)
+            L = x.shape[-1]
+            if out is None:
+                out = torch.empty_like(x)
+            _abi.check(_abi.lib().af_karatsuba_cyclic_i64(_arrays.batch_of(tuple(x.shape)), L, x.data_ptr(),
+                                                          y.data_ptr(), out.data_ptr(), _arrays.stream_handle()),
+                       "af_karatsuba_cyclic_i64")
+            return out
+        return cyc
+    raise NotImplementedError("cyclic wraps the GPU Karatsuba only")

@@ -1,0 +1,152 @@
+"""Planned transforms (Data.Array.Accelerate.Fourier.Planned).
+
+``transform(sign, n)`` is the reference's ``transform :: Sign a -> Int ->
+Transform (sh:.Int) (Complex a)`` (Planned.hs:103-106): a function over arrays
+whose inner extent is n, unnormalised, sharing its plan (twiddles, chirps,
+Rader kernels) between calls as the partially applied Haskell function shares
+its cache.  The work runs in the gfx950 kernels behind the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+
+from . import _abi, _arrays
+from .sign import Sign
+
+_plan_cache: dict = {}
+_plan_lock = threading.Lock()
+
+
+class Plan:
+    """Owns one af_plan."""
+
+    def __init__(self, length, sign, kind, precision, algorithm=_abi.ALG_AUTO, flags=0):
+        self.desc = _abi.PlanDesc(length, sign, kind, precision, algorithm, flags)
+        h = ctypes.c_void_p()
+        _abi.check(_abi.lib().af_plan_create(ctypes.byref(self.desc), ctypes.byref(h)),
+                   f"af_plan_create(n={length}, sign={sign}, kind={kind})")
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _abi._lib is not None:
+            _abi.lib().af_plan_destroy(h)
+            self.handle = None
+
+    def workspace_size(self, batch):
+        n = ctypes.c_size_t()
+        _abi.check(_abi.lib().af_plan_workspace_size(self.handle, batch, ctypes.byref(n)), "workspace")
+        return n.value
+
+
+def get_plan(length, sign, kind, precision, algorithm=_abi.ALG_AUTO, flags=0) -> Plan:
+    key = (length, sign, kind, precision, algorithm, flags)
+    with _plan_lock:
+        p = _plan_cache.get(key)
+        if p is None:
+            p = Plan(length, sign, kind, precision, algorithm, flags)
+            _plan_cache[key] = p
+        return p
+
+
+def describe(length, sign=-1, kind=_abi.C2C, precision=_abi.F32, algorithm=_abi.ALG_AUTO) -> str:
+    """Host-only text of the device plan for a length."""
+    d = _abi.PlanDesc(length, sign, kind, precision, algorithm, 0)
+    buf = ctypes.create_string_buffer(4096)
+    st = _abi.lib().af_plan_describe(ctypes.byref(d), buf, len(buf))
+    if st not in (_abi.SUCCESS, _abi.UNSUPPORTED):
+        raise _abi.AfError(st, "af_plan_describe")
+    return buf.value.decode()
+
+
+class Transform:
+    """``Planned.transform sign n`` as a callable over GPU arrays."""
+
+    def __init__(self, sign: Sign, n: int, algorithm: int = _abi.ALG_AUTO, plain_twiddles: bool = False):
+        if not isinstance(sign, Sign):
+            sign = Sign(int(sign))
+        if n < 0:
+            raise ValueError("length must be non-negative")
+        self.sign = sign
+        self.n = int(n)
+        self.algorithm = algorithm
+        self.flags = _abi.FLAG_PLAIN_TWIDDLES if plain_twiddles else 0
+
+    def plan(self, precision) -> Plan:
+        return get_plan(self.n, self.sign.value, _abi.C2C, precision, self.algorithm, self.flags)
+
+    def __call__(self, x, out=None):
+        re, im, shape = _arrays.complex_view(x)
+        if not shape or shape[-1] != self.n:
+            # Planned.hs:98-101: the length must equal the inner extent
+            raise ValueError(f"inner extent {shape[-1] if shape else None} differs from the planned length {self.n}")
+        prec = _arrays.precision_of_complex(x)
+        if out is None:
+            out = _arrays.empty_like_complex(x)
+        ore, oim = _arrays.ptrs(out)
+        batch = _arrays.batch_of(shape)
+        p = self.plan(prec)
+        _abi.check(_abi.lib().af_execute(p.handle, batch, re, im, ore, oim, None, _arrays.stream_handle()),
+                   "af_execute")
+        return out
+
+    def vjp(self, g, out=None):
+        """Adjoint (conjugate transpose) applied to the cotangent g."""
+        re, im, shape = _arrays.complex_view(g, "cotangent")
+        if shape[-1] != self.n:
+            raise ValueError("cotangent has the wrong inner extent")
+        prec = _arrays.precision_of_complex(g)
+        if out is None:
+            out = _arrays.empty_like_complex(g)
+        ore, oim = _arrays.ptrs(out)
+        p = self.plan(prec)
+        _abi.check(_abi.lib().af_vjp_execute(p.handle, _arrays.batch_of(shape), re, im, ore, oim, None,
+                                             _arrays.stream_handle()), "af_vjp_execute")
+        return out
+
+
+def transform(sign: Sign, n: int) -> Transform:
+    """Planned.transform (Planned.hs:103-106)."""
+    return Transform(sign, n)
+
+
+def transform_decompose(sign: Sign, n: int) -> Transform:
+    """Planned.transformDecompose (Planned.hs:114-119): no Bluestein."""
+    return Transform(sign, n, _abi.ALG_DECOMPOSE)
+
+
+def transform_chirp2(sign: Sign, n: int) -> Transform:
+    """Planned.transformChirp2 (Planned.hs:704-708): Bluestein, power-of-two pad."""
+    return Transform(sign, n, _abi.ALG_CHIRP2)
+
+
+def transform_chirp235(sign: Sign, n: int) -> Transform:
+    """Planned.transformChirp235 (Planned.hs:716-720): Bluestein, 5-smooth pad."""
+    return Transform(sign, n, _abi.ALG_CHIRP235)
+
+
+def convolve_cyclic(n: int):
+    """Planned.convolveCyclic (Planned.hs:779-792): normalised cyclic convolution.
+
+    Signals must have equal size and must not be empty."""
+    if n < 1:
+        raise ValueError("convolveCyclic needs a non-empty length")
+
+    def conv(x, y, out=None):
+        xr, xi, xs = _arrays.complex_view(x, "x")
+        yr, yi, ys = _arrays.complex_view(y, "y")
+        if xs != ys or xs[-1] != n:
+            raise ValueError("operands must have equal shape with inner extent n")
+        prec = _arrays.precision_of_complex(x)
+        if _arrays.precision_of_complex(y) != prec:
+            raise TypeError("operands must have the same precision")
+        if out is None:
+            out = _arrays.empty_like_complex(x)
+        ore, oim = _arrays.ptrs(out)
+        p = get_plan(n, -1, _abi.CONV_CYCLIC, prec)
+        _abi.check(_abi.lib().af_convolve_cyclic(p.handle, _arrays.batch_of(xs), xr, xi, yr, yi, ore, oim, None,
+                                                 _arrays.stream_handle()), "af_convolve_cyclic")
+        return out
+
+    return conv

@@ -1,0 +1,126 @@
+"""Real-signal transforms (Data.Array.Accelerate.Fourier.Real).
+
+``to_spectrum(sub, x)`` is ``toSpectrum :: Transform (sh:.Int) (Complex a) ->
+Acc (Array (sh:.Int) a) -> Acc (Array (sh:.Int) (Complex a))`` (Real.hs:49-70)
+with ``sub`` the half-length complex transform; the result is the full (not
+halved) spectrum.  ``from_spectrum(sub, s)`` is ``fromSpectrum`` (Real.hs:92-107),
+unnormalised.  Both run as one fused gfx950 pass: the half-length FFT with the
+(de)interleaving and the (un)tangling folded into its loads and stores.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _abi, _arrays
+from .planned import Transform, get_plan
+
+
+def _real_prec(x):
+    if x.dtype == torch.float64:
+        return _abi.F64
+    if x.dtype == torch.float32:
+        return _abi.F32
+    raise TypeError("real arrays are float64 or float32")
+
+
+def _check_real(x, what):
+    _arrays._check_tensor(x, what)
+    if x.is_complex():
+        raise TypeError(f"{what} must be real")
+
+
+def to_spectrum(sub: Transform, x, out=None, planar=False):
+    _check_real(x, "x")
+    n = x.shape[-1]
+    if n % 2:
+        raise ValueError("toSpectrum needs an even length (Real.hs:45-47)")
+    if sub.n != n // 2:
+        raise ValueError("the sub-transform must have half the signal length")
+    prec = _real_prec(x)
+    if out is None:
+        if planar:
+            out = (torch.empty_like(x), torch.empty_like(x))
+        else:
+            out = torch.empty(x.shape, dtype=_arrays.complex_dtype(prec), device=x.device)
+    ore, oim = _arrays.ptrs(out)
+    p = get_plan(n, sub.sign.value, _abi.R2C_FULL, prec, flags=sub.flags)
+    _abi.check(_abi.lib().af_to_spectrum(p.handle, _arrays.batch_of(tuple(x.shape)), x.data_ptr(), ore, oim, None,
+                                         _arrays.stream_handle()), "af_to_spectrum")
+    return out
+
+
+def from_spectrum(sub: Transform, spec, out=None):
+    re, im, shape = _arrays.complex_view(spec, "spectrum")
+    n = shape[-1]
+    if n % 2:
+        raise ValueError("fromSpectrum needs an even length (Real.hs:88-90)")
+    if sub.n != n // 2:
+        raise ValueError("the sub-transform must have half the signal length")
+    prec = _arrays.precision_of_complex(spec)
+    if out is None:
+        dev = spec[0].device if isinstance(spec, tuple) else spec.device
+        out = torch.empty(shape, dtype=_arrays.real_dtype(prec), device=dev)
+    p = get_plan(n, sub.sign.value, _abi.C2R_FULL, prec, flags=sub.flags)
+    _abi.check(_abi.lib().af_from_spectrum(p.handle, _arrays.batch_of(shape), re, im, out.data_ptr(), None,
+                                           _arrays.stream_handle()), "af_from_spectrum")
+    return out
+
+
+def vjp_to_spectrum(g, out=None):
+    """Adjoint of to_spectrum(transform(forward, n/2), .) applied to a complex cotangent."""
+    re, im, shape = _arrays.complex_view(g, "cotangent")
+    n = shape[-1]
+    prec = _arrays.precision_of_complex(g)
+    if out is None:
+        dev = g[0].device if isinstance(g, tuple) else g.device
+        out = torch.empty(shape, dtype=_arrays.real_dtype(prec), device=dev)
+    p = get_plan(n, -1, _abi.R2C_FULL, prec)
+    _abi.check(_abi.lib().af_vjp_to_spectrum(p.handle, _arrays.batch_of(shape), re, im, out.data_ptr(), None,
+                                             _arrays.stream_handle()), "af_vjp_to_spectrum")
+    return out
+
+
+def vjp_from_spectrum(g, out=None):
+    """Adjoint of from_spectrum(transform(inverse, n/2), .) applied to a real cotangent."""
+    _check_real(g, "cotangent")
+    n = g.shape[-1]
+    prec = _real_prec(g)
+    if out is None:
+        out = torch.empty(g.shape, dtype=_arrays.complex_dtype(prec), device=g.device)
+    ore, oim = _arrays.ptrs(out)
+    p = get_plan(n, 1, _abi.C2R_FULL, prec)
+    _abi.check(_abi.lib().af_vjp_from_spectrum(p.handle, _arrays.batch_of(tuple(g.shape)), g.data_ptr(), ore, oim,
+                                               None, _arrays.stream_handle()), "af_vjp_from_spectrum")
+    return out
+
+
+def convolve(x, k, out=None):
+    """fromSpectrum (transform inverse (n/2)) (toSpectrum (transform forward (n/2)) x *
+    toSpectrum (transform forward (n/2)) k): n times the cyclic convolution, one fused launch."""
+    _check_real(x, "x")
+    _check_real(k, "k")
+    if x.shape != k.shape or x.dtype != k.dtype:
+        raise ValueError("signal and kernel must have equal shape and dtype")
+    n = x.shape[-1]
+    prec = _real_prec(x)
+    if out is None:
+        out = torch.empty_like(x)
+    p = get_plan(n, -1, _abi.REAL_CONV, prec)
+    _abi.check(_abi.lib().af_real_convolve(p.handle, _arrays.batch_of(tuple(x.shape)), x.data_ptr(), k.data_ptr(),
+                                           out.data_ptr(), _arrays.stream_handle()), "af_real_convolve")
+    return out
+
+
+def convolve_vjp(x, k, w, dx=None, dk=None):
+    """Gradients of sum(w * convolve(x, k)) with respect to x and k (one fused launch)."""
+    for t, nm in ((x, "x"), (k, "k"), (w, "w")):
+        _check_real(t, nm)
+    n = x.shape[-1]
+    prec = _real_prec(x)
+    dx = torch.empty_like(x) if dx is None else dx
+    dk = torch.empty_like(k) if dk is None else dk
+    p = get_plan(n, -1, _abi.REAL_CONV, prec)
+    _abi.check(_abi.lib().af_vjp_real_convolve(p.handle, _arrays.batch_of(tuple(x.shape)), x.data_ptr(),
+                                               k.data_ptr(), w.data_ptr(), dx.data_ptr(), dk.data_ptr(),
+                                               _arrays.stream_handle()), "af_vjp_real_convolve")
+    return dx, dk

@@ -1,0 +1,56 @@
+"""Seeded synthetic signals: a splitmix64 stream per seed.
+
+Uniform doubles take the top 53 bits of each 64-bit draw; normals come from
+Box-Muller on pairs of uniforms.  The stream is counter based, so any slice of
+it can be generated independently and reproduced in C from the same seed.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64(seed: int, count: int, start: int = 0) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        i = np.arange(start + 1, start + count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + i * _GAMMA
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, count: int, lo: float = -1.0, hi: float = 1.0, start: int = 0) -> np.ndarray:
+    u = (splitmix64(seed, count, start) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return lo + (hi - lo) * u
+
+
+def normal(seed: int, count: int, start: int = 0) -> np.ndarray:
+    m = (count + 1) // 2
+    u = uniform(seed, 2 * m, 0.0, 1.0, 2 * start)
+    u1 = np.maximum(u[0::2], 1e-300)
+    u2 = u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = np.empty(2 * m)
+    z[0::2] = r * np.cos(2 * np.pi * u2)
+    z[1::2] = r * np.sin(2 * np.pi * u2)
+    return z[:count]
+
+
+def complex_uniform(seed: int, shape) -> np.ndarray:
+    """Real parts then imaginary parts, each uniform in [-1, 1)."""
+    n = int(np.prod(shape))
+    v = uniform(seed, 2 * n)
+    return (v[:n] + 1j * v[n:]).reshape(shape)
+
+
+def complex_normal(seed: int, shape) -> np.ndarray:
+    n = int(np.prod(shape))
+    v = normal(seed, 2 * n)
+    return (v[:n] + 1j * v[n:]).reshape(shape)
+
+
+def integers(seed: int, count: int, lo: int, hi: int) -> np.ndarray:
+    """Uniform integers in [lo, hi)."""
+    u = uniform(seed, count, 0.0, 1.0)
+    return (lo + np.floor(u * (hi - lo))).astype(np.int64)

@@ -1,0 +1,1233 @@
+// C ABI of the MI355X transform path (include/accelerate_fourier.h).
+//
+// Plans hold the host planner's decision (planner.hpp) and the device tables
+// generated in FP64 on the GPU at creation.  Execution turns a plan plus the
+// caller's pointers into one to three AfPass descriptors and launches them on
+// the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "accelerate_fourier.h"
+#include "kernels.hpp"
+#include "planner.hpp"
+
+using namespace af;
+
+namespace {
+
+af_status hip_status(hipError_t e) { return e == hipErrorOutOfMemory ? AF_OUT_OF_MEMORY : AF_HIP_ERROR; }
+
+#define HIPCHK(x)                                       \
+    do {                                                \
+        hipError_t e__ = (x);                           \
+        if (e__ != hipSuccess) return hip_status(e__);  \
+    } while (0)
+#define AFCHK(x)                                 \
+    do {                                         \
+        af_status s__ = (x);                     \
+        if (s__ != AF_SUCCESS) return s__;       \
+    } while (0)
+
+std::once_flag g_init_flag;
+af_status g_init_status = AF_SUCCESS;
+
+af_status device_init() {
+    std::call_once(g_init_flag, [] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            (void)hipGetLastError();
+            g_init_status = AF_NO_DEVICE;
+            return;
+        }
+        g_init_status = AF_SUCCESS;
+    });
+    if (g_init_status != AF_SUCCESS) return g_init_status;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return AF_UNSUPPORTED;
+    return AF_SUCCESS;
+}
+
+// Device buffer owned by a plan.
+struct DevBuf {
+    void *p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct Owner {
+    std::vector<std::unique_ptr<DevBuf>> bufs;
+    af_status alloc(void **out, size_t bytes) {
+        auto b = std::make_unique<DevBuf>();
+        if (bytes == 0) bytes = 16;
+        HIPCHK(hipMalloc(&b->p, bytes));
+        *out = b->p;
+        bufs.push_back(std::move(b));
+        return AF_SUCCESS;
+    }
+};
+
+int tw_bytes(int) { return 16; }  // float4 hi/lo or double2
+
+// Device tables of one FFT core.
+struct Core {
+    CoreShape sh;
+    void *tw = nullptr;                 // single: W_N^m, m < N
+    void *tw1 = nullptr, *tw2 = nullptr;  // four-step: W_N1, W_N2
+    void *ft_coarse = nullptr, *ft_fine = nullptr;  // double2, four-step twiddles W_N
+    int ftShift = 0;
+};
+
+af_status build_core(int prec, const CoreShape &sh, Core &c, Owner &own, hipStream_t s) {
+    c.sh = sh;
+    if (sh.kind == CORE_SINGLE) {
+        AFCHK(own.alloc(&c.tw, (size_t)sh.N * tw_bytes(prec)));
+        HIPCHK(gen_twiddles(prec, c.tw, sh.N, sh.N, 1, s));
+    } else if (sh.kind == CORE_FOUR) {
+        AFCHK(own.alloc(&c.tw1, (size_t)sh.N1 * tw_bytes(prec)));
+        AFCHK(own.alloc(&c.tw2, (size_t)sh.N2 * tw_bytes(prec)));
+        HIPCHK(gen_twiddles(prec, c.tw1, sh.N1, sh.N1, 1, s));
+        HIPCHK(gen_twiddles(prec, c.tw2, sh.N2, sh.N2, 1, s));
+        int shift = 0;
+        while (((int64_t)1 << (2 * shift)) < sh.N) ++shift;
+        c.ftShift = shift;
+        const int64_t T = (int64_t)1 << shift, nc = (sh.N + T - 1) / T;
+        AFCHK(own.alloc(&c.ft_coarse, (size_t)nc * 16));
+        AFCHK(own.alloc(&c.ft_fine, (size_t)T * 16));
+        HIPCHK(gen_twiddles_d(c.ft_coarse, sh.N, nc, T, s));
+        HIPCHK(gen_twiddles_d(c.ft_fine, sh.N, T, 1, s));
+    }
+    return AF_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Pass construction
+// ---------------------------------------------------------------------------
+
+int pidx_h(int i, int ps) { return ps ? i + (i >> ps) : i; }
+
+int lds_bytes_for(int prec, int L, int B, int ps) {
+    const int ld = L + (B > 1 ? 1 : 0);
+    return (pidx_h(B * ld - 1, ps) + 1) * elem_bytes(prec);
+}
+
+bool tile_fits(int prec, int L, int B) {
+    if ((int64_t)B * L > 16384) return false;  // 16 elements per thread, 1024 threads
+    return lds_bytes_for(prec, L, B, 4) <= lds_tile_bytes_cap();
+}
+
+// Lines per tile for passes over contiguous rows.
+int choose_B_rows(int prec, int64_t L, int64_t nlines_total) {
+    const int64_t target = prec == AF_F64 ? 2048 : 4096;
+    int B = 1;
+    while (B < 64 && (2 * B) * L <= target && tile_fits(prec, (int)L, 2 * B) &&
+           (nlines_total + 2 * B - 1) / (2 * B) >= 512)
+        B *= 2;
+    return B;
+}
+
+// Lines per tile for passes whose loads or stores are strided: 128-byte runs.
+int choose_B_cols(int prec, int64_t L) {
+    int B = prec == AF_F64 ? 8 : 16;
+    while (B > 1 && !tile_fits(prec, (int)L, B)) B /= 2;
+    return B;
+}
+
+struct Endpoint {
+    const void *re = nullptr;
+    const void *im = nullptr;  // nullptr: interleaved
+    int64_t bs = 0;            // row stride in elements
+};
+
+AfPass base_pass(int prec, int L, const std::vector<int> &radices, const void *tw, int B, int64_t nlines,
+                 int64_t nrows, int sign, bool plain) {
+    AfPass d;
+    std::memset(&d, 0, sizeof d);
+    d.L = L;
+    d.nst = (int)radices.size();
+    int p = 1;
+    for (int i = 0; i < d.nst; ++i) {
+        d.radix[i] = radices[i];
+        d.pst[i] = p;
+        p *= radices[i];
+    }
+    d.B = B;
+    d.ld = L + (B > 1 ? 1 : 0);
+    d.padshift = 4;
+    d.nlines = nlines;
+    d.nrows = nrows;
+    d.ntiles = (nrows * nlines + B - 1) / B;
+    int nt = (int)((((int64_t)B * L + 15) / 16 + 63) / 64 * 64);
+    if (nt < 64) nt = 64;
+    if (nt > 1024) nt = 1024;
+    d.nthreads = nt;
+    d.lds_bytes = lds_bytes_for(prec, L, B, d.padshift);
+    d.tw = tw;
+    d.sign = sign;
+    d.flags = plain ? AFP_PLAIN_TW : 0;
+    d.in_len = INT64_MAX;
+    d.out_len = INT64_MAX;
+    d.scale = 1.0;
+    return d;
+}
+
+void set_in(AfPass &d, const Endpoint &e, int64_t es, int64_t ls) {
+    d.in_re = e.re;
+    d.in_im = e.im;
+    d.in_bs = e.bs;
+    d.in_es = es;
+    d.in_ls = ls;
+}
+void set_out(AfPass &d, const Endpoint &e, int64_t es, int64_t ls) {
+    d.out_re = (void *)e.re;
+    d.out_im = (void *)e.im;
+    d.out_bs = e.bs;
+    d.out_es = es;
+    d.out_ls = ls;
+}
+
+void set_ft(AfPass &d, const Core &c, int sign) {
+    d.flags |= AFP_POSTTW;
+    d.ftN = c.sh.N;
+    d.ftShift = c.ftShift;
+    d.ftSign = sign;
+    d.ft_coarse = c.ft_coarse;
+    d.ft_fine = c.ft_fine;
+}
+
+// Plain transform of `nrows` rows (natural order in and out).  For the
+// four-step core, `row_order_out` leaves the spectrum in the row order
+// (k1 * N2 + k2 holds X[k1 + N1 k2]) used by the convolution passes.
+void c2c_passes(int prec, const Core &c, int64_t nrows, int sign, bool plain, const Endpoint &in,
+                const Endpoint &out, void *scratch, bool row_order_out, std::vector<AfPass> &ps) {
+    if (c.sh.kind == CORE_SINGLE) {
+        const int L = (int)c.sh.N;
+        AfPass d = base_pass(prec, L, c.sh.r, c.tw, choose_B_rows(prec, L, nrows), 1, nrows, sign, plain);
+        set_in(d, in, 1, 0);
+        set_out(d, out, 1, 0);
+        ps.push_back(d);
+        return;
+    }
+    const int64_t N = c.sh.N, N1 = c.sh.N1, N2 = c.sh.N2;
+    Endpoint sc;
+    sc.re = scratch;
+    sc.bs = N;
+    AfPass a = base_pass(prec, (int)N1, c.sh.r1, c.tw1, choose_B_cols(prec, N1), N2, nrows, sign, plain);
+    set_in(a, in, N2, 1);
+    set_out(a, sc, N2, 1);
+    a.in_lines_fast = a.out_lines_fast = 1;
+    set_ft(a, c, sign);
+    ps.push_back(a);
+    AfPass b = base_pass(prec, (int)N2, c.sh.r2, c.tw2, choose_B_cols(prec, N2), N1, nrows, sign, plain);
+    set_in(b, sc, 1, N2);
+    if (row_order_out) {
+        set_out(b, out, 1, N2);
+    } else {
+        set_out(b, out, N1, 1);
+        b.out_lines_fast = 1;
+    }
+    ps.push_back(b);
+}
+
+// Convolution with a spectrum: F+( spec * F-(pre(x)) ), then post-operations.
+// The caller sets the pre/post fields on the first/last returned pass.
+// Single core: one pass.  Four-step core: A (columns, forward, twiddle),
+// B (rows: forward, spectrum, inverse, conjugate twiddle), C (columns, inverse).
+void conv_passes(int prec, const Core &c, int64_t nrows, bool plain, const Endpoint &in, const Endpoint &out,
+                 const void *spec, int64_t spec_bs, bool spec_plain, void *scratch, std::vector<AfPass> &ps) {
+    const int spf = spec_plain ? AFP_SPEC_PLAIN : 0;
+    if (c.sh.kind == CORE_SINGLE) {
+        const int L = (int)c.sh.N;
+        AfPass d = base_pass(prec, L, c.sh.r, c.tw, choose_B_rows(prec, L, nrows), 1, nrows, -1, plain);
+        set_in(d, in, 1, 0);
+        set_out(d, out, 1, 0);
+        d.flags |= AFP_CONV | spf;
+        d.spec = spec;
+        d.spec_bs = spec_bs;
+        d.spec_ls = 0;
+        ps.push_back(d);
+        return;
+    }
+    const int64_t N = c.sh.N, N1 = c.sh.N1, N2 = c.sh.N2;
+    Endpoint sc;
+    sc.re = scratch;
+    sc.bs = N;
+    AfPass a = base_pass(prec, (int)N1, c.sh.r1, c.tw1, choose_B_cols(prec, N1), N2, nrows, -1, plain);
+    set_in(a, in, N2, 1);
+    set_out(a, sc, N2, 1);
+    a.in_lines_fast = a.out_lines_fast = 1;
+    set_ft(a, c, -1);
+    ps.push_back(a);
+    AfPass b = base_pass(prec, (int)N2, c.sh.r2, c.tw2, choose_B_rows(prec, N2, nrows * N1), N1, nrows, -1, plain);
+    set_in(b, sc, 1, N2);
+    set_out(b, sc, 1, N2);
+    b.flags |= AFP_CONV | spf;
+    b.spec = spec;
+    b.spec_bs = spec_bs;
+    b.spec_ls = N2;
+    set_ft(b, c, +1);
+    ps.push_back(b);
+    AfPass cc = base_pass(prec, (int)N1, c.sh.r1, c.tw1, choose_B_cols(prec, N1), N2, nrows, +1, plain);
+    set_in(cc, sc, N2, 1);
+    set_out(cc, out, N2, 1);
+    cc.in_lines_fast = cc.out_lines_fast = 1;
+    ps.push_back(cc);
+}
+
+af_status launch_all(int prec, const std::vector<AfPass> &ps, hipStream_t s) {
+    for (const AfPass &d : ps) HIPCHK(launch_pass(prec, d, s));
+    return AF_SUCCESS;
+}
+
+// Stream-ordered scratch for one call.
+struct Scratch {
+    void *p = nullptr;
+    hipStream_t s = nullptr;
+    bool owned = false;
+    af_status get(size_t bytes, void *user, hipStream_t stream) {
+        s = stream;
+        if (bytes == 0) return AF_SUCCESS;
+        if (user) {
+            p = user;
+            return AF_SUCCESS;
+        }
+        HIPCHK(hipMallocAsync(&p, bytes, stream));
+        owned = true;
+        return AF_SUCCESS;
+    }
+    ~Scratch() {
+        if (owned && p) (void)hipFreeAsync(p, s);
+    }
+};
+
+int prec_of(const af_plan_desc &d) { return d.precision == AF_F64 ? PREC_F64 : PREC_F32; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
+
+struct af_plan_s {
+    af_plan_desc desc{};
+    int prec = PREC_F32;
+    bool plain = false;
+    PlanShape shape;
+    Core core;
+    Owner own;
+    int32_t *in_map = nullptr, *out_map = nullptr;  // Rader gathers / scatters
+    void *chirp = nullptr;                          // Bluestein chirp, table format, length n
+    void *spec = nullptr;                           // convolution spectrum, table format
+    void *aux_tw = nullptr;                         // real packing twiddles W_n^k, k < n/2
+    af_plan dual = nullptr;                         // the opposite sign (adjoint of Rader / Bluestein)
+    af_plan sub = nullptr;                          // convolveCyclic of non-smooth lengths: C2C forward
+    ~af_plan_s() {
+        if (dual) af_plan_destroy(dual);
+        if (sub) af_plan_destroy(sub);
+    }
+    size_t scratch_per_row() const {
+        const size_t e = (size_t)elem_bytes(desc.precision);
+        size_t n = 0;
+        if (desc.kind == AF_CONV_CYCLIC) {
+            if (sub) return e * (size_t)desc.length * 2;
+            n = (size_t)core.sh.N * (core.sh.kind == CORE_FOUR ? 2 : 1);
+            return e * n;
+        }
+        if (core.sh.kind == CORE_FOUR) n = (size_t)core.sh.N + (shape.algo == ALG_RADER ? 2 : 0);
+        return e * n;
+    }
+};
+
+namespace {
+
+af_status validate_desc(const af_plan_desc *d) {
+    if (!d) return AF_INVALID_ARG;
+    if (d->length < 0) return AF_INVALID_ARG;
+    if (d->sign != 1 && d->sign != -1) return AF_INVALID_ARG;
+    if (d->precision != AF_F64 && d->precision != AF_F32) return AF_INVALID_ARG;
+    if (d->algorithm < AF_ALG_AUTO || d->algorithm > AF_ALG_CHIRP235) return AF_INVALID_ARG;
+    if (d->flags & ~AF_FLAG_PLAIN_TWIDDLES) return AF_INVALID_ARG;
+    switch (d->kind) {
+    case AF_C2C: break;
+    case AF_R2C_FULL:
+    case AF_C2R_FULL:
+        if (d->length % 2) return AF_INVALID_ARG;  // Real.hs:45-47, 88-90
+        break;
+    case AF_CONV_CYCLIC:
+        if (d->length < 1) return AF_INVALID_ARG;  // Planned.hs:776-778
+        break;
+    case AF_REAL_CONV:
+        if (d->length % 2 || d->length < 2) return AF_INVALID_ARG;
+        break;
+    default: return AF_INVALID_ARG;
+    }
+    return AF_SUCCESS;
+}
+
+// Length of the device FFT core a kind needs.
+int64_t core_length(const af_plan_desc &d) {
+    if (d.kind == AF_R2C_FULL || d.kind == AF_C2R_FULL || d.kind == AF_REAL_CONV) return d.length / 2;
+    return d.length;
+}
+
+af_status shape_for(const af_plan_desc &d, PlanShape &sh, std::string &why) {
+    const int prec = d.precision;
+    const int64_t n = core_length(d);
+    if (d.length > ((int64_t)1 << 30)) {
+        why = "length above 2^30";
+        return AF_UNSUPPORTED;
+    }
+    if (d.kind == AF_C2C || d.kind == AF_CONV_CYCLIC) {
+        if (!plan_shape(prec, n, d.algorithm, sh, why)) return AF_UNSUPPORTED;
+        return AF_SUCCESS;
+    }
+    // real kinds: one pass over the half-length transform
+    sh = PlanShape();
+    sh.n = n;
+    sh.algo = ALG_STOCKHAM;
+    sh.core = core_shape(prec, n < 1 ? 1 : n);
+    if (sh.core.kind != CORE_SINGLE) {
+        why = "half length must be smooth and fit one LDS pass";
+        return AF_UNSUPPORTED;
+    }
+    if (d.kind == AF_REAL_CONV && lds_bytes_for(prec, (int)std::max<int64_t>(n, 1), 1, 4) * 2 > lds_tile_bytes_cap()) {
+        why = "two half-length spectra must fit in LDS";
+        return AF_UNSUPPORTED;
+    }
+    return AF_SUCCESS;
+}
+
+// FP64 forward DFT of `count` interleaved values on the device (plan time).
+af_status fft_f64_forward(int64_t N, const void *in, void *out, hipStream_t s) {
+    af_plan_desc d{};
+    d.length = N;
+    d.sign = -1;
+    d.kind = AF_C2C;
+    d.precision = AF_F64;
+    d.algorithm = AF_ALG_AUTO;
+    af_plan p = nullptr;
+    AFCHK(af_plan_create(&d, &p));
+    af_status st = af_execute(p, 1, in, nullptr, out, nullptr, nullptr, s);
+    if (st == AF_SUCCESS) st = hipStreamSynchronize(s) == hipSuccess ? AF_SUCCESS : AF_HIP_ERROR;
+    af_plan_destroy(p);
+    return st;
+}
+
+af_status build_rader(af_plan P, hipStream_t s) {
+    const int64_t p = P->shape.n, L = p - 1;
+    const int64_t g = multiplicative_generator(p);
+    std::vector<int32_t> powg(L), inv_pow(L);
+    int64_t x = 1;
+    for (int64_t i = 0; i < L; ++i) {
+        powg[i] = (int32_t)x;
+        x = (int64_t)(((unsigned __int128)x * (uint64_t)g) % (uint64_t)p);
+    }
+    for (int64_t i = 0; i < L; ++i) inv_pow[i] = powg[(L - i) % L];  // g^-i
+    AFCHK(P->own.alloc((void **)&P->in_map, L * 4));
+    AFCHK(P->own.alloc((void **)&P->out_map, L * 4));
+    HIPCHK(hipMemcpyAsync(P->in_map, inv_pow.data(), L * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(P->out_map, powg.data(), L * 4, hipMemcpyHostToDevice, s));
+    // kernel b_i = exp(sign 2 pi i g^i / p), spectrum F-(b) / (p-1)
+    void *kb = nullptr, *kf = nullptr;
+    HIPCHK(hipMalloc(&kb, L * 16));
+    HIPCHK(hipMalloc(&kf, L * 16));
+    af_status st = AF_SUCCESS;
+    if (gen_rader_kernel_d(kb, P->out_map, p, L, P->desc.sign, s) != hipSuccess) st = AF_HIP_ERROR;
+    if (st == AF_SUCCESS) st = fft_f64_forward(L, kb, kf, s);
+    if (st == AF_SUCCESS) st = P->own.alloc(&P->spec, L * 16);
+    if (st == AF_SUCCESS) {
+        const bool four = P->core.sh.kind == CORE_FOUR;
+        if (convert_to_w(P->prec, P->spec, kf, L, 1.0 / (double)L, four ? P->core.sh.N1 : 0,
+                         four ? P->core.sh.N2 : 0, s) != hipSuccess)
+            st = AF_HIP_ERROR;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && st == AF_SUCCESS) st = AF_HIP_ERROR;
+    (void)hipFree(kb);
+    (void)hipFree(kf);
+    return st;
+}
+
+af_status build_bluestein(af_plan P, hipStream_t s) {
+    const int64_t n = P->shape.n, M = P->shape.M;
+    void *cd = nullptr, *hb = nullptr, *hf = nullptr;
+    HIPCHK(hipMalloc(&cd, n * 16));
+    HIPCHK(hipMalloc(&hb, M * 16));
+    HIPCHK(hipMalloc(&hf, M * 16));
+    af_status st = AF_SUCCESS;
+    // chirp c_k = exp(sign i pi k^2 / n), k < n; kernel h = conj(c) wrapped over M
+    if (gen_chirp_d(cd, n, n, P->desc.sign, 0, 0, s) != hipSuccess) st = AF_HIP_ERROR;
+    if (st == AF_SUCCESS) st = P->own.alloc(&P->chirp, n * 16);
+    if (st == AF_SUCCESS && convert_to_w(P->prec, P->chirp, cd, n, 1.0, 0, 0, s) != hipSuccess) st = AF_HIP_ERROR;
+    if (st == AF_SUCCESS && gen_chirp_d(hb, n, M, P->desc.sign, M, 1, s) != hipSuccess) st = AF_HIP_ERROR;
+    if (st == AF_SUCCESS) st = fft_f64_forward(M, hb, hf, s);
+    if (st == AF_SUCCESS) st = P->own.alloc(&P->spec, M * 16);
+    if (st == AF_SUCCESS) {
+        const bool four = P->core.sh.kind == CORE_FOUR;
+        if (convert_to_w(P->prec, P->spec, hf, M, 1.0 / (double)M, four ? P->core.sh.N1 : 0,
+                         four ? P->core.sh.N2 : 0, s) != hipSuccess)
+            st = AF_HIP_ERROR;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && st == AF_SUCCESS) st = AF_HIP_ERROR;
+    (void)hipFree(cd);
+    (void)hipFree(hb);
+    (void)hipFree(hf);
+    return st;
+}
+
+std::once_flag g_attr_flag;
+void set_kernel_attributes() { init_kernel_attributes(); }
+
+}  // namespace
+
+extern "C" {
+
+const char *af_status_string(af_status s) {
+    switch (s) {
+    case AF_SUCCESS: return "success";
+    case AF_INVALID_ARG: return "invalid argument (a precondition of the reference is violated)";
+    case AF_UNSUPPORTED: return "unsupported by this build";
+    case AF_NO_DEVICE: return "no gfx950 device";
+    case AF_HIP_ERROR: return "HIP runtime error";
+    case AF_OUT_OF_MEMORY: return "out of device memory";
+    case AF_INTERNAL: return "internal error";
+    case AF_TIMEOUT: return "device wait timed out";
+    }
+    return "unknown status";
+}
+
+const char *af_version(void) { return "accelerate-fourier-amd 0.1 (gfx950)"; }
+
+af_status af_plan_describe(const af_plan_desc *desc, char *buf, size_t buflen) {
+    AFCHK(validate_desc(desc));
+    if (!buf || buflen == 0) return AF_INVALID_ARG;
+    PlanShape sh;
+    std::string why;
+    af_status st = shape_for(*desc, sh, why);
+    std::string s;
+    const char *kinds[] = {"c2c", "r2c", "c2r", "convcyclic", "realconv"};
+    s = std::string(kinds[desc->kind]) + (desc->precision == AF_F64 ? ".f64:" : ".f32:");
+    if (st == AF_SUCCESS) s += describe_shape(sh);
+    else s += "unsupported(" + why + ")";
+    std::snprintf(buf, buflen, "%s", s.c_str());
+    return st == AF_SUCCESS && s.size() < buflen ? AF_SUCCESS : (st != AF_SUCCESS ? st : AF_INVALID_ARG);
+}
+
+static af_status plan_create_impl(const af_plan_desc *desc, af_plan *plan, bool with_dual);
+
+af_status af_plan_create(const af_plan_desc *desc, af_plan *plan) { return plan_create_impl(desc, plan, true); }
+
+static af_status plan_create_impl(const af_plan_desc *desc, af_plan *plan, bool with_dual) {
+    if (!plan) return AF_INVALID_ARG;
+    *plan = nullptr;
+    AFCHK(validate_desc(desc));
+    PlanShape sh;
+    std::string why;
+    AFCHK(shape_for(*desc, sh, why));
+    AFCHK(device_init());
+    std::call_once(g_attr_flag, set_kernel_attributes);
+    std::unique_ptr<af_plan_s> P(new af_plan_s);
+    P->desc = *desc;
+    P->prec = prec_of(*desc);
+    P->plain = (desc->flags & AF_FLAG_PLAIN_TWIDDLES) != 0;
+    P->shape = sh;
+    hipStream_t s = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    af_status st = AF_SUCCESS;
+    if (desc->kind == AF_CONV_CYCLIC && sh.algo != ALG_STOCKHAM && sh.algo != ALG_COPY) {
+        // non-smooth length: three full transforms and a pointwise product
+        af_plan_desc d2 = *desc;
+        d2.kind = AF_C2C;
+        d2.sign = -1;
+        st = af_plan_create(&d2, &P->sub);
+        if (st == AF_SUCCESS) {
+            CoreShape one;
+            one.kind = CORE_SINGLE;
+            one.N = 1;
+            P->core.sh = one;
+        }
+    } else if (sh.algo == ALG_COPY) {
+        CoreShape one;
+        one.kind = CORE_SINGLE;
+        one.N = 1;
+        P->core.sh = one;
+    } else if (sh.algo != ALG_EMPTY) {
+        st = build_core(P->prec, sh.core, P->core, P->own, s);
+    }
+    if (st == AF_SUCCESS && desc->kind == AF_C2C) {
+        if (sh.algo == ALG_RADER) st = build_rader(P.get(), s);
+        else if (sh.algo == ALG_BLUESTEIN) st = build_bluestein(P.get(), s);
+        if (st == AF_SUCCESS && with_dual && (sh.algo == ALG_RADER || sh.algo == ALG_BLUESTEIN)) {
+            // the adjoint needs the tables of the opposite sign (cacheDuplex, Planned.hs:335-341)
+            af_plan_desc d2 = *desc;
+            d2.sign = -desc->sign;
+            st = plan_create_impl(&d2, &P->dual, false);
+        }
+    }
+    if (st == AF_SUCCESS &&
+        (desc->kind == AF_R2C_FULL || desc->kind == AF_C2R_FULL || desc->kind == AF_REAL_CONV) && sh.n >= 1) {
+        st = P->own.alloc(&P->aux_tw, (size_t)sh.n * 16);
+        if (st == AF_SUCCESS && gen_twiddles(P->prec, P->aux_tw, 2 * sh.n, sh.n, 1, s) != hipSuccess)
+            st = AF_HIP_ERROR;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && st == AF_SUCCESS) st = AF_HIP_ERROR;
+    (void)hipStreamDestroy(s);
+    if (st != AF_SUCCESS) return st;
+    *plan = P.release();
+    return AF_SUCCESS;
+}
+
+af_status af_plan_destroy(af_plan plan) {
+    if (!plan) return AF_INVALID_ARG;
+    delete plan;
+    return AF_SUCCESS;
+}
+
+af_status af_plan_workspace_size(af_plan plan, int64_t batch, size_t *bytes) {
+    if (!plan || !bytes || batch < 0) return AF_INVALID_ARG;
+    *bytes = plan->scratch_per_row() * (size_t)batch;
+    return AF_SUCCESS;
+}
+
+}  // extern "C"
+
+namespace {
+
+Endpoint ep(const void *re, const void *im, int64_t bs) {
+    Endpoint e;
+    e.re = re;
+    e.im = im;
+    e.bs = bs;
+    return e;
+}
+
+// Passes of a complex transform with an explicit sign (the plan's own, or the
+// opposite for the adjoint of Stockham plans).  `scratch` holds
+// scratch_per_row() bytes per row.
+af_status build_c2c(af_plan P, int sign, int64_t batch, const Endpoint &in, const Endpoint &out, void *scratch,
+                    std::vector<AfPass> &ps) {
+    if (sign != P->desc.sign && (P->shape.algo == ALG_RADER || P->shape.algo == ALG_BLUESTEIN)) {
+        // Rader kernels and chirps depend on the sign: the opposite sign is the dual plan's
+        if (!P->dual) return AF_INTERNAL;
+        P = P->dual;
+    }
+    const int64_t n = P->shape.n;
+    const int prec = P->prec;
+    if (n == 0 || batch == 0) return AF_SUCCESS;
+    switch (P->shape.algo) {
+    case ALG_COPY: {
+        AfPass d = base_pass(prec, 1, {}, nullptr, 64, 1, batch, sign, P->plain);
+        set_in(d, in, 1, 0);
+        set_out(d, out, 1, 0);
+        ps.push_back(d);
+        break;
+    }
+    case ALG_STOCKHAM:
+        c2c_passes(prec, P->core, batch, sign, P->plain, in, out, scratch, false, ps);
+        break;
+    case ALG_RADER: {
+        const int64_t L = n - 1;
+        const bool four = P->core.sh.kind == CORE_FOUR;
+        const size_t first = ps.size();
+        conv_passes(prec, P->core, batch, P->plain, in, out, P->spec, 0, false, scratch, ps);
+        AfPass &a = ps[first];
+        a.in_map = P->in_map;
+        a.in_len = L;
+        AfPass &last = ps.back();
+        last.out_map = P->out_map;
+        last.out_len = L;
+        if (!four) {
+            last.flags |= AFP_RADER;
+        } else {
+            // x0 of every row is saved by the first pass, so in-place calls are safe
+            void *side = (char *)scratch + (size_t)batch * L * elem_bytes(P->desc.precision);
+            void *dc = (char *)side + (size_t)batch * elem_bytes(P->desc.precision);
+            a.flags |= AFP_X0_CAPTURE;
+            a.x0_out = side;
+            ps[first + 1].flags |= AFP_DC_CAPTURE;
+            ps[first + 1].dc_out = dc;
+            last.flags |= AFP_RADER_OUT;
+            last.x0_re = side;
+            last.x0_im = nullptr;
+            last.x0_bs = 1;
+            last.dc_in = dc;
+        }
+        break;
+    }
+    case ALG_BLUESTEIN: {
+        const size_t first = ps.size();
+        conv_passes(prec, P->core, batch, P->plain, in, out, P->spec, 0, false, scratch, ps);
+        ps[first].in_len = n;
+        ps[first].in_mul = P->chirp;
+        ps.back().out_len = n;
+        ps.back().out_mul = P->chirp;
+        break;
+    }
+    default: return AF_INTERNAL;
+    }
+    return AF_SUCCESS;
+}
+
+af_status run_c2c(af_plan P, int sign, int64_t batch, const void *in_re, const void *in_im, void *out_re,
+                  void *out_im, void *ws, hipStream_t s) {
+    const int64_t n = P->shape.n;
+    if (n == 0 || batch == 0) return AF_SUCCESS;
+    Scratch sc;
+    AFCHK(sc.get(P->scratch_per_row() * (size_t)batch, ws, s));
+    std::vector<AfPass> ps;
+    AFCHK(build_c2c(P, sign, batch, ep(in_re, in_im, n), ep(out_re, out_im, n), sc.p, ps));
+    return launch_all(P->prec, ps, s);
+}
+
+bool bad_ptrs(const void *a, const void *b) { return !a || !b; }
+
+}  // namespace
+
+extern "C" {
+
+af_status af_execute(af_plan plan, int64_t batch, const void *in_re, const void *in_im, void *out_re, void *out_im,
+                     void *workspace, void *stream) {
+    if (!plan || plan->desc.kind != AF_C2C || batch < 0) return AF_INVALID_ARG;
+    if (batch > 0 && plan->desc.length > 0 && bad_ptrs(in_re, out_re)) return AF_INVALID_ARG;
+    if (batch * std::max<int64_t>(plan->desc.length, 1) > ((int64_t)1 << 40)) return AF_INVALID_ARG;
+    return run_c2c(plan, plan->desc.sign, batch, in_re, in_im, out_re, out_im, workspace, (hipStream_t)stream);
+}
+
+af_status af_plan_num_passes(af_plan plan, int64_t batch, int32_t *npasses) {
+    if (!plan || !npasses || batch < 0 || plan->desc.kind != AF_C2C) return AF_INVALID_ARG;
+    std::vector<AfPass> ps;
+    const int64_t n = plan->desc.length;
+    AFCHK(build_c2c(plan, plan->desc.sign, batch, ep((void *)16, nullptr, n), ep((void *)16, nullptr, n),
+                    (void *)16, ps));
+    *npasses = (int32_t)ps.size();
+    return AF_SUCCESS;
+}
+
+af_status af_execute_pass(af_plan plan, int32_t index, int64_t batch, const void *in_re, const void *in_im,
+                          void *out_re, void *out_im, void *workspace, void *stream) {
+    if (!plan || plan->desc.kind != AF_C2C || batch < 0 || index < 0) return AF_INVALID_ARG;
+    if (batch > 0 && plan->desc.length > 0 && bad_ptrs(in_re, out_re)) return AF_INVALID_ARG;
+    if (plan->scratch_per_row() > 0 && !workspace) return AF_INVALID_ARG;
+    const int64_t n = plan->desc.length;
+    std::vector<AfPass> ps;
+    AFCHK(build_c2c(plan, plan->desc.sign, batch, ep(in_re, in_im, n), ep(out_re, out_im, n), workspace, ps));
+    if (index >= (int32_t)ps.size()) return AF_INVALID_ARG;
+    HIPCHK(launch_pass(plan->prec, ps[index], (hipStream_t)stream));
+    return AF_SUCCESS;
+}
+
+af_status af_vjp_execute(af_plan plan, int64_t batch, const void *g_re, const void *g_im, void *out_re, void *out_im,
+                         void *workspace, void *stream) {
+    if (!plan || plan->desc.kind != AF_C2C || batch < 0) return AF_INVALID_ARG;
+    if (batch > 0 && plan->desc.length > 0 && bad_ptrs(g_re, out_re)) return AF_INVALID_ARG;
+    // adjoint of the unnormalised DFT with sign s: the DFT with sign -s
+    if (plan->dual)
+        return run_c2c(plan->dual, plan->dual->desc.sign, batch, g_re, g_im, out_re, out_im, workspace,
+                       (hipStream_t)stream);
+    return run_c2c(plan, -plan->desc.sign, batch, g_re, g_im, out_re, out_im, workspace, (hipStream_t)stream);
+}
+
+af_status af_to_spectrum(af_plan plan, int64_t batch, const void *x, void *out_re, void *out_im, void *workspace,
+                         void *stream) {
+    (void)workspace;
+    if (!plan || plan->desc.kind != AF_R2C_FULL || batch < 0) return AF_INVALID_ARG;
+    const int64_t n = plan->desc.length, n2 = n / 2;
+    if (n == 0 || batch == 0) return AF_SUCCESS;
+    if (bad_ptrs(x, out_re)) return AF_INVALID_ARG;
+    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, choose_B_rows(plan->prec, n2, batch), 1,
+                         batch, plan->desc.sign, plan->plain);
+    set_in(d, ep(x, nullptr, n2), 1, 0);
+    set_out(d, ep(out_re, out_im, n), 1, 0);
+    d.flags |= AFP_REAL_UNTANGLE;
+    d.aux_tw = plan->aux_tw;
+    HIPCHK(launch_pass(plan->prec, d, (hipStream_t)stream));
+    return AF_SUCCESS;
+}
+
+af_status af_from_spectrum(af_plan plan, int64_t batch, const void *in_re, const void *in_im, void *y,
+                           void *workspace, void *stream) {
+    (void)workspace;
+    if (!plan || plan->desc.kind != AF_C2R_FULL || batch < 0) return AF_INVALID_ARG;
+    const int64_t n = plan->desc.length, n2 = n / 2;
+    if (n == 0 || batch == 0) return AF_SUCCESS;
+    if (bad_ptrs(in_re, y)) return AF_INVALID_ARG;
+    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, choose_B_rows(plan->prec, n2, batch), 1,
+                         batch, plan->desc.sign, plan->plain);
+    set_in(d, ep(in_re, in_im, n), 1, 0);
+    set_out(d, ep(y, nullptr, n2), 1, 0);
+    d.flags |= AFP_REAL_ENTANGLE;
+    d.aux_tw = plan->aux_tw;
+    HIPCHK(launch_pass(plan->prec, d, (hipStream_t)stream));
+    return AF_SUCCESS;
+}
+
+af_status af_vjp_to_spectrum(af_plan plan, int64_t batch, const void *g_re, const void *g_im, void *dx,
+                             void *workspace, void *stream) {
+    (void)workspace;
+    if (!plan || plan->desc.kind != AF_R2C_FULL || batch < 0) return AF_INVALID_ARG;
+    if (plan->desc.sign != -1) return AF_UNSUPPORTED;  // the DFT's real restriction only
+    const int64_t n = plan->desc.length, n2 = n / 2;
+    if (n == 0 || batch == 0) return AF_SUCCESS;
+    if (bad_ptrs(g_re, dx)) return AF_INVALID_ARG;
+    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, choose_B_rows(plan->prec, n2, batch), 1,
+                         batch, +1, plan->plain);
+    set_in(d, ep(g_re, g_im, n), 1, 0);
+    set_out(d, ep(dx, nullptr, n2), 1, 0);
+    d.flags |= AFP_UNTANGLE_ADJ;
+    d.aux_tw = plan->aux_tw;
+    HIPCHK(launch_pass(plan->prec, d, (hipStream_t)stream));
+    return AF_SUCCESS;
+}
+
+af_status af_vjp_from_spectrum(af_plan plan, int64_t batch, const void *g, void *out_re, void *out_im,
+                               void *workspace, void *stream) {
+    (void)workspace;
+    if (!plan || plan->desc.kind != AF_C2R_FULL || batch < 0) return AF_INVALID_ARG;
+    const int64_t n = plan->desc.length, n2 = n / 2;
+    if (n == 0 || batch == 0) return AF_SUCCESS;
+    if (bad_ptrs(g, out_re)) return AF_INVALID_ARG;
+    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, choose_B_rows(plan->prec, n2, batch), 1,
+                         batch, -plan->desc.sign, plan->plain);
+    set_in(d, ep(g, nullptr, n2), 1, 0);
+    set_out(d, ep(out_re, out_im, n), 1, 0);
+    d.flags |= AFP_ENTANGLE_ADJ;
+    d.aux_tw = plan->aux_tw;
+    HIPCHK(launch_pass(plan->prec, d, (hipStream_t)stream));
+    return AF_SUCCESS;
+}
+
+static AfPass real_conv_geom(af_plan plan, int64_t batch) {
+    const int64_t n2 = plan->desc.length / 2;
+    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, 1, 1, batch, -1, plan->plain);
+    d.aux_tw = plan->aux_tw;
+    d.lds_bytes = 2 * (pidx_h(d.ld, d.padshift) + 1) * elem_bytes(plan->prec);
+    return d;
+}
+
+af_status af_real_convolve(af_plan plan, int64_t batch, const void *x, const void *k, void *y, void *stream) {
+    if (!plan || plan->desc.kind != AF_REAL_CONV || batch < 0) return AF_INVALID_ARG;
+    if (batch == 0) return AF_SUCCESS;
+    if (!x || !k || !y) return AF_INVALID_ARG;
+    HIPCHK(launch_real_conv(plan->prec, real_conv_geom(plan, batch), x, k, y, (hipStream_t)stream));
+    return AF_SUCCESS;
+}
+
+af_status af_vjp_real_convolve(af_plan plan, int64_t batch, const void *x, const void *k, const void *w, void *dx,
+                               void *dk, void *stream) {
+    if (!plan || plan->desc.kind != AF_REAL_CONV || batch < 0) return AF_INVALID_ARG;
+    if (batch == 0) return AF_SUCCESS;
+    if (!x || !k || !w || !dx || !dk) return AF_INVALID_ARG;
+    HIPCHK(launch_real_conv_vjp(plan->prec, real_conv_geom(plan, batch), x, k, w, dx, dk, (hipStream_t)stream));
+    return AF_SUCCESS;
+}
+
+af_status af_convolve_cyclic(af_plan plan, int64_t batch, const void *x_re, const void *x_im, const void *y_re,
+                             const void *y_im, void *out_re, void *out_im, void *workspace, void *stream) {
+    if (!plan || plan->desc.kind != AF_CONV_CYCLIC || batch < 0) return AF_INVALID_ARG;
+    if (batch == 0) return AF_SUCCESS;
+    if (!x_re || !y_re || !out_re) return AF_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = plan->desc.length;
+    const int prec = plan->prec;
+    const size_t eb = (size_t)elem_bytes(plan->desc.precision);
+    Scratch sc;
+    AFCHK(sc.get(plan->scratch_per_row() * (size_t)batch, workspace, s));
+    const Endpoint X = ep(x_re, x_im, n), Y = ep(y_re, y_im, n), O = ep(out_re, out_im, n);
+    std::vector<AfPass> ps;
+    if (plan->sub) {
+        // convolveCyclic (Planned.hs:779-821): F+( F-(x) / n * F-(y) ) with general sub-plans
+        void *s1 = sc.p, *s2 = (char *)sc.p + (size_t)batch * n * eb;
+        AFCHK(af_execute(plan->sub, batch, x_re, x_im, s1, nullptr, nullptr, s));
+        AFCHK(af_execute(plan->sub, batch, y_re, y_im, s2, nullptr, nullptr, s));
+        AfPass m = base_pass(prec, 1, {}, nullptr, 64, n, batch, -1, plan->plain);
+        set_in(m, ep(s1, nullptr, n), 0, 1);
+        set_out(m, ep(s1, nullptr, n), 0, 1);
+        m.flags |= AFP_CONV | AFP_SPEC_PLAIN | AFP_SCALE;
+        m.spec = s2;
+        m.spec_bs = n;
+        m.spec_ls = 1;
+        m.scale = 1.0 / (double)n;
+        HIPCHK(launch_pass(prec, m, s));
+        return run_c2c(plan->sub, +1, batch, s1, nullptr, out_re, out_im, nullptr, s);
+    }
+    const bool four = plan->core.sh.kind == CORE_FOUR;
+    void *spec = sc.p;
+    void *scr = four ? (char *)sc.p + (size_t)batch * n * eb : nullptr;
+    // spectrum of y, scaled by 1/n, in the order the convolution passes read it
+    c2c_passes(prec, plan->core, batch, -1, plan->plain, Y, ep(spec, nullptr, n), scr, true, ps);
+    ps.back().flags |= AFP_SCALE;
+    ps.back().scale = 1.0 / (double)n;
+    conv_passes(prec, plan->core, batch, plan->plain, X, O, spec, n, true, scr, ps);
+    return launch_all(prec, ps, s);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Karatsuba
+// ---------------------------------------------------------------------------
+
+namespace {
+
+int karatsuba_P(int64_t Lp) {
+    int d = 0;
+    int64_t s = Lp;
+    while (s > 64) {
+        ++d;
+        s = (Lp + ((int64_t)1 << d) - 1) >> d;
+    }
+    return (int)(s << d);
+}
+
+const int kKaratsubaMaxP = 4608;
+
+af_status karatsuba_run(const std::vector<int64_t> &meta_in, int64_t npairs, int maxP, const int64_t *x,
+                        const int64_t *y, int64_t *z, int cyclic, hipStream_t s) {
+    AFCHK(device_init());
+    std::call_once(g_attr_flag, set_kernel_attributes);
+    if (npairs == 0) return AF_SUCCESS;
+    if (maxP > kKaratsubaMaxP) return AF_UNSUPPORTED;
+    // longest pairs first, for load balance
+    std::vector<int64_t> order(npairs);
+    for (int64_t i = 0; i < npairs; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return meta_in[3 * a + 1] > meta_in[3 * b + 1]; });
+    std::vector<int64_t> meta(3 * npairs);
+    for (int64_t i = 0; i < npairs; ++i)
+        for (int j = 0; j < 3; ++j) meta[3 * i + j] = meta_in[3 * order[i] + j];
+    void *dm = nullptr;
+    HIPCHK(hipMallocAsync(&dm, meta.size() * 8, s));
+    af_status st = AF_SUCCESS;
+    if (hipMemcpyAsync(dm, meta.data(), meta.size() * 8, hipMemcpyHostToDevice, s) != hipSuccess) st = AF_HIP_ERROR;
+    // the host vector dies with this frame: wait for the copy before returning
+    if (st == AF_SUCCESS && hipStreamSynchronize(s) != hipSuccess) st = AF_HIP_ERROR;
+    if (st == AF_SUCCESS && launch_karatsuba(x, y, z, (const int64_t *)dm, npairs, maxP, cyclic, s) != hipSuccess)
+        st = AF_HIP_ERROR;
+    (void)hipFreeAsync(dm, s);
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+af_status af_karatsuba_i64(int64_t batch, int64_t len, const int64_t *x, const int64_t *y, int64_t *z, void *stream) {
+    if (batch < 0 || len < 0) return AF_INVALID_ARG;
+    if (batch == 0 || len == 0) return AF_SUCCESS;
+    if (!x || !y || !z) return AF_INVALID_ARG;
+    std::vector<int64_t> meta(3 * batch);
+    for (int64_t r = 0; r < batch; ++r) {
+        meta[3 * r] = r * len;
+        meta[3 * r + 1] = len;
+        meta[3 * r + 2] = r * (2 * len - 1);
+    }
+    return karatsuba_run(meta, batch, karatsuba_P(len), x, y, z, 0, (hipStream_t)stream);
+}
+
+af_status af_karatsuba_cyclic_i64(int64_t batch, int64_t len, const int64_t *x, const int64_t *y, int64_t *z,
+                                  void *stream) {
+    if (batch < 0 || len < 0) return AF_INVALID_ARG;
+    if (batch == 0 || len == 0) return AF_SUCCESS;
+    if (!x || !y || !z) return AF_INVALID_ARG;
+    std::vector<int64_t> meta(3 * batch);
+    for (int64_t r = 0; r < batch; ++r) {
+        meta[3 * r] = r * len;
+        meta[3 * r + 1] = len;
+        meta[3 * r + 2] = r * len;
+    }
+    return karatsuba_run(meta, batch, karatsuba_P(len), x, y, z, 1, (hipStream_t)stream);
+}
+
+af_status af_karatsuba_ragged_i64(int64_t npairs, const int64_t *offsets, const int64_t *x, const int64_t *y,
+                                  int64_t *z, void *stream) {
+    if (npairs < 0 || (npairs > 0 && !offsets)) return AF_INVALID_ARG;
+    if (npairs == 0) return AF_SUCCESS;
+    if (!x || !y || !z) return AF_INVALID_ARG;
+    if (offsets[0] != 0) return AF_INVALID_ARG;
+    std::vector<int64_t> meta(3 * npairs);
+    int maxP = 0;
+    for (int64_t p = 0; p < npairs; ++p) {
+        const int64_t L = offsets[p + 1] - offsets[p];
+        if (L < 1) return AF_INVALID_ARG;  // equal, non-empty operand lengths per pair
+        meta[3 * p] = offsets[p];
+        meta[3 * p + 1] = L;
+        meta[3 * p + 2] = 2 * offsets[p] - p;
+        maxP = std::max(maxP, karatsuba_P(L));
+    }
+    return karatsuba_run(meta, npairs, maxP, x, y, z, 0, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// Host index maps and number theory
+// ---------------------------------------------------------------------------
+
+af_status af_rader_permutations(int64_t p, int64_t *perm, int64_t *rev, int64_t *inv) {
+    if (!is_prime(p) || p < 3 || !perm || !rev || !inv) return AF_INVALID_ARG;
+    const int64_t L = p - 1, g = multiplicative_generator(p);
+    int64_t x = 1;
+    for (int64_t i = 0; i < L; ++i) {
+        perm[i] = x - 1;
+        x = (int64_t)(((unsigned __int128)x * (uint64_t)g) % (uint64_t)p);
+    }
+    for (int64_t i = 0; i < L; ++i) rev[i] = perm[(L - i) % L];
+    for (int64_t i = 0; i < L; ++i) inv[perm[i]] = i;
+    return AF_SUCCESS;
+}
+
+af_status af_good_thomas_maps(int64_t n, int64_t m, int64_t *in_map, int64_t *out_map) {
+    if (n < 1 || m < 1 || !in_map || !out_map) return AF_INVALID_ARG;
+    int64_t a = n, b = m;
+    while (b) { int64_t t = a % b; a = b; b = t; }
+    if (a != 1) return AF_INVALID_ARG;
+    const int64_t nm = n * m;
+    for (int64_t k = 0; k < m; ++k)
+        for (int64_t j = 0; j < n; ++j) in_map[k * n + j] = (n * k + m * j) % nm;
+    for (int64_t q = 0; q < nm; ++q) out_map[q] = (q % n) * m + (q % m);
+    return AF_SUCCESS;
+}
+
+int64_t af_multiplicative_generator(int64_t p) { return is_prime(p) ? multiplicative_generator(p) : -1; }
+int64_t af_ceiling_5smooth(int64_t n) { return n < 1 ? 1 : ceil_5smooth(n); }
+int64_t af_ceiling_power_of_two(int64_t n) { return ceil_pow2(n); }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Uneven batches: all rows' passes as grouped launches in one hipGraph
+// ---------------------------------------------------------------------------
+
+struct af_batch_s {
+    int64_t count = 0;
+    int sign = -1;
+    int precision = AF_F32;
+    int prec = PREC_F32;
+    std::vector<int64_t> lengths, offsets;
+    std::map<int64_t, af_plan> plans;
+    std::vector<size_t> scratch_off;
+    Owner own;
+    void *scratch = nullptr;
+    struct Group {
+        int phase = 0, nthreads = 0, lds = 0;
+        std::vector<int64_t> rows;  // transform indices
+        int64_t desc_off = 0, prefix_off = 0, total_tiles = 0;
+    };
+    std::vector<Group> groups;
+    std::vector<int> npasses;
+    AfPass *d_descs = nullptr;
+    int64_t *d_prefix = nullptr;
+    std::vector<AfPass> h_descs;
+    std::vector<int64_t> h_prefix;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void *last[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool uploaded = false;
+    std::string text;
+    ~af_batch_s() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto &kv : plans) af_plan_destroy(kv.second);
+    }
+};
+
+namespace {
+
+Endpoint offset_ep(const void *re, const void *im, int64_t off, int64_t n, size_t rb) {
+    Endpoint e;
+    if (im) {
+        e.re = (const char *)re + off * rb;
+        e.im = (const char *)im + off * rb;
+    } else {
+        e.re = (const char *)re + off * 2 * rb;
+        e.im = nullptr;
+    }
+    e.bs = n;
+    return e;
+}
+
+af_status batch_passes(af_batch B, int64_t i, const void *in_re, const void *in_im, void *out_re, void *out_im,
+                       std::vector<AfPass> &ps) {
+    const int64_t n = B->lengths[i];
+    if (n == 0) return AF_SUCCESS;
+    af_plan P = B->plans.at(n);
+    const size_t rb = B->precision == AF_F64 ? 8 : 4;
+    return build_c2c(P, B->sign, 1, offset_ep(in_re, in_im, B->offsets[i], n, rb),
+                     offset_ep(out_re, out_im, B->offsets[i], n, rb), (char *)B->scratch + B->scratch_off[i], ps);
+}
+
+}  // namespace
+
+extern "C" {
+
+af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign, int32_t precision, af_batch *batch) {
+    if (!batch) return AF_INVALID_ARG;
+    *batch = nullptr;
+    if (count < 0 || (count > 0 && !lengths) || (sign != 1 && sign != -1)) return AF_INVALID_ARG;
+    if (precision != AF_F64 && precision != AF_F32) return AF_INVALID_ARG;
+    for (int64_t i = 0; i < count; ++i)
+        if (lengths[i] < 0) return AF_INVALID_ARG;
+    AFCHK(device_init());
+    std::call_once(g_attr_flag, set_kernel_attributes);
+    std::unique_ptr<af_batch_s> B(new af_batch_s);
+    B->count = count;
+    B->sign = sign;
+    B->precision = precision;
+    B->prec = precision == AF_F64 ? PREC_F64 : PREC_F32;
+    B->lengths.assign(lengths, lengths + count);
+    B->offsets.resize(count + 1, 0);
+    for (int64_t i = 0; i < count; ++i) B->offsets[i + 1] = B->offsets[i] + lengths[i];
+    size_t scr = 0;
+    B->scratch_off.resize(count);
+    for (int64_t i = 0; i < count; ++i) {
+        const int64_t n = lengths[i];
+        B->scratch_off[i] = scr;
+        if (n == 0) continue;
+        if (!B->plans.count(n)) {
+            af_plan_desc d{};
+            d.length = n;
+            d.sign = sign;
+            d.kind = AF_C2C;
+            d.precision = precision;
+            d.algorithm = AF_ALG_AUTO;
+            af_plan p = nullptr;
+            AFCHK(plan_create_impl(&d, &p, false));
+            B->plans[n] = p;
+        }
+        scr += (B->plans[n]->scratch_per_row() + 255) / 256 * 256;
+    }
+    AFCHK(B->own.alloc(&B->scratch, scr));
+    // pass structure with placeholder pointers
+    std::vector<std::vector<AfPass>> per(count);
+    int maxp = 0;
+    B->npasses.resize(count);
+    const char *fake = (const char *)0x1000;
+    for (int64_t i = 0; i < count; ++i) {
+        AFCHK(batch_passes(B.get(), i, fake, nullptr, (void *)fake, nullptr, per[i]));
+        B->npasses[i] = (int)per[i].size();
+        maxp = std::max(maxp, B->npasses[i]);
+    }
+    const int classes[] = {64, 128, 256, 512, 1024};
+    int64_t ndesc = 0, nprefix = 0;
+    for (int ph = 0; ph < maxp; ++ph)
+        for (int c = 0; c < 5; ++c) {
+            af_batch_s::Group g;
+            g.phase = ph;
+            g.nthreads = classes[c];
+            for (int64_t i = 0; i < count; ++i) {
+                if (B->npasses[i] <= ph) continue;
+                const AfPass &d = per[i][ph];
+                const int cls = d.nthreads <= 64 ? 0 : d.nthreads <= 128 ? 1 : d.nthreads <= 256 ? 2
+                                : d.nthreads <= 512 ? 3 : 4;
+                if (cls != c) continue;
+                g.rows.push_back(i);
+                g.lds = std::max(g.lds, d.lds_bytes);
+            }
+            if (g.rows.empty()) continue;
+            g.desc_off = ndesc;
+            g.prefix_off = nprefix;
+            int64_t t = 0;
+            for (int64_t i : g.rows) {
+                B->h_prefix.push_back(t);
+                t += per[i][ph].ntiles;
+            }
+            B->h_prefix.push_back(t);
+            g.total_tiles = t;
+            ndesc += (int64_t)g.rows.size();
+            nprefix += (int64_t)g.rows.size() + 1;
+            B->groups.push_back(g);
+        }
+    B->h_descs.resize(ndesc);
+    AFCHK(B->own.alloc((void **)&B->d_descs, std::max<int64_t>(ndesc, 1) * sizeof(AfPass)));
+    AFCHK(B->own.alloc((void **)&B->d_prefix, std::max<int64_t>(nprefix, 1) * 8));
+    if (nprefix) HIPCHK(hipMemcpy(B->d_prefix, B->h_prefix.data(), nprefix * 8, hipMemcpyHostToDevice));
+    // capture the grouped launches once
+    hipStream_t s = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    af_status st = AF_SUCCESS;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) st = AF_HIP_ERROR;
+    for (const auto &g : B->groups) {
+        if (st != AF_SUCCESS) break;
+        if (launch_pass_grouped(B->prec, B->d_descs + g.desc_off, B->d_prefix + g.prefix_off, (int)g.rows.size(),
+                                g.total_tiles, g.nthreads, g.lds, s) != hipSuccess)
+            st = AF_HIP_ERROR;
+    }
+    hipGraph_t graph = nullptr;
+    if (hipStreamEndCapture(s, &graph) != hipSuccess && st == AF_SUCCESS) st = AF_HIP_ERROR;
+    (void)hipStreamDestroy(s);
+    if (st != AF_SUCCESS) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    B->graph = graph;
+    HIPCHK(hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0));
+    // description: algorithm counts
+    std::map<std::string, int64_t> counts;
+    for (int64_t i = 0; i < count; ++i) {
+        if (!lengths[i]) continue;
+        const int a = B->plans[lengths[i]]->shape.algo;
+        const int k = B->plans[lengths[i]]->core.sh.kind;
+        std::string key = a == ALG_RADER ? "rader" : a == ALG_BLUESTEIN ? "bluestein" : a == ALG_COPY ? "copy" : "stockham";
+        key += k == CORE_FOUR ? "/fourstep" : "/single";
+        counts[key]++;
+    }
+    B->text = "batch" + std::to_string(count) + " launches=" + std::to_string(B->groups.size());
+    for (auto &kv : counts) B->text += " " + kv.first + "=" + std::to_string(kv.second);
+    *batch = B.release();
+    return AF_SUCCESS;
+}
+
+af_status af_batch_destroy(af_batch batch) {
+    if (!batch) return AF_INVALID_ARG;
+    delete batch;
+    return AF_SUCCESS;
+}
+
+af_status af_batch_describe(af_batch batch, char *buf, size_t buflen) {
+    if (!batch || !buf || !buflen) return AF_INVALID_ARG;
+    std::snprintf(buf, buflen, "%s", batch->text.c_str());
+    return AF_SUCCESS;
+}
+
+af_status af_batch_execute(af_batch B, const void *in_re, const void *in_im, void *out_re, void *out_im,
+                           void *stream) {
+    if (!B) return AF_INVALID_ARG;
+    if (B->offsets.back() > 0 && (!in_re || !out_re)) return AF_INVALID_ARG;
+    if ((in_im == nullptr) != (out_im == nullptr)) return AF_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const void *cur[4] = {in_re, in_im, out_re, out_im};
+    if (!B->uploaded || std::memcmp(cur, B->last, sizeof cur) != 0) {
+        std::vector<AfPass> ps;
+        for (const auto &g : B->groups) {
+            int64_t j = g.desc_off;
+            for (int64_t i : g.rows) {
+                ps.clear();
+                AFCHK(batch_passes(B, i, in_re, in_im, out_re, out_im, ps));
+                B->h_descs[j++] = ps[g.phase];
+            }
+        }
+        if (!B->h_descs.empty())
+            HIPCHK(hipMemcpyAsync(B->d_descs, B->h_descs.data(), B->h_descs.size() * sizeof(AfPass),
+                                  hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::memcpy(B->last, cur, sizeof cur);
+        B->uploaded = true;
+    }
+    if (B->groups.empty()) return AF_SUCCESS;
+    HIPCHK(hipGraphLaunch(B->exec, s));
+    return AF_SUCCESS;
+}
+
+}  // extern "C"

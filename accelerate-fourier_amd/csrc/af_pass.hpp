@@ -26,6 +26,7 @@ enum AfPassFlags : int32_t {
     AFP_UNTANGLE_ADJ = 1 << 10, // load: transpose of the untangling (adjoint of toSpectrum's packing)
     AFP_ENTANGLE_ADJ = 1 << 11, // store: transpose of the entangling (adjoint of fromSpectrum's packing)
     AFP_SPEC_PLAIN = 1 << 12,   // the convolution spectrum holds plain complex values of the precision
+    AFP_X0_CAPTURE = 1 << 13,   // copy row element 0 of the input to x0_out (Rader, before any store)
 };
 
 struct AfPass {
@@ -80,5 +81,6 @@ struct AfPass {
     const void *x0_im;
     int64_t x0_bs;
     const void *dc_in;
+    void *x0_out;                 // complex (interleaved) per row
     double scale;
 };

@@ -335,6 +335,8 @@ __device__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C
         tl.row[tid] = v ? gl / d.nlines : 0;
         tl.line[tid] = v ? gl - (gl / d.nlines) * d.nlines : 0;
         if (v && (d.flags & AFP_RADER)) s_x0[tid] = ld_c<T>(d.in_re, d.in_im, tl.row[tid] * d.in_bs);
+        if (v && (d.flags & AFP_X0_CAPTURE) && tl.line[tid] == 0)
+            ((C *)d.x0_out)[tl.row[tid]] = ld_c<T>(d.in_re, d.in_im, tl.row[tid] * d.in_bs);
     }
     __syncthreads();
 

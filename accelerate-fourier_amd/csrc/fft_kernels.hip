@@ -229,6 +229,20 @@ __global__ void k_convert(WT *__restrict__ out, const double2 *__restrict__ in, 
 // Host launchers
 // ---------------------------------------------------------------------------
 
+void init_kernel_attributes() {
+    const int lds = 160 * 1024;
+    (void)hipFuncSetAttribute((const void *)k_pass<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_pass<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_pass_grouped<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_pass_grouped<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_real_conv<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_real_conv<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_real_conv_vjp<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_real_conv_vjp<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    init_karatsuba_attributes();
+    (void)hipGetLastError();
+}
+
 static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 template <typename T>

@@ -132,6 +132,10 @@ k_karatsuba(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__restric
     }
 }
 
+void init_karatsuba_attributes() {
+    (void)hipFuncSetAttribute((const void *)k_karatsuba, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
 hipError_t launch_karatsuba(const int64_t *x, const int64_t *y, int64_t *z, const int64_t *dev_meta, int64_t npairs,
                             int max_P, int cyclic, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;

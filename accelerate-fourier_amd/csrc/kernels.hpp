@@ -24,6 +24,10 @@ hipError_t gen_rader_kernel_d(void *out, const int32_t *powg, int64_t p, int64_t
 hipError_t convert_to_w(int prec, void *out, const void *in, int64_t count, double scale, int64_t N1, int64_t N2,
                         hipStream_t s);
 
+// Allow every kernel the full 160 KiB of LDS (called once).
+void init_kernel_attributes();
+void init_karatsuba_attributes();
+
 // Karatsuba (karatsuba.hip)
 hipError_t launch_karatsuba(const int64_t *x, const int64_t *y, int64_t *z, const int64_t *dev_meta, int64_t npairs,
                             int max_P, int cyclic, hipStream_t s);

@@ -1,0 +1,249 @@
+"""Benchmark of the accelerate-fourier transform path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Headline workload: batched power-of-two complex FFT in Complex Float, 64 rows
+of length 65536 (FP32 storage and arithmetic, FP64-generated hi/lo twiddles),
+forward, planar (Accelerate's SoA) layout, inputs N(0,1) from seed 1, resident
+in HBM.  One step = one forward transform of the whole batch.  Prints one JSON
+line: GFLOP/s at 5 N log2 N, the roofline of the dominant kernel (HIP events on
+the stream the kernels run on), the CPU baseline (the oracle, a C restatement
+of the reference's algorithm, on host cores), and side measurements (plain
+FP64 path, 2^24, the FP64 256 x 1024 case, the real convolution with its
+adjoint, Karatsuba, and the error of the FP32 path against FP64).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(REPO, "accelerate-fourier_amd"), os.path.join(REPO, "oracle")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "FP32 batched FFT GFLOP/s(5NlgN) & %HBM peak vs FP64 plain; fwd+bwd conv/s"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--skip-extras", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    return ap.parse_args()
+
+
+class C2C:
+    """A C2C plan called straight through the ABI with a preallocated workspace."""
+
+    def __init__(self, n, batch, prec, sign=-1, planar=True):
+        import torch
+        from accelerate_fourier import _abi, planned
+
+        self.torch, self.abi = torch, _abi
+        self.n, self.batch, self.prec, self.planar = n, batch, prec, planar
+        self.plan = planned.Plan(n, sign, _abi.C2C, prec)
+        ws = self.plan.workspace_size(batch)
+        self.ws = torch.empty(max(ws, 16), dtype=torch.uint8, device="cuda")
+        cnt = ctypes.c_int32()
+        _abi.check(_abi.lib().af_plan_num_passes(self.plan.handle, batch, ctypes.byref(cnt)), "num_passes")
+        self.npasses = cnt.value
+
+    def set_io(self, x):
+        """x: complex numpy array (batch, n)."""
+        torch = self.torch
+        rt = torch.float64 if self.prec == self.abi.F64 else torch.float32
+        ct = torch.complex128 if self.prec == self.abi.F64 else torch.complex64
+        if self.planar:
+            self.xin = (torch.from_numpy(x.real.copy()).to("cuda", rt), torch.from_numpy(x.imag.copy()).to("cuda", rt))
+            self.xout = (torch.empty_like(self.xin[0]), torch.empty_like(self.xin[1]))
+            self.ptrs = (self.xin[0].data_ptr(), self.xin[1].data_ptr(), self.xout[0].data_ptr(), self.xout[1].data_ptr())
+        else:
+            self.xin = torch.from_numpy(x).to("cuda", ct)
+            self.xout = torch.empty_like(self.xin)
+            self.ptrs = (self.xin.data_ptr(), None, self.xout.data_ptr(), None)
+
+    def run(self):
+        s = self.torch.cuda.current_stream().cuda_stream
+        a = self.ptrs
+        st = self.abi.lib().af_execute(self.plan.handle, self.batch, a[0], a[1], a[2], a[3], self.ws.data_ptr(), s)
+        self.abi.check(st, "af_execute")
+
+    def run_pass(self, i):
+        s = self.torch.cuda.current_stream().cuda_stream
+        a = self.ptrs
+        st = self.abi.lib().af_execute_pass(self.plan.handle, i, self.batch, a[0], a[1], a[2], a[3],
+                                            self.ws.data_ptr(), s)
+        self.abi.check(st, "af_execute_pass")
+
+    def result(self):
+        self.torch.cuda.synchronize()
+        if self.planar:
+            return self.xout[0].cpu().numpy() + 1j * self.xout[1].cpu().numpy()
+        return self.xout.cpu().numpy()
+
+
+def time_it(fn, steps, warmup):
+    """Average ms per call, HIP events on the current stream around `steps` calls."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def rel_l2(a, b):
+    import numpy as np
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU: every box has one MI355X, and the "
+                                   "transform path shards only across independent rows (see DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from accelerate_fourier import _abi, synthetic
+
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
+    torch.cuda.init()
+    steps, warmup = args.steps, args.warmup
+    n, batch = 65536, 64
+    flops = 5.0 * n * math.log2(n) * batch
+    alg_bytes = 2 * 8 * n * batch  # one read and one write of the C32 array
+
+    x = synthetic.complex_normal(1, (batch, n))
+    f32 = C2C(n, batch, _abi.F32)
+    f32.set_io(x)
+    ms = time_it(f32.run, steps, warmup)
+    pass_ms = [time_it(lambda i=i: f32.run_pass(i), steps, warmup) for i in range(f32.npasses)]
+    dom = int(np.argmax(pass_ms))
+    # every pass of the four-step transform reads and writes the whole array once
+    achieved = alg_bytes / (pass_ms[dom] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": flops / (ms * 1e-3) / 1e9,
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: N(0,1) re/im from splitmix64 seed 1, resident in HBM",
+        "config": {"workload": "batched C2C FFT, Complex Float, 64 x 65536, forward, planar layout",
+                   "batch": batch, "length": n, "plan": f"{f32.npasses} passes"},
+        "hbm_GBps_end_to_end": alg_bytes / (ms * 1e-3) / 1e9,
+        "hbm_frac_end_to_end": alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_pass<float> pass {dom} of {f32.npasses}",
+                     "pass_ms": pass_ms},
+    }
+    y32 = f32.result()
+
+    if not args.skip_extras:
+        extras = {}
+        # FP32 error against the FP64 path and the oracle on sampled rows
+        f64 = C2C(n, batch, _abi.F64)
+        f64.set_io(x)
+        ms64 = time_it(f64.run, steps, warmup)
+        y64 = f64.result()
+        extras["fp32_rel_l2_vs_fp64"] = rel_l2(y32, y64)
+        extras["fp32_max_abs_vs_fp64"] = float(np.max(np.abs(y32 - y64)))
+        import oracle
+        xs = x[:2]
+        extras["fp64_rel_l2_vs_oracle_2rows"] = rel_l2(y64[:2], oracle.transform(-1, xs))
+        extras["fp64_same_workload_ms"] = ms64
+        extras["fp32_over_fp64_speedup"] = ms64 / ms
+        del f64
+        # one 2^24 transform
+        n24 = 1 << 24
+        x24 = synthetic.complex_normal(1, (1, n24))
+        g = C2C(n24, 1, _abi.F32)
+        g.set_io(x24)
+        ms24 = time_it(g.run, max(steps // 2, 3), 2)
+        b24 = 16.0 * n24
+        extras["c2b_2p24_f32_ms"] = ms24
+        extras["c2b_2p24_f32_GFLOPs"] = 5.0 * n24 * 24 / (ms24 * 1e-3) / 1e9
+        extras["c2b_2p24_hbm_frac"] = b24 / (ms24 * 1e-3) / 1e9 / HBM_PEAK_GBS
+        y24 = g.result()
+        extras["c2b_2p24_rel_l2_vs_numpy_fp64"] = rel_l2(y24[0], np.fft.fft(x24[0]))
+        del g
+        # plain FP64, 256 x 1024
+        x1 = synthetic.complex_uniform(0, (256, 1024))
+        c1 = C2C(1024, 256, _abi.F64)
+        c1.set_io(x1)
+        ms1 = time_it(c1.run, steps, warmup)
+        extras["c1_f64_256x1024_us"] = ms1 * 1e3
+        extras["c1_f64_GBps"] = 2 * 16 * 1024 * 256 / (ms1 * 1e-3) / 1e9
+        # real convolution and its adjoint, FP32 512 x 8192
+        from accelerate_fourier import real
+        N = 8192
+        xr = torch.from_numpy(synthetic.normal(3, 512 * N).reshape(512, N)).to("cuda", torch.float32)
+        kr = torch.from_numpy(synthetic.normal(4, 512 * N).reshape(512, N) / math.sqrt(N)).to("cuda", torch.float32)
+        wr = torch.from_numpy(synthetic.normal(5, 512 * N).reshape(512, N)).to("cuda", torch.float32)
+        yr, dx, dk = torch.empty_like(xr), torch.empty_like(xr), torch.empty_like(xr)
+
+        def fb():
+            real.convolve(xr, kr, out=yr)
+            real.convolve_vjp(xr, kr, wr, dx, dk)
+        msc = time_it(fb, steps, warmup)
+        extras["c4_conv_fwd_bwd_ms"] = msc
+        extras["c4_conv_per_s"] = 512 / (msc * 1e-3)
+        extras["c4_hbm_frac"] = 32 * 512 * N / (msc * 1e-3) / 1e9 / HBM_PEAK_GBS
+        # Karatsuba, 1024 pairs, lengths U[256, 4096]
+        from accelerate_fourier import convolution
+        lens = synthetic.integers(4, 1024, 256, 4097)
+        off = np.concatenate([[0], np.cumsum(lens)])
+        xa = torch.from_numpy(synthetic.integers(40, int(off[-1]), -2 ** 15, 2 ** 15)).to("cuda")
+        ya = torch.from_numpy(synthetic.integers(41, int(off[-1]), -2 ** 15, 2 ** 15)).to("cuda")
+        za = torch.empty(2 * int(off[-1]) - 1024, dtype=torch.int64, device="cuda")
+        msk = time_it(lambda: convolution.karatsuba_ragged(off, xa, ya, out=za), max(steps // 4, 3), 1)
+        extras["c5_karatsuba_ms"] = msk
+        extras["c5_pairs_per_s"] = 1024 / (msk * 1e-3)
+        extras["c5_schoolbook_equiv_GMACs"] = float(np.sum(lens.astype(np.float64) ** 2)) / (msk * 1e-3) / 1e9
+        out["extras"] = extras
+
+    if not args.skip_cpu:
+        import oracle
+        threads = oracle.num_threads()
+        rows = 16
+        xs = x[:rows]
+        t0 = time.perf_counter()
+        reps = 0
+        while True:
+            oracle.transform(-1, xs, precision="f32")
+            reps += 1
+            if time.perf_counter() - t0 > 10.0:
+                break
+        dt = (time.perf_counter() - t0) / reps
+        out["cpu_baseline"] = {"value": 5.0 * n * math.log2(n) * rows / dt / 1e9, "unit": "GFLOP/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{rows} of the 64 rows x 65536, FP32 oracle (C restatement of the "
+                                         f"reference's split-radix plan), {reps} repetitions over ~10 s"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

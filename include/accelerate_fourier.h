@@ -109,6 +109,18 @@ af_status af_execute(af_plan plan, int64_t batch,
                      void *workspace, void *stream);
 
 /*
+ * The launches of one af_execute call, one at a time, for profiling and for
+ * callers that interleave other work between them.  af_plan_num_passes gives
+ * the count for `batch` rows; pass i of af_execute_pass runs launch i of the
+ * same sequence (workspace required when af_plan_workspace_size is not 0).
+ * No reference counterpart: the reference has no notion of launches.
+ */
+af_status af_plan_num_passes(af_plan plan, int64_t batch, int32_t *npasses);
+af_status af_execute_pass(af_plan plan, int32_t index, int64_t batch,
+                          const void *in_re, const void *in_im,
+                          void *out_re, void *out_im, void *workspace, void *stream);
+
+/*
  * Real signal to full spectrum (AF_R2C_FULL plans).
  * Replaces: Real.toSpectrum (Planned.transform sign (n/2)) (Real.hs:49-70).
  * x: batch x n reals; out: batch x n complex (the full, not halved, spectrum).

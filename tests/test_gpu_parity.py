@@ -1,0 +1,180 @@
+"""GPU parity of the transform path against the oracle (the C restatement of
+the reference's algorithms), through the C ABI.
+
+Tolerances: FP64 relative L2 <= 1e-12 against the FP64 oracle; FP32 relative
+L2 <= 1e-6 against the FP64 oracle (the FP32 path's target); Int64 bit-exact.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from accelerate_fourier import _abi, convolution, planned, real  # noqa: E402
+from accelerate_fourier.sign import forward, inverse, Sign  # noqa: E402
+from accelerate_fourier import synthetic  # noqa: E402
+
+DEV = "cuda"
+
+
+def rel_l2(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def to_dev(x, dtype):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(DEV, dtype)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+# lengths covering every device plan: single-pass Stockham (all radices),
+# four-step, Rader (single and four-step core) and Bluestein (single, four-step)
+LENGTHS_SMALL = list(range(0, 41)) + [48, 60, 64, 77, 97, 100, 121, 127, 128, 169, 255, 256, 257]
+LENGTHS_MID = [1000, 1024, 2310, 4096, 4099, 8191, 8192, 10007, 15015, 16384]
+LENGTHS_BIG = [16385 * 0 + 20000, 32768, 65521, 65536, 100003]
+
+
+def _check_c2c(n, sign, prec, layout, rows=3, seed=7):
+    x = synthetic.complex_uniform(seed + n, (rows, n))
+    ref = orc_transform(sign, x)
+    ct = torch.complex128 if prec == "f64" else torch.complex64
+    rt = torch.float64 if prec == "f64" else torch.float32
+    t = planned.transform(Sign(sign), n)
+    if layout == "planar":
+        xd = (to_dev(x.real, rt), to_dev(x.imag, rt))
+        y = t(xd)
+        got = host(y[0]) + 1j * host(y[1])
+    else:
+        y = t(to_dev(x, ct))
+        got = host(y)
+    tol = tol_f64(n) if prec == "f64" else 1e-6
+    if n == 0:
+        assert got.shape == (rows, 0)
+        return
+    err = rel_l2(got, ref)
+    assert err <= tol, f"n={n} sign={sign} {prec} {layout}: rel-L2 {err:.3e} ({planned.describe(n)})"
+
+
+def tol_f64(n):
+    """The oracle follows the reference's Bluestein chirp, whose phase pi k^2/n
+    is not reduced modulo 2n (Private.hs:428-436): its own FP64 error grows
+    like eps * n on chirp plans, so the bound widens by that much there."""
+    import oracle
+    return 1e-12 + (4e-15 * n if oracle.plan_string(n).startswith("chirp") else 0.0)
+
+
+def orc_transform(sign, x):
+    import oracle
+    return oracle.transform(sign, x)
+
+
+@pytest.mark.parametrize("n", LENGTHS_SMALL)
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_c2c_small_f64(n, sign):
+    _check_c2c(n, sign, "f64", "interleaved")
+
+
+@pytest.mark.parametrize("n", LENGTHS_SMALL[::3])
+def test_c2c_small_f32_planar(n):
+    _check_c2c(n, -1, "f32", "planar")
+
+
+@pytest.mark.parametrize("n", LENGTHS_MID)
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_c2c_mid(n, prec):
+    _check_c2c(n, -1, prec, "interleaved", rows=2)
+    _check_c2c(n, 1, prec, "planar", rows=2)
+
+
+@pytest.mark.parametrize("n", LENGTHS_BIG)
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_c2c_big(n, prec):
+    _check_c2c(n, -1, prec, "interleaved", rows=2)
+
+
+def test_config1_plain_f64():
+    """256 x 1024 Complex Double, U[-1,1) from seed 0: forward spectrum and
+    round trip within 1e-12 relative L2 of the FP64 oracle."""
+    import oracle
+    x = synthetic.complex_uniform(0, (256, 1024))
+    xd = to_dev(x, torch.complex128)
+    X = planned.transform(forward, 1024)(xd)
+    back = planned.transform(inverse, 1024)(X)
+    assert rel_l2(host(X), oracle.transform(-1, x)) <= 1e-12
+    assert rel_l2(host(back) / 1024, x) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [2, 4, 6, 8, 10, 16, 30, 64, 100, 1024, 8192])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_real_spectrum(n, prec):
+    import oracle
+    rt = torch.float64 if prec == "f64" else torch.float32
+    tol = 1e-12 if prec == "f64" else 1e-6
+    x = synthetic.normal(11 + n, 3 * n).reshape(3, n)
+    S = real.to_spectrum(planned.transform(forward, n // 2), to_dev(x, rt))
+    ref = oracle.to_spectrum(x)
+    assert rel_l2(host(S), ref) <= tol
+    y = real.from_spectrum(planned.transform(inverse, n // 2), to_dev(ref, S.dtype))
+    assert rel_l2(host(y), oracle.from_spectrum(ref)) <= tol
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_real_convolution(prec):
+    import oracle
+    rt = torch.float64 if prec == "f64" else torch.float32
+    tol = 1e-12 if prec == "f64" else 1e-6
+    n = 1024
+    x = synthetic.normal(3, 4 * n).reshape(4, n)
+    k = synthetic.normal(4, 4 * n).reshape(4, n) / np.sqrt(n)
+    y = real.convolve(to_dev(x, rt), to_dev(k, rt))
+    ref = oracle.from_spectrum(oracle.to_spectrum(x) * oracle.to_spectrum(k))
+    assert rel_l2(host(y), ref) <= tol
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 100, 97, 1024, 20000])
+def test_convolve_cyclic(n):
+    import oracle
+    x = synthetic.complex_uniform(20 + n, (2, n))
+    y = synthetic.complex_uniform(30 + n, (2, n))
+    z = planned.convolve_cyclic(n)(to_dev(x, torch.complex128), to_dev(y, torch.complex128))
+    assert rel_l2(host(z), oracle.convolve_cyclic(x, y)) <= 1e-12
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 5, 64, 65, 100, 256, 1000, 4096])
+def test_karatsuba_bit_exact(L):
+    import oracle
+    x = synthetic.integers(40 + L, 3 * L, -2 ** 15, 2 ** 15).reshape(3, L)
+    y = synthetic.integers(50 + L, 3 * L, -2 ** 15, 2 ** 15).reshape(3, L)
+    z = convolution.karatsuba(to_dev(x, torch.int64), to_dev(y, torch.int64))
+    assert np.array_equal(host(z), oracle.schoolbook(x, y))
+
+
+def test_karatsuba_wraparound():
+    import oracle
+    L = 777
+    x = synthetic.integers(5, L, -2 ** 62, 2 ** 62).reshape(1, L)
+    y = synthetic.integers(6, L, -2 ** 62, 2 ** 62).reshape(1, L)
+    z = convolution.karatsuba(to_dev(x, torch.int64), to_dev(y, torch.int64))
+    assert np.array_equal(host(z), oracle.karatsuba(x, y))
+
+
+def test_uneven_batch():
+    import oracle
+    lengths = [64, 1, 0, 60, 97, 257, 1000, 4099, 20000, 65536, 17, 65521, 100003, 5]
+    off = np.concatenate([[0], np.cumsum(lengths)])
+    x = synthetic.complex_uniform(2, (int(off[-1]),))
+    from accelerate_fourier.batch import UnevenBatch
+    b = UnevenBatch(lengths, forward, "f64")
+    y = host(b(to_dev(x, torch.complex128)))
+    for i, n in enumerate(lengths):
+        if n == 0:
+            continue
+        ref = oracle.transform(-1, x[off[i]:off[i + 1]])
+        assert rel_l2(y[off[i]:off[i + 1]], ref) <= tol_f64(n), f"row {i} length {n}"
