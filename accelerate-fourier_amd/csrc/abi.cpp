@@ -89,8 +89,10 @@ struct Core {
     CoreShape sh;
     void *tw = nullptr;                 // single: W_N^m, m < N
     void *tw1 = nullptr, *tw2 = nullptr;  // four-step: W_N1, W_N2
-    void *ft_coarse = nullptr, *ft_fine = nullptr;  // double2, four-step twiddles W_N
+    void *ft_coarse = nullptr, *ft_fine = nullptr;  // four-step twiddles W_N, table format
     int ftShift = 0;
+    // FP64 copies (the same tables for FP64 plans)
+    void *tw_d = nullptr, *tw1_d = nullptr, *tw2_d = nullptr, *ft_coarse_d = nullptr, *ft_fine_d = nullptr;
 };
 
 af_status build_core(int prec, const CoreShape &sh, Core &c, Owner &own, hipStream_t s) {
@@ -109,8 +111,31 @@ af_status build_core(int prec, const CoreShape &sh, Core &c, Owner &own, hipStre
         const int64_t T = (int64_t)1 << shift, nc = (sh.N + T - 1) / T;
         AFCHK(own.alloc(&c.ft_coarse, (size_t)nc * 16));
         AFCHK(own.alloc(&c.ft_fine, (size_t)T * 16));
-        HIPCHK(gen_twiddles_d(c.ft_coarse, sh.N, nc, T, s));
-        HIPCHK(gen_twiddles_d(c.ft_fine, sh.N, T, 1, s));
+        HIPCHK(gen_twiddles(prec, c.ft_coarse, sh.N, nc, T, s));
+        HIPCHK(gen_twiddles(prec, c.ft_fine, sh.N, T, 1, s));
+    }
+    if (prec == PREC_F64) {
+        c.tw_d = c.tw;
+        c.tw1_d = c.tw1;
+        c.tw2_d = c.tw2;
+        c.ft_coarse_d = c.ft_coarse;
+        c.ft_fine_d = c.ft_fine;
+        return AF_SUCCESS;
+    }
+    // FP32 plans keep FP64 copies for the power-of-two kernel's twiddle recurrences
+    auto dtab = [&](void **out, int64_t N, int64_t count, int64_t mul) -> af_status {
+        AFCHK(own.alloc(out, (size_t)count * 16));
+        HIPCHK(gen_twiddles_d(*out, N, count, mul, s));
+        return AF_SUCCESS;
+    };
+    if (sh.kind == CORE_SINGLE) {
+        AFCHK(dtab(&c.tw_d, sh.N, sh.N, 1));
+    } else if (sh.kind == CORE_FOUR) {
+        const int64_t T = (int64_t)1 << c.ftShift, nc = (sh.N + T - 1) / T;
+        AFCHK(dtab(&c.tw1_d, sh.N1, sh.N1, 1));
+        AFCHK(dtab(&c.tw2_d, sh.N2, sh.N2, 1));
+        AFCHK(dtab(&c.ft_coarse_d, sh.N, nc, T));
+        AFCHK(dtab(&c.ft_fine_d, sh.N, T, 1));
     }
     return AF_SUCCESS;
 }
@@ -141,9 +166,10 @@ int choose_B_rows(int prec, int64_t L, int64_t nlines_total) {
     return B;
 }
 
-// Lines per tile for passes whose loads or stores are strided: 128-byte runs.
-int choose_B_cols(int prec, int64_t L) {
-    int B = prec == AF_F64 ? 8 : 16;
+// Lines per tile for passes whose loads or stores are strided: runs of 128
+// bytes per component array (planar layouts need twice the lines).
+int choose_B_cols(int prec, int64_t L, bool planar = false) {
+    int B = (prec == AF_F64 ? 8 : 16) * (planar ? 2 : 1);
     while (B > 1 && !tile_fits(prec, (int)L, B)) B /= 2;
     return B;
 }
@@ -154,8 +180,8 @@ struct Endpoint {
     int64_t bs = 0;            // row stride in elements
 };
 
-AfPass base_pass(int prec, int L, const std::vector<int> &radices, const void *tw, int B, int64_t nlines,
-                 int64_t nrows, int sign, bool plain) {
+AfPass base_pass(int prec, int L, const std::vector<int> &radices, const void *tw, const void *tw_d, int B,
+                 int64_t nlines, int64_t nrows, int sign, bool plain) {
     AfPass d;
     std::memset(&d, 0, sizeof d);
     d.L = L;
@@ -178,6 +204,7 @@ AfPass base_pass(int prec, int L, const std::vector<int> &radices, const void *t
     d.nthreads = nt;
     d.lds_bytes = lds_bytes_for(prec, L, B, d.padshift);
     d.tw = tw;
+    d.tw_d = tw_d;
     d.sign = sign;
     d.flags = plain ? AFP_PLAIN_TW : 0;
     d.in_len = INT64_MAX;
@@ -208,6 +235,8 @@ void set_ft(AfPass &d, const Core &c, int sign) {
     d.ftSign = sign;
     d.ft_coarse = c.ft_coarse;
     d.ft_fine = c.ft_fine;
+    d.ft_coarse_d = c.ft_coarse_d;
+    d.ft_fine_d = c.ft_fine_d;
 }
 
 // Plain transform of `nrows` rows (natural order in and out).  For the
@@ -216,10 +245,13 @@ void set_ft(AfPass &d, const Core &c, int sign) {
 void c2c_passes(int prec, const Core &c, int64_t nrows, int sign, bool plain, const Endpoint &in,
                 const Endpoint &out, void *scratch, bool row_order_out, std::vector<AfPass> &ps) {
     if (c.sh.kind == CORE_SINGLE) {
+        // rows are the lines of one long "row": tiles of B rows need no row split
         const int L = (int)c.sh.N;
-        AfPass d = base_pass(prec, L, c.sh.r, c.tw, choose_B_rows(prec, L, nrows), 1, nrows, sign, plain);
-        set_in(d, in, 1, 0);
-        set_out(d, out, 1, 0);
+        AfPass d = base_pass(prec, L, c.sh.r, c.tw, c.tw_d, choose_B_rows(prec, L, nrows), nrows, 1, sign, plain);
+        Endpoint i2 = in, o2 = out;
+        set_in(d, i2, 1, in.bs);
+        set_out(d, o2, 1, out.bs);
+        d.in_bs = d.out_bs = 0;
         ps.push_back(d);
         return;
     }
@@ -227,13 +259,15 @@ void c2c_passes(int prec, const Core &c, int64_t nrows, int sign, bool plain, co
     Endpoint sc;
     sc.re = scratch;
     sc.bs = N;
-    AfPass a = base_pass(prec, (int)N1, c.sh.r1, c.tw1, choose_B_cols(prec, N1), N2, nrows, sign, plain);
+    AfPass a = base_pass(prec, (int)N1, c.sh.r1, c.tw1, c.tw1_d, choose_B_cols(prec, N1, in.im != nullptr), N2, nrows, sign,
+                         plain);
     set_in(a, in, N2, 1);
     set_out(a, sc, N2, 1);
     a.in_lines_fast = a.out_lines_fast = 1;
     set_ft(a, c, sign);
     ps.push_back(a);
-    AfPass b = base_pass(prec, (int)N2, c.sh.r2, c.tw2, choose_B_cols(prec, N2), N1, nrows, sign, plain);
+    AfPass b = base_pass(prec, (int)N2, c.sh.r2, c.tw2, c.tw2_d,
+                         choose_B_cols(prec, N2, out.im != nullptr && !row_order_out), N1, nrows, sign, plain);
     set_in(b, sc, 1, N2);
     if (row_order_out) {
         set_out(b, out, 1, N2);
@@ -253,7 +287,7 @@ void conv_passes(int prec, const Core &c, int64_t nrows, bool plain, const Endpo
     const int spf = spec_plain ? AFP_SPEC_PLAIN : 0;
     if (c.sh.kind == CORE_SINGLE) {
         const int L = (int)c.sh.N;
-        AfPass d = base_pass(prec, L, c.sh.r, c.tw, choose_B_rows(prec, L, nrows), 1, nrows, -1, plain);
+        AfPass d = base_pass(prec, L, c.sh.r, c.tw, c.tw_d, choose_B_rows(prec, L, nrows), 1, nrows, -1, plain);
         set_in(d, in, 1, 0);
         set_out(d, out, 1, 0);
         d.flags |= AFP_CONV | spf;
@@ -267,13 +301,13 @@ void conv_passes(int prec, const Core &c, int64_t nrows, bool plain, const Endpo
     Endpoint sc;
     sc.re = scratch;
     sc.bs = N;
-    AfPass a = base_pass(prec, (int)N1, c.sh.r1, c.tw1, choose_B_cols(prec, N1), N2, nrows, -1, plain);
+    AfPass a = base_pass(prec, (int)N1, c.sh.r1, c.tw1, c.tw1_d, choose_B_cols(prec, N1), N2, nrows, -1, plain);
     set_in(a, in, N2, 1);
     set_out(a, sc, N2, 1);
     a.in_lines_fast = a.out_lines_fast = 1;
     set_ft(a, c, -1);
     ps.push_back(a);
-    AfPass b = base_pass(prec, (int)N2, c.sh.r2, c.tw2, choose_B_rows(prec, N2, nrows * N1), N1, nrows, -1, plain);
+    AfPass b = base_pass(prec, (int)N2, c.sh.r2, c.tw2, c.tw2_d, choose_B_rows(prec, N2, nrows * N1), N1, nrows, -1, plain);
     set_in(b, sc, 1, N2);
     set_out(b, sc, 1, N2);
     b.flags |= AFP_CONV | spf;
@@ -282,7 +316,7 @@ void conv_passes(int prec, const Core &c, int64_t nrows, bool plain, const Endpo
     b.spec_ls = N2;
     set_ft(b, c, +1);
     ps.push_back(b);
-    AfPass cc = base_pass(prec, (int)N1, c.sh.r1, c.tw1, choose_B_cols(prec, N1), N2, nrows, +1, plain);
+    AfPass cc = base_pass(prec, (int)N1, c.sh.r1, c.tw1, c.tw1_d, choose_B_cols(prec, N1), N2, nrows, +1, plain);
     set_in(cc, sc, N2, 1);
     set_out(cc, out, N2, 1);
     cc.in_lines_fast = cc.out_lines_fast = 1;
@@ -630,7 +664,7 @@ af_status build_c2c(af_plan P, int sign, int64_t batch, const Endpoint &in, cons
     if (n == 0 || batch == 0) return AF_SUCCESS;
     switch (P->shape.algo) {
     case ALG_COPY: {
-        AfPass d = base_pass(prec, 1, {}, nullptr, 64, 1, batch, sign, P->plain);
+        AfPass d = base_pass(prec, 1, {}, nullptr, nullptr, 64, 1, batch, sign, P->plain);
         set_in(d, in, 1, 0);
         set_out(d, out, 1, 0);
         ps.push_back(d);
@@ -748,7 +782,7 @@ af_status af_to_spectrum(af_plan plan, int64_t batch, const void *x, void *out_r
     const int64_t n = plan->desc.length, n2 = n / 2;
     if (n == 0 || batch == 0) return AF_SUCCESS;
     if (bad_ptrs(x, out_re)) return AF_INVALID_ARG;
-    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, choose_B_rows(plan->prec, n2, batch), 1,
+    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, plan->core.tw_d, choose_B_rows(plan->prec, n2, batch), 1,
                          batch, plan->desc.sign, plan->plain);
     set_in(d, ep(x, nullptr, n2), 1, 0);
     set_out(d, ep(out_re, out_im, n), 1, 0);
@@ -765,7 +799,7 @@ af_status af_from_spectrum(af_plan plan, int64_t batch, const void *in_re, const
     const int64_t n = plan->desc.length, n2 = n / 2;
     if (n == 0 || batch == 0) return AF_SUCCESS;
     if (bad_ptrs(in_re, y)) return AF_INVALID_ARG;
-    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, choose_B_rows(plan->prec, n2, batch), 1,
+    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, plan->core.tw_d, choose_B_rows(plan->prec, n2, batch), 1,
                          batch, plan->desc.sign, plan->plain);
     set_in(d, ep(in_re, in_im, n), 1, 0);
     set_out(d, ep(y, nullptr, n2), 1, 0);
@@ -783,7 +817,7 @@ af_status af_vjp_to_spectrum(af_plan plan, int64_t batch, const void *g_re, cons
     const int64_t n = plan->desc.length, n2 = n / 2;
     if (n == 0 || batch == 0) return AF_SUCCESS;
     if (bad_ptrs(g_re, dx)) return AF_INVALID_ARG;
-    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, choose_B_rows(plan->prec, n2, batch), 1,
+    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, plan->core.tw_d, choose_B_rows(plan->prec, n2, batch), 1,
                          batch, +1, plan->plain);
     set_in(d, ep(g_re, g_im, n), 1, 0);
     set_out(d, ep(dx, nullptr, n2), 1, 0);
@@ -800,7 +834,7 @@ af_status af_vjp_from_spectrum(af_plan plan, int64_t batch, const void *g, void 
     const int64_t n = plan->desc.length, n2 = n / 2;
     if (n == 0 || batch == 0) return AF_SUCCESS;
     if (bad_ptrs(g, out_re)) return AF_INVALID_ARG;
-    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, choose_B_rows(plan->prec, n2, batch), 1,
+    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, plan->core.tw_d, choose_B_rows(plan->prec, n2, batch), 1,
                          batch, -plan->desc.sign, plan->plain);
     set_in(d, ep(g, nullptr, n2), 1, 0);
     set_out(d, ep(out_re, out_im, n), 1, 0);
@@ -812,7 +846,7 @@ af_status af_vjp_from_spectrum(af_plan plan, int64_t batch, const void *g, void 
 
 static AfPass real_conv_geom(af_plan plan, int64_t batch) {
     const int64_t n2 = plan->desc.length / 2;
-    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, 1, 1, batch, -1, plan->plain);
+    AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, plan->core.tw_d, 1, 1, batch, -1, plan->plain);
     d.aux_tw = plan->aux_tw;
     d.lds_bytes = 2 * (pidx_h(d.ld, d.padshift) + 1) * elem_bytes(plan->prec);
     return d;
@@ -853,7 +887,7 @@ af_status af_convolve_cyclic(af_plan plan, int64_t batch, const void *x_re, cons
         void *s1 = sc.p, *s2 = (char *)sc.p + (size_t)batch * n * eb;
         AFCHK(af_execute(plan->sub, batch, x_re, x_im, s1, nullptr, nullptr, s));
         AFCHK(af_execute(plan->sub, batch, y_re, y_im, s2, nullptr, nullptr, s));
-        AfPass m = base_pass(prec, 1, {}, nullptr, 64, n, batch, -1, plan->plain);
+        AfPass m = base_pass(prec, 1, {}, nullptr, nullptr, 64, n, batch, -1, plan->plain);
         set_in(m, ep(s1, nullptr, n), 0, 1);
         set_out(m, ep(s1, nullptr, n), 0, 1);
         m.flags |= AFP_CONV | AFP_SPEC_PLAIN | AFP_SCALE;

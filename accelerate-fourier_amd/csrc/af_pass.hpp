@@ -82,5 +82,10 @@ struct AfPass {
     int64_t x0_bs;
     const void *dc_in;
     void *x0_out;                 // complex (interleaved) per row
+    // FP64 copies of the stage and four-step twiddle tables (double2), read by
+    // the power-of-two kernel once per butterfly
+    const void *tw_d;
+    const void *ft_coarse_d;
+    const void *ft_fine_d;
     double scale;
 };

@@ -299,13 +299,42 @@ __device__ __forceinline__ void st_c(void *re, void *im, int64_t a, typename Cx<
     }
 }
 
-// FP64 value of the four-step twiddle W_N^(sign * a), a < N.
-__device__ __forceinline__ double2 ft_twiddle(const AfPass &d, int64_t a) {
+// Four-step twiddle W_N^(sign * a), a < N, as the product coarse[a >> shift] *
+// fine[a & mask] of two table entries.  FP64: plain double product.  FP32: the
+// hi parts' product plus the first-order lo corrections, accurate to about one
+// FP32 ulp.
+__device__ __forceinline__ double2 ft_twiddle_t(const AfPass &d, uint64_t a, double) {
     const double2 c = ((const double2 *)d.ft_coarse)[a >> d.ftShift];
-    const double2 f = ((const double2 *)d.ft_fine)[a & ((int64_t(1) << d.ftShift) - 1)];
+    const double2 f = ((const double2 *)d.ft_fine)[a & ((uint64_t(1) << d.ftShift) - 1)];
     double2 w = cmul(c, f);
     if (d.ftSign > 0) w.y = -w.y;
     return w;
+}
+// FP64 four-step twiddle from the FP64 tables (both precisions).
+__device__ __forceinline__ double2 ft_twiddle_d(const AfPass &d, uint64_t a) {
+    const double2 c = ((const double2 *)d.ft_coarse_d)[a >> d.ftShift];
+    const double2 f = ((const double2 *)d.ft_fine_d)[a & ((uint64_t(1) << d.ftShift) - 1)];
+    double2 w = cmul(c, f);
+    if (d.ftSign > 0) w.y = -w.y;
+    return w;
+}
+__device__ __forceinline__ float2 ft_twiddle_t(const AfPass &d, uint64_t a, float) {
+    const float4 c = ((const float4 *)d.ft_coarse)[a >> d.ftShift];
+    const float4 f = ((const float4 *)d.ft_fine)[a & ((uint64_t(1) << d.ftShift) - 1)];
+    float re = fmaf(c.x, f.z, fmaf(-c.y, f.w, fmaf(c.z, f.x, -c.w * f.y)));
+    float im = fmaf(c.x, f.w, fmaf(c.y, f.z, fmaf(c.z, f.y, c.w * f.x)));
+    re = fmaf(c.x, f.x, fmaf(-c.y, f.y, re));
+    im = fmaf(c.x, f.y, fmaf(c.y, f.x, im));
+    return make_float2(re, d.ftSign > 0 ? -im : im);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b mod 8), and each XCD has its own L2.  Give each XCD a contiguous range of
+// tiles instead, so neighbouring tiles -- which share 128-byte lines when a
+// tile is narrower than a line -- meet in the same L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t G) {
+    const int64_t q = G >> 3, r = G & 7, x = b & 7, k = b >> 3;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
 struct TileLines {
@@ -469,8 +498,8 @@ __device__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C
         if (d.flags & AFP_POSTTW) {
             // l * e < 2^62 for every supported N, so the product is exact in 64 bits
             const uint64_t le = (uint64_t)l * (uint64_t)e, N = (uint64_t)d.ftN;
-            const int64_t a = (int64_t)(((N & (N - 1)) == 0) ? (le & (N - 1)) : (le % N));
-            v = dmul(v, ft_twiddle(d, a), plain);
+            const uint64_t a = ((N & (N - 1)) == 0) ? (le & (N - 1)) : (le % N);
+            v = cmul(v, ft_twiddle_t(d, a, T(0)));
         }
         if (d.out_mul) v = vmul(v, ((const W *)d.out_mul)[o], plain);
         if (d.flags & AFP_SCALE) v = cscale(v, T(d.scale));

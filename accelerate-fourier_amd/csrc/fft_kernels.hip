@@ -22,6 +22,10 @@ __global__ void __launch_bounds__(1024) k_pass(const AfPass d) {
     __shared__ TileLines tl;
     __shared__ typename Cx<T>::C s_x0[64], s_dc[64];
     auto *lds = (typename Cx<T>::C *)smem;
+    if ((int64_t)gridDim.x == d.ntiles) {
+        pass_tile<T>(d, xcd_tile(blockIdx.x, gridDim.x), lds, tl, s_x0, s_dc);
+        return;
+    }
     for (int64_t tile = blockIdx.x; tile < d.ntiles; tile += gridDim.x)
         pass_tile<T>(d, tile, lds, tl, s_x0, s_dc);
 }
@@ -240,6 +244,7 @@ void init_kernel_attributes() {
     (void)hipFuncSetAttribute((const void *)k_real_conv_vjp<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     (void)hipFuncSetAttribute((const void *)k_real_conv_vjp<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     init_karatsuba_attributes();
+    init_fast_attributes();
     (void)hipGetLastError();
 }
 
@@ -254,6 +259,10 @@ static hipError_t launch_pass_t(const AfPass &d, hipStream_t s) {
 }
 
 hipError_t launch_pass(int prec, const AfPass &d, hipStream_t s) {
+    if (d.ntiles <= 0) return hipSuccess;
+    bool done = false;
+    hipError_t e = launch_fast(prec, d, s, done);
+    if (done || e != hipSuccess) return e;
     return prec == PREC_F64 ? launch_pass_t<double>(d, s) : launch_pass_t<float>(d, s);
 }
 

@@ -27,6 +27,9 @@ hipError_t convert_to_w(int prec, void *out, const void *in, int64_t count, doub
 // Allow every kernel the full 160 KiB of LDS (called once).
 void init_kernel_attributes();
 void init_karatsuba_attributes();
+void init_fast_attributes();
+// Power-of-two passes with compile-time stages (fast_pass.hip); done = false when not eligible.
+hipError_t launch_fast(int prec, const AfPass &d, hipStream_t s, bool &done);
 
 // Karatsuba (karatsuba.hip)
 hipError_t launch_karatsuba(const int64_t *x, const int64_t *y, int64_t *z, const int64_t *dev_meta, int64_t npairs,
