@@ -39,6 +39,18 @@ def _check_tensor(t, what):
         raise ValueError(f"{what} must be contiguous (transform axis innermost)")
 
 
+def materialize(x):
+    """Resolve torch's lazy conjugate / negative views: the kernels read memory."""
+    if isinstance(x, tuple):
+        return tuple(materialize(t) for t in x)
+    if isinstance(x, torch.Tensor):
+        if x.is_conj():
+            x = x.resolve_conj()
+        if x.is_neg():
+            x = x.resolve_neg()
+    return x
+
+
 def complex_view(x, what="input"):
     """(re_ptr, im_ptr or None, shape) of a complex array in either layout."""
     if isinstance(x, tuple):

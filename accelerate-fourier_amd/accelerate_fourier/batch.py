@@ -30,8 +30,11 @@ class UnevenBatch:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _abi._lib is not None:
-            _abi.lib().af_batch_destroy(h)
+        if h is not None and h.value and _abi is not None and _abi._lib is not None:
+            try:
+                _abi._lib.af_batch_destroy(h)
+            except Exception:  # interpreter teardown
+                pass
             self.handle = None
 
     def describe(self) -> str:

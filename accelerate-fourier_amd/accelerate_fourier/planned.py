@@ -30,8 +30,11 @@ class Plan:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _abi._lib is not None:
-            _abi.lib().af_plan_destroy(h)
+        if h is not None and h.value and _abi is not None and _abi._lib is not None:
+            try:
+                _abi._lib.af_plan_destroy(h)
+            except Exception:  # interpreter teardown
+                pass
             self.handle = None
 
     def workspace_size(self, batch):
@@ -77,6 +80,7 @@ class Transform:
         return get_plan(self.n, self.sign.value, _abi.C2C, precision, self.algorithm, self.flags)
 
     def __call__(self, x, out=None):
+        x = _arrays.materialize(x)
         re, im, shape = _arrays.complex_view(x)
         if not shape or shape[-1] != self.n:
             # Planned.hs:98-101: the length must equal the inner extent
@@ -93,6 +97,7 @@ class Transform:
 
     def vjp(self, g, out=None):
         """Adjoint (conjugate transpose) applied to the cotangent g."""
+        g = _arrays.materialize(g)
         re, im, shape = _arrays.complex_view(g, "cotangent")
         if shape[-1] != self.n:
             raise ValueError("cotangent has the wrong inner extent")
@@ -134,6 +139,7 @@ def convolve_cyclic(n: int):
         raise ValueError("convolveCyclic needs a non-empty length")
 
     def conv(x, y, out=None):
+        x, y = _arrays.materialize(x), _arrays.materialize(y)
         xr, xi, xs = _arrays.complex_view(x, "x")
         yr, yi, ys = _arrays.complex_view(y, "y")
         if xs != ys or xs[-1] != n:

@@ -30,6 +30,7 @@ def _check_real(x, what):
 
 
 def to_spectrum(sub: Transform, x, out=None, planar=False):
+    x = _arrays.materialize(x)
     _check_real(x, "x")
     n = x.shape[-1]
     if n % 2:
@@ -50,6 +51,7 @@ def to_spectrum(sub: Transform, x, out=None, planar=False):
 
 
 def from_spectrum(sub: Transform, spec, out=None):
+    spec = _arrays.materialize(spec)
     re, im, shape = _arrays.complex_view(spec, "spectrum")
     n = shape[-1]
     if n % 2:
@@ -68,6 +70,7 @@ def from_spectrum(sub: Transform, spec, out=None):
 
 def vjp_to_spectrum(g, out=None):
     """Adjoint of to_spectrum(transform(forward, n/2), .) applied to a complex cotangent."""
+    g = _arrays.materialize(g)
     re, im, shape = _arrays.complex_view(g, "cotangent")
     n = shape[-1]
     prec = _arrays.precision_of_complex(g)
@@ -82,6 +85,7 @@ def vjp_to_spectrum(g, out=None):
 
 def vjp_from_spectrum(g, out=None):
     """Adjoint of from_spectrum(transform(inverse, n/2), .) applied to a real cotangent."""
+    g = _arrays.materialize(g)
     _check_real(g, "cotangent")
     n = g.shape[-1]
     prec = _real_prec(g)

@@ -4,9 +4,10 @@ from __future__ import annotations
 
 def scale_down(x):
     """Multiply by recip n, n the inner extent (scaleDown)."""
+    n = x[0].shape[-1] if isinstance(x, tuple) else x.shape[-1]
+    if n == 0:  # recip 0 scales an empty array: nothing to do
+        return x
+    r = 1.0 / n
     if isinstance(x, tuple):
-        n = x[0].shape[-1]
-        r = 1.0 / n
         return (x[0] * r, x[1] * r)
-    n = x.shape[-1]
-    return x * (1.0 / n)
+    return x * r

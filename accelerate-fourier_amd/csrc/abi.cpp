@@ -369,7 +369,9 @@ struct af_plan_s {
     void *spec = nullptr;                           // convolution spectrum, table format
     void *aux_tw = nullptr;                         // real packing twiddles W_n^k, k < n/2
     af_plan dual = nullptr;                         // the opposite sign (adjoint of Rader / Bluestein)
-    af_plan sub = nullptr;                          // convolveCyclic of non-smooth lengths: C2C forward
+    af_plan sub = nullptr;                          // convolveCyclic of non-smooth lengths, or the half-length
+                                                    // transform of unfused real kinds: C2C
+    bool real_fused = false;
     ~af_plan_s() {
         if (dual) af_plan_destroy(dual);
         if (sub) af_plan_destroy(sub);
@@ -430,20 +432,31 @@ af_status shape_for(const af_plan_desc &d, PlanShape &sh, std::string &why) {
         if (!plan_shape(prec, n, d.algorithm, sh, why)) return AF_UNSUPPORTED;
         return AF_SUCCESS;
     }
-    // real kinds: one pass over the half-length transform
+    // real kinds: the half-length transform
     sh = PlanShape();
-    sh.n = n;
-    sh.algo = ALG_STOCKHAM;
-    sh.core = core_shape(prec, n < 1 ? 1 : n);
-    if (sh.core.kind != CORE_SINGLE) {
-        why = "half length must be smooth and fit one LDS pass";
-        return AF_UNSUPPORTED;
+    if (n == 0) {
+        sh.algo = ALG_EMPTY;
+        return AF_SUCCESS;
     }
-    if (d.kind == AF_REAL_CONV && lds_bytes_for(prec, (int)std::max<int64_t>(n, 1), 1, 4) * 2 > lds_tile_bytes_cap()) {
-        why = "two half-length spectra must fit in LDS";
-        return AF_UNSUPPORTED;
+    if (n == 1) {
+        sh.n = 1;
+        sh.algo = ALG_STOCKHAM;
+        sh.core.kind = CORE_SINGLE;
+        sh.core.N = 1;
+        return AF_SUCCESS;
     }
+    if (!plan_shape(prec, n, AF_ALG_AUTO, sh, why)) return AF_UNSUPPORTED;
     return AF_SUCCESS;
+}
+
+// Real kinds run as one fused pass when the half length is smooth and fits
+// LDS (two half-length buffers for the convolution); otherwise as a
+// half-length C2C plan of any kind plus the packing in global memory.
+bool real_fusable(const af_plan_desc &d, const PlanShape &sh) {
+    if (sh.algo != ALG_STOCKHAM || sh.core.kind != CORE_SINGLE) return false;
+    if (d.kind == AF_REAL_CONV)
+        return lds_bytes_for(d.precision, (int)sh.core.N, 1, 4) * 2 <= lds_tile_bytes_cap();
+    return true;
 }
 
 // FP64 forward DFT of `count` interleaved values on the device (plan time).
@@ -555,8 +568,10 @@ af_status af_plan_describe(const af_plan_desc *desc, char *buf, size_t buflen) {
     std::string s;
     const char *kinds[] = {"c2c", "r2c", "c2r", "convcyclic", "realconv"};
     s = std::string(kinds[desc->kind]) + (desc->precision == AF_F64 ? ".f64:" : ".f32:");
-    if (st == AF_SUCCESS) s += describe_shape(sh);
-    else s += "unsupported(" + why + ")";
+    const bool real_kind = desc->kind == AF_R2C_FULL || desc->kind == AF_C2R_FULL || desc->kind == AF_REAL_CONV;
+    if (st != AF_SUCCESS) s += "unsupported(" + why + ")";
+    else if (real_kind && sh.algo != ALG_EMPTY && !real_fusable(*desc, sh)) s += "packed(" + describe_shape(sh) + ")";
+    else s += describe_shape(sh);
     std::snprintf(buf, buflen, "%s", s.c_str());
     return st == AF_SUCCESS && s.size() < buflen ? AF_SUCCESS : (st != AF_SUCCESS ? st : AF_INVALID_ARG);
 }
@@ -594,6 +609,14 @@ static af_status plan_create_impl(const af_plan_desc *desc, af_plan *plan, bool 
             one.N = 1;
             P->core.sh = one;
         }
+    } else if ((desc->kind == AF_R2C_FULL || desc->kind == AF_C2R_FULL || desc->kind == AF_REAL_CONV) &&
+               sh.algo != ALG_EMPTY && !real_fusable(*desc, sh)) {
+        af_plan_desc d2 = *desc;
+        d2.kind = AF_C2C;
+        d2.length = sh.n;
+        d2.algorithm = AF_ALG_AUTO;
+        if (desc->kind == AF_REAL_CONV) d2.sign = -1;
+        st = af_plan_create(&d2, &P->sub);
     } else if (sh.algo == ALG_COPY) {
         CoreShape one;
         one.kind = CORE_SINGLE;
@@ -612,6 +635,8 @@ static af_status plan_create_impl(const af_plan_desc *desc, af_plan *plan, bool 
             st = plan_create_impl(&d2, &P->dual, false);
         }
     }
+    if (desc->kind == AF_R2C_FULL || desc->kind == AF_C2R_FULL || desc->kind == AF_REAL_CONV)
+        P->real_fused = sh.algo != ALG_EMPTY && real_fusable(*desc, sh);
     if (st == AF_SUCCESS &&
         (desc->kind == AF_R2C_FULL || desc->kind == AF_C2R_FULL || desc->kind == AF_REAL_CONV) && sh.n >= 1) {
         st = P->own.alloc(&P->aux_tw, (size_t)sh.n * 16);
@@ -729,6 +754,58 @@ af_status run_c2c(af_plan P, int sign, int64_t batch, const void *in_re, const v
 
 bool bad_ptrs(const void *a, const void *b) { return !a || !b; }
 
+// Unfused real kinds: the half-length C2C plan P->sub plus the packing kernels.
+af_status real_general(af_plan P, int op, int64_t batch, const void *a_re, const void *a_im, const void *b,
+                       const void *c, void *o1, void *o1_im, void *o2, hipStream_t s) {
+    const int prec = P->prec;
+    const int64_t n2 = P->shape.n;
+    const size_t eb = (size_t)elem_bytes(P->desc.precision);
+    const size_t half = (size_t)batch * n2 * eb;
+    af_plan sub = P->sub;
+    const int pl = P->plain ? 1 : 0;
+    Scratch sc;
+    switch (op) {
+    case 0:  // toSpectrum: transform the real rows viewed as interleaved complex, untangle
+        AFCHK(sc.get(half, nullptr, s));
+        AFCHK(run_c2c(sub, P->desc.sign, batch, a_re, nullptr, sc.p, nullptr, nullptr, s));
+        HIPCHK(launch_pack(prec, PACK_UNTANGLE, sc.p, nullptr, nullptr, o1, o1_im, P->aux_tw, n2, batch, pl, 0, s));
+        return AF_SUCCESS;
+    case 1:  // fromSpectrum: entangle, transform into the real rows
+        AFCHK(sc.get(half, nullptr, s));
+        HIPCHK(launch_pack(prec, PACK_ENTANGLE, a_re, a_im, nullptr, sc.p, nullptr, P->aux_tw, n2, batch, pl, 0, s));
+        return run_c2c(sub, P->desc.sign, batch, sc.p, nullptr, o1, nullptr, nullptr, s);
+    case 2:  // adjoint of toSpectrum: Hermitian part, entangle, inverse-sign transform
+        AFCHK(sc.get(half, nullptr, s));
+        HIPCHK(launch_pack(prec, PACK_ENTANGLE, a_re, a_im, nullptr, sc.p, nullptr, P->aux_tw, n2, batch, pl, 1, s));
+        return run_c2c(sub, +1, batch, sc.p, nullptr, o1, nullptr, nullptr, s);
+    case 3:  // adjoint of fromSpectrum: opposite-sign transform, adjoint entangling
+        AFCHK(sc.get(half, nullptr, s));
+        AFCHK(run_c2c(sub, -P->desc.sign, batch, a_re, nullptr, sc.p, nullptr, nullptr, s));
+        HIPCHK(launch_pack(prec, PACK_ENTANGLE_ADJ, sc.p, nullptr, nullptr, o1, o1_im, P->aux_tw, n2, batch, pl, 0, s));
+        return AF_SUCCESS;
+    case 4: {  // real convolution y = fromSpectrum(S_x * S_k)
+        // the product reads partners (k, -k): its output must not alias its inputs
+        AFCHK(sc.get(3 * half, nullptr, s));
+        void *za = sc.p, *zb = (char *)sc.p + half, *zp = (char *)sc.p + 2 * half;
+        AFCHK(run_c2c(sub, -1, batch, a_re, nullptr, za, nullptr, nullptr, s));
+        AFCHK(run_c2c(sub, -1, batch, b, nullptr, zb, nullptr, nullptr, s));
+        HIPCHK(launch_pack(prec, PACK_PRODUCT, za, nullptr, zb, zp, nullptr, P->aux_tw, n2, batch, pl, 0, s));
+        return run_c2c(sub, +1, batch, zp, nullptr, o1, nullptr, nullptr, s);
+    }
+    default: {  // its adjoint: dk = fromSpectrum(S_w conj S_x), dx = fromSpectrum(S_w conj S_k)
+        AFCHK(sc.get(3 * half, nullptr, s));
+        void *zw = sc.p, *zx = (char *)sc.p + half, *zp = (char *)sc.p + 2 * half;
+        AFCHK(run_c2c(sub, -1, batch, c, nullptr, zw, nullptr, nullptr, s));
+        AFCHK(run_c2c(sub, -1, batch, a_re, nullptr, zx, nullptr, nullptr, s));
+        HIPCHK(launch_pack(prec, PACK_PRODUCT, zw, nullptr, zx, zp, nullptr, P->aux_tw, n2, batch, pl, 1, s));
+        AFCHK(run_c2c(sub, +1, batch, zp, nullptr, o2, nullptr, nullptr, s));
+        AFCHK(run_c2c(sub, -1, batch, b, nullptr, zx, nullptr, nullptr, s));
+        HIPCHK(launch_pack(prec, PACK_PRODUCT, zw, nullptr, zx, zp, nullptr, P->aux_tw, n2, batch, pl, 1, s));
+        return run_c2c(sub, +1, batch, zp, nullptr, o1, nullptr, nullptr, s);
+    }
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -782,6 +859,8 @@ af_status af_to_spectrum(af_plan plan, int64_t batch, const void *x, void *out_r
     const int64_t n = plan->desc.length, n2 = n / 2;
     if (n == 0 || batch == 0) return AF_SUCCESS;
     if (bad_ptrs(x, out_re)) return AF_INVALID_ARG;
+    if (!plan->real_fused)
+        return real_general(plan, 0, batch, x, nullptr, nullptr, nullptr, out_re, out_im, nullptr, (hipStream_t)stream);
     AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, plan->core.tw_d, choose_B_rows(plan->prec, n2, batch), 1,
                          batch, plan->desc.sign, plan->plain);
     set_in(d, ep(x, nullptr, n2), 1, 0);
@@ -799,6 +878,8 @@ af_status af_from_spectrum(af_plan plan, int64_t batch, const void *in_re, const
     const int64_t n = plan->desc.length, n2 = n / 2;
     if (n == 0 || batch == 0) return AF_SUCCESS;
     if (bad_ptrs(in_re, y)) return AF_INVALID_ARG;
+    if (!plan->real_fused)
+        return real_general(plan, 1, batch, in_re, in_im, nullptr, nullptr, y, nullptr, nullptr, (hipStream_t)stream);
     AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, plan->core.tw_d, choose_B_rows(plan->prec, n2, batch), 1,
                          batch, plan->desc.sign, plan->plain);
     set_in(d, ep(in_re, in_im, n), 1, 0);
@@ -817,6 +898,8 @@ af_status af_vjp_to_spectrum(af_plan plan, int64_t batch, const void *g_re, cons
     const int64_t n = plan->desc.length, n2 = n / 2;
     if (n == 0 || batch == 0) return AF_SUCCESS;
     if (bad_ptrs(g_re, dx)) return AF_INVALID_ARG;
+    if (!plan->real_fused)
+        return real_general(plan, 2, batch, g_re, g_im, nullptr, nullptr, dx, nullptr, nullptr, (hipStream_t)stream);
     AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, plan->core.tw_d, choose_B_rows(plan->prec, n2, batch), 1,
                          batch, +1, plan->plain);
     set_in(d, ep(g_re, g_im, n), 1, 0);
@@ -834,6 +917,8 @@ af_status af_vjp_from_spectrum(af_plan plan, int64_t batch, const void *g, void 
     const int64_t n = plan->desc.length, n2 = n / 2;
     if (n == 0 || batch == 0) return AF_SUCCESS;
     if (bad_ptrs(g, out_re)) return AF_INVALID_ARG;
+    if (!plan->real_fused)
+        return real_general(plan, 3, batch, g, nullptr, nullptr, nullptr, out_re, out_im, nullptr, (hipStream_t)stream);
     AfPass d = base_pass(plan->prec, (int)n2, plan->core.sh.r, plan->core.tw, plan->core.tw_d, choose_B_rows(plan->prec, n2, batch), 1,
                          batch, -plan->desc.sign, plan->plain);
     set_in(d, ep(g, nullptr, n2), 1, 0);
@@ -856,6 +941,8 @@ af_status af_real_convolve(af_plan plan, int64_t batch, const void *x, const voi
     if (!plan || plan->desc.kind != AF_REAL_CONV || batch < 0) return AF_INVALID_ARG;
     if (batch == 0) return AF_SUCCESS;
     if (!x || !k || !y) return AF_INVALID_ARG;
+    if (!plan->real_fused)
+        return real_general(plan, 4, batch, x, nullptr, k, nullptr, y, nullptr, nullptr, (hipStream_t)stream);
     HIPCHK(launch_real_conv(plan->prec, real_conv_geom(plan, batch), x, k, y, (hipStream_t)stream));
     return AF_SUCCESS;
 }
@@ -865,6 +952,8 @@ af_status af_vjp_real_convolve(af_plan plan, int64_t batch, const void *x, const
     if (!plan || plan->desc.kind != AF_REAL_CONV || batch < 0) return AF_INVALID_ARG;
     if (batch == 0) return AF_SUCCESS;
     if (!x || !k || !w || !dx || !dk) return AF_INVALID_ARG;
+    if (!plan->real_fused)
+        return real_general(plan, 5, batch, x, nullptr, k, w, dx, nullptr, dk, (hipStream_t)stream);
     HIPCHK(launch_real_conv_vjp(plan->prec, real_conv_geom(plan, batch), x, k, w, dx, dk, (hipStream_t)stream));
     return AF_SUCCESS;
 }

@@ -175,6 +175,77 @@ k_real_conv_vjp(const AfPass d, const T *__restrict__ x, const T *__restrict__ k
 }
 
 // ---------------------------------------------------------------------------
+// Real packing in global memory, for half lengths the fused pass cannot take.
+// Z holds rows of L interleaved half-length values; full spectra have 2L.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ void k_untangle(const typename Cx<T>::C *__restrict__ Z, void *out_re, void *out_im,
+                           const typename Cx<T>::W *__restrict__ aux, int64_t L, int64_t rows, int plain) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= L * rows) return;
+    const int64_t r = g / L, k = g - r * L;
+    typename Cx<T>::C lo, hi;
+    untangle<T>(Z[r * L + k], Z[r * L + (L - k) % L], aux[k], plain != 0, lo, hi);
+    st_c<T>(out_re, out_im, r * 2 * L + k, lo);
+    st_c<T>(out_re, out_im, r * 2 * L + k + L, hi);
+}
+
+// mode 0: fromSpectrum's entangling of S; mode 1: the same after the
+// Hermitian projection (G_k + conj G_-k) / 2 (the adjoint of toSpectrum).
+template <typename T>
+__global__ void k_entangle(const void *s_re, const void *s_im, typename Cx<T>::C *__restrict__ z,
+                           const typename Cx<T>::W *__restrict__ aux, int64_t L, int64_t rows, int plain, int mode) {
+    using C = typename Cx<T>::C;
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= L * rows) return;
+    const int64_t r = g / L, k = g - r * L, b = r * 2 * L;
+    C p0 = ld_c<T>(s_re, s_im, b + k), p1 = ld_c<T>(s_re, s_im, b + k + L);
+    if (mode == 1) {
+        const int64_t kr = (L - k) % L;
+        if (k == 0) {
+            p0 = mk(p0.x, T(0));
+            p1 = mk(p1.x, T(0));
+        } else {
+            p0 = cscale(cadd(p0, cconj(ld_c<T>(s_re, s_im, b + kr + L))), T(0.5));
+            p1 = cscale(cadd(p1, cconj(ld_c<T>(s_re, s_im, b + kr))), T(0.5));
+        }
+    }
+    z[r * L + k] = entangle<T>(p0, p1, aux[k], plain != 0);
+}
+
+// adjoint of the entangling: dS_k = U_k + c, dS_{k+L} = U_k - c, c = -i W_{2L}^k U_k
+template <typename T>
+__global__ void k_entangle_adj(const typename Cx<T>::C *__restrict__ U, void *out_re, void *out_im,
+                               const typename Cx<T>::W *__restrict__ aux, int64_t L, int64_t rows, int plain) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= L * rows) return;
+    const int64_t r = g / L, k = g - r * L;
+    const auto u = U[r * L + k];
+    const auto t = twmul(u, aux[k], T(-1), plain != 0);
+    const auto c = mk(t.y, -t.x);
+    st_c<T>(out_re, out_im, r * 2 * L + k, cadd(u, c));
+    st_c<T>(out_re, out_im, r * 2 * L + k + L, csub(u, c));
+}
+
+// Pointwise product of two untangled spectra given by their half-length
+// transforms, entangled again: the middle of the real convolution.
+template <typename T>
+__global__ void k_real_product(const typename Cx<T>::C *__restrict__ A, const typename Cx<T>::C *__restrict__ Bz,
+                               typename Cx<T>::C *__restrict__ z, const typename Cx<T>::W *__restrict__ aux,
+                               int64_t L, int64_t rows, int plain, int conj_b) {
+    using C = typename Cx<T>::C;
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= L * rows) return;
+    const int64_t r = g / L, k = g - r * L, kr = (L - k) % L, b = r * L;
+    C sa_lo, sa_hi, sb_lo, sb_hi;
+    untangle<T>(A[b + k], A[b + kr], aux[k], plain != 0, sa_lo, sa_hi);
+    untangle<T>(Bz[b + k], Bz[b + kr], aux[k], plain != 0, sb_lo, sb_hi);
+    if (conj_b) { sb_lo = cconj(sb_lo); sb_hi = cconj(sb_hi); }
+    z[b + k] = entangle<T>(cmul(sa_lo, sb_lo), cmul(sa_hi, sb_hi), aux[k], plain != 0);
+}
+
+// ---------------------------------------------------------------------------
 // Table generation in FP64 (sincospi of exactly reduced rational angles)
 // ---------------------------------------------------------------------------
 
@@ -339,6 +410,38 @@ hipError_t convert_to_w(int prec, void *out, const void *in, int64_t count, doub
         hipLaunchKernelGGL(k_convert<float4>, dim3(grid_for(count, 256)), dim3(256), 0, s, (float4 *)out,
                            (const double2 *)in, count, scale, N1, N2);
     return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_pack_t(int op, const void *a, const void *a_im, const void *b, void *o_re, void *o_im,
+                                const void *aux, int64_t L, int64_t rows, int plain, int arg, hipStream_t s) {
+    using C = typename Cx<T>::C;
+    using W = typename Cx<T>::W;
+    const int64_t n = L * rows;
+    if (n <= 0) return hipSuccess;
+    const dim3 g(grid_for(n, 256)), bl(256);
+    switch (op) {
+    case PACK_UNTANGLE:
+        hipLaunchKernelGGL(k_untangle<T>, g, bl, 0, s, (const C *)a, o_re, o_im, (const W *)aux, L, rows, plain);
+        break;
+    case PACK_ENTANGLE:
+        hipLaunchKernelGGL(k_entangle<T>, g, bl, 0, s, a, a_im, (C *)o_re, (const W *)aux, L, rows, plain, arg);
+        break;
+    case PACK_ENTANGLE_ADJ:
+        hipLaunchKernelGGL(k_entangle_adj<T>, g, bl, 0, s, (const C *)a, o_re, o_im, (const W *)aux, L, rows, plain);
+        break;
+    default:
+        hipLaunchKernelGGL(k_real_product<T>, g, bl, 0, s, (const C *)a, (const C *)b, (C *)o_re, (const W *)aux, L,
+                           rows, plain, arg);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(int prec, int op, const void *a, const void *a_im, const void *b, void *o_re, void *o_im,
+                       const void *aux, int64_t L, int64_t rows, int plain, int arg, hipStream_t s) {
+    return prec == PREC_F64 ? launch_pack_t<double>(op, a, a_im, b, o_re, o_im, aux, L, rows, plain, arg, s)
+                            : launch_pack_t<float>(op, a, a_im, b, o_re, o_im, aux, L, rows, plain, arg, s);
 }
 
 }  // namespace af

@@ -24,6 +24,11 @@ hipError_t gen_rader_kernel_d(void *out, const int32_t *powg, int64_t p, int64_t
 hipError_t convert_to_w(int prec, void *out, const void *in, int64_t count, double scale, int64_t N1, int64_t N2,
                         hipStream_t s);
 
+// Real packing in global memory (fft_kernels.hip)
+enum { PACK_UNTANGLE = 0, PACK_ENTANGLE = 1, PACK_ENTANGLE_ADJ = 2, PACK_PRODUCT = 3 };
+hipError_t launch_pack(int prec, int op, const void *a, const void *a_im, const void *b, void *o_re, void *o_im,
+                       const void *aux, int64_t L, int64_t rows, int plain, int arg, hipStream_t s);
+
 // Allow every kernel the full 160 KiB of LDS (called once).
 void init_kernel_attributes();
 void init_karatsuba_attributes();

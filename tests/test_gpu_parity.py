@@ -178,3 +178,40 @@ def test_uneven_batch():
             continue
         ref = oracle.transform(-1, x[off[i]:off[i + 1]])
         assert rel_l2(y[off[i]:off[i + 1]], ref) <= tol_f64(n), f"row {i} length {n}"
+
+
+@pytest.mark.parametrize("n", [86, 58, 40000, 2 * 65521])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_real_spectrum_packed(n, prec):
+    """Half lengths that are prime or above one LDS pass: half-length C2C plan
+    plus the packing kernels."""
+    import oracle
+    rt = torch.float64 if prec == "f64" else torch.float32
+    x = synthetic.normal(13 + n, 2 * n).reshape(2, n)
+    S = real.to_spectrum(planned.transform(forward, n // 2), to_dev(x, rt))
+    ref = oracle.to_spectrum(x)
+    tol = tol_f64(n // 2) if prec == "f64" else 1e-6
+    assert rel_l2(host(S), ref) <= tol
+    y = real.from_spectrum(planned.transform(inverse, n // 2), to_dev(ref, S.dtype))
+    assert rel_l2(host(y), oracle.from_spectrum(ref)) <= tol
+    from accelerate_fourier import autograd as AG  # noqa: F401
+    g = to_dev(synthetic.complex_uniform(5, (2, n)), torch.complex128 if prec == "f64" else torch.complex64)
+    dx = host(real.vjp_to_spectrum(g))
+    gg = host(g).astype(np.complex128)
+    ref_dx = np.real(np.fft.ifft(gg) * n)  # Re(F^H g)
+    assert rel_l2(dx, ref_dx) <= (1e-11 if prec == "f64" else 1e-6)
+
+
+@pytest.mark.parametrize("n", [86, 40000])
+def test_real_convolution_packed(n):
+    import oracle
+    x = synthetic.normal(3, 2 * n).reshape(2, n)
+    k = synthetic.normal(4, 2 * n).reshape(2, n) / np.sqrt(n)
+    w = synthetic.normal(5, 2 * n).reshape(2, n)
+    xd, kd, wd = to_dev(x, torch.float64), to_dev(k, torch.float64), to_dev(w, torch.float64)
+    y = real.convolve(xd, kd)
+    Sx, Sk, Sw = oracle.to_spectrum(x), oracle.to_spectrum(k), oracle.to_spectrum(w)
+    assert rel_l2(host(y), oracle.from_spectrum(Sx * Sk)) <= tol_f64(n // 2)
+    dx, dk = real.convolve_vjp(xd, kd, wd)
+    assert rel_l2(host(dx), oracle.from_spectrum(Sw * np.conj(Sk))) <= tol_f64(n // 2)
+    assert rel_l2(host(dk), oracle.from_spectrum(Sw * np.conj(Sx))) <= tol_f64(n // 2)
