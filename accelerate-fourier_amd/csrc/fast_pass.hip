@@ -55,25 +55,88 @@ __device__ __forceinline__ void bf_map(int bf, bool lines_fast, int &b, int &i) 
 __device__ __host__ constexpr int fpad(int e) { return e + (e >> 4); }
 __device__ __host__ constexpr int fast_ld(int L, int B) { return B > 1 ? fpad(L) + 1 : fpad(L); }
 
+// Stage twiddles come from an LDS copy of the W_L table when it is small
+// (L <= 1024: at most 16 KiB), otherwise from one FP64 load per butterfly and
+// FP64 powers.  Four-step twiddles come from FP64 recurrences (an LDS-table
+// variant exists below, disabled by kFtLdsEntries).
+template <int L> struct FastTables {
+    static constexpr bool STAGE_LDS = L <= 1024;
+    static constexpr int STAGE_ENTRIES = STAGE_LDS ? L : 0;
+};
+// Measured on MI355X: LDS copies of the four-step tables (two 16-byte LDS reads
+// and a compensated product per element) ran slower than one FP64 load pair per
+// butterfly plus FP64 recurrences, so the LDS variant is kept off.
+__host__ __device__ constexpr int kFtLdsEntries() { return 0; }
+
+__host__ __device__ inline bool ft_in_lds(int64_t ftN, int ftShift) {
+    const int64_t T = (int64_t)1 << ftShift, nc = (ftN + T - 1) / T;
+    return nc + T <= kFtLdsEntries();
+}
+
+// W_N^(sign a) = coarse[a >> shift] * fine[a & mask]: FP32 product of two hi/lo
+// pairs with first-order lo corrections (about one ulp), FP64 plain product.
+__device__ __forceinline__ float2 ft_prod(const float4 *co, const float4 *fi, uint32_t a, int shift, int sign) {
+    const float4 c = co[a >> shift], f = fi[a & ((1u << shift) - 1)];
+    float re = fmaf(c.x, f.z, fmaf(-c.y, f.w, fmaf(c.z, f.x, -c.w * f.y)));
+    float im = fmaf(c.x, f.w, fmaf(c.y, f.z, fmaf(c.z, f.y, c.w * f.x)));
+    re = fmaf(c.x, f.x, fmaf(-c.y, f.y, re));
+    im = fmaf(c.x, f.y, fmaf(c.y, f.x, im));
+    return make_float2(re, sign > 0 ? -im : im);
+}
+__device__ __forceinline__ double2 ft_prod(const double2 *co, const double2 *fi, uint32_t a, int shift, int sign) {
+    double2 w = cmul(co[a >> shift], fi[a & ((1u << shift) - 1)]);
+    if (sign > 0) w.y = -w.y;
+    return w;
+}
+
+template <typename T, int L, int B>
+__host__ __device__ constexpr size_t fast_lds_bytes(bool ft_lds, int ft_entries) {
+    return (size_t)fast_ld(L, B) * B * sizeof(typename Cx<T>::C) +
+           (size_t)(FastTables<L>::STAGE_ENTRIES + (ft_lds ? ft_entries : 0)) * 16;
+}
+
 template <typename T, int L, int B>
 __global__ void __launch_bounds__(B * L / 16)
 k_fast(const AfPass d) {
     using C = typename Cx<T>::C;
     using W = typename Cx<T>::W;
     using PL = P2Plan<L>;
+    using FT = FastTables<L>;
     constexpr int NT = B * L / 16;
     constexpr int LD = fast_ld(L, B);
     extern __shared__ __align__(16) unsigned char smem_fast[];
     C *lds = (C *)smem_fast;
+    W *stab = (W *)(smem_fast + (size_t)LD * B * sizeof(C));
+    W *ftc = stab + FT::STAGE_ENTRIES;
     const int t = threadIdx.x;
     const bool plain = (d.flags & AFP_PLAIN_TW) != 0;
+    const bool posttw = (d.flags & AFP_POSTTW) != 0;
+    const bool ftl = posttw && ft_in_lds(d.ftN, d.ftShift);
+    const int ftT = 1 << d.ftShift;
+    const int ftNc = (int)((d.ftN + ftT - 1) >> d.ftShift);
+    W *ftf = ftc + (ftl ? ftNc : 0);
     const T s = T(d.sign);
+    // tables into LDS; the first barrier below orders them before any use
+    if (FT::STAGE_LDS)
+        for (int i = t; i < L; i += NT) stab[i] = ((const W *)d.tw)[i];
+    if (ftl) {
+        for (int i = t; i < ftNc; i += NT) ftc[i] = ((const W *)d.ft_coarse)[i];
+        for (int i = t; i < ftT; i += NT) ftf[i] = ((const W *)d.ft_fine)[i];
+    }
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t gl0 = tile * B;
     // all lines of a tile lie in one row (B divides the lines per row)
     const int64_t row = gl0 / d.nlines;
     const int64_t line0 = gl0 - row * d.nlines;
-    const int64_t ib = row * d.in_bs, ob = row * d.out_bs;
+    const int64_t ib = row * d.in_bs + line0 * d.in_ls, ob = row * d.out_bs + line0 * d.out_ls;
+    const int ies = (int)d.in_es, oes = (int)d.out_es, ils = (int)d.in_ls, ols = (int)d.out_ls;
+    // row base pointers; element offsets below stay 32-bit (b * ls + e * es < 2^31)
+    const T *in_re = (const T *)d.in_re, *in_im = (const T *)d.in_im;
+    const C *in_c = (const C *)d.in_re;
+    T *out_re = (T *)d.out_re, *out_im = (T *)d.out_im;
+    C *out_c = (C *)d.out_re;
+    if (in_im) { in_re += ib; in_im += ib; } else { in_c += ib; }
+    if (out_im) { out_re += ob; out_im += ob; } else { out_c += ob; }
 
     C v[16];
 #pragma unroll
@@ -96,8 +159,10 @@ k_fast(const AfPass d) {
             for (int j = 0; j < R; ++j) {
                 const int e = i + j * NB;
                 if (first) {
-                    const int64_t l = line0 + b;
-                    v[q * R + j] = gload<T>(d, ib + l * d.in_ls + (int64_t)e * d.in_es);
+                    const int off = b * ils + e * ies;
+                    if (in_im) v[q * R + j] = mk(__builtin_nontemporal_load(in_re + off),
+                                                 __builtin_nontemporal_load(in_im + off));
+                    else v[q * R + j] = in_c[off];
                 } else {
                     v[q * R + j] = lds[b * LD + fpad(e)];
                 }
@@ -114,15 +179,20 @@ k_fast(const AfPass d) {
             else bf_map<B, L / 2>(t + q * NT, lf, b, i);
             const int k = i % p;
             if (p > 1) {
-                // w^j for j = 1..R-1 from one FP64 load: W_L^(k L / (p R)) and its powers
                 const int tws = L / (p * R);
-                double2 w1 = ((const double2 *)d.tw_d)[k * tws];
-                if (d.sign > 0) w1.y = -w1.y;
-                double2 wj = w1;
+                if (FT::STAGE_LDS) {
 #pragma unroll
-                for (int j = 1; j < R; ++j) {
-                    v[q * R + j] = dmul(v[q * R + j], wj, plain);
-                    if (j + 1 < R) wj = cmul(wj, w1);
+                    for (int j = 1; j < R; ++j) v[q * R + j] = twmul(v[q * R + j], stab[j * k * tws], s, plain);
+                } else {
+                    // w^j for j = 1..R-1 from one FP64 load and FP64 powers
+                    double2 w1 = ((const double2 *)d.tw_d)[k * tws];
+                    if (d.sign > 0) w1.y = -w1.y;
+                    double2 wj = w1;
+#pragma unroll
+                    for (int j = 1; j < R; ++j) {
+                        v[q * R + j] = dmul(v[q * R + j], wj, plain);
+                        if (j + 1 < R) wj = cmul(wj, w1);
+                    }
                 }
             }
             if (R == 16) dft<16>(&v[q * R], s);
@@ -134,25 +204,35 @@ k_fast(const AfPass d) {
 #pragma unroll
                 for (int j = 0; j < R; ++j) lds[b * LD + fpad(o + j * p)] = v[q * R + j];
             } else {
-                const int64_t l = line0 + b;
-                // four-step twiddles W_N^(l e), e = i + j NB: base W_N^(l i) times step^j,
-                // step = W_N^(l NB), in FP64 (l * e < N <= 2^30 for every four-step length)
+                const uint32_t l = (uint32_t)(line0 + b);
+                // four-step twiddles W_N^(l e), e = i + j NB (l * e < N <= 2^30)
                 double2 tw_e = make_double2(1.0, 0.0), tw_step = tw_e;
-                if (d.flags & AFP_POSTTW) {
-                    const uint32_t mask = (uint32_t)(d.ftN - 1);
-                    tw_e = ft_twiddle_d(d, ((uint32_t)l * (uint32_t)i) & mask);
-                    tw_step = ft_twiddle_d(d, ((uint32_t)l * (uint32_t)NB) & mask);
+                const uint32_t mask = (uint32_t)(d.ftN - 1);
+                if (posttw && !ftl) {
+                    tw_e = ft_twiddle_d(d, (l * (uint32_t)i) & mask);
+                    tw_step = ft_twiddle_d(d, (l * (uint32_t)NB) & mask);
                 }
 #pragma unroll
                 for (int j = 0; j < R; ++j) {
                     const int e = i + j * NB;
                     C x = v[q * R + j];
-                    if (d.flags & AFP_POSTTW) {
-                        x = dmul(x, tw_e, plain);
-                        if (j + 1 < R) tw_e = cmul(tw_e, tw_step);
+                    if (posttw) {
+                        if (ftl) {
+                            x = cmul(x, ft_prod(ftc, ftf, (l * (uint32_t)e) & mask, d.ftShift, d.ftSign));
+                        } else {
+                            // the FP64 value rounded once to the working precision
+                            x = cmul(x, mk(T(tw_e.x), T(tw_e.y)));
+                            if (j + 1 < R) tw_e = cmul(tw_e, tw_step);
+                        }
                     }
                     if (d.flags & AFP_SCALE) x = cscale(x, T(d.scale));
-                    gstore<T>(d, ob + l * d.out_ls + (int64_t)e * d.out_es, x);
+                    const int off = b * ols + e * oes;
+                    if (out_im) {
+                        out_re[off] = x.x;
+                        out_im[off] = x.y;
+                    } else {
+                        out_c[off] = x;
+                    }
                 }
             }
         }
@@ -163,7 +243,10 @@ k_fast(const AfPass d) {
 template <typename T, int L, int B>
 static bool try_launch(const AfPass &d, hipStream_t s, hipError_t &err) {
     if (d.L != L || d.B != B) return false;
-    const size_t lds = (size_t)fast_ld(L, B) * B * sizeof(typename Cx<T>::C);
+    const bool ftl = (d.flags & AFP_POSTTW) && ft_in_lds(d.ftN, d.ftShift);
+    const int T_ = 1 << d.ftShift;
+    const int ft_entries = ftl ? (int)((d.ftN + T_ - 1) / T_) + T_ : 0;
+    const size_t lds = fast_lds_bytes<T, L, B>(ftl, ft_entries);
     hipLaunchKernelGGL((k_fast<T, L, B>), dim3((unsigned)d.ntiles), dim3(B * L / 16), lds, s, d);
     err = hipGetLastError();
     return true;
@@ -181,7 +264,14 @@ bool fast_pass_eligible(int prec, const AfPass &d) {
     if ((d.flags & AFP_POSTTW) && (d.ftN & (d.ftN - 1))) return false;
     if (d.nlines % d.B) return false;
     if (d.ntiles >= ((int64_t)1 << 31)) return false;
-    const int lds = fast_ld(d.L, d.B) * d.B * (prec == PREC_F64 ? 16 : 8);
+    const int64_t row_span = (int64_t)(d.B - 1) * d.in_ls + (int64_t)(d.L - 1) * d.in_es;
+    const int64_t row_span_o = (int64_t)(d.B - 1) * d.out_ls + (int64_t)(d.L - 1) * d.out_es;
+    if (row_span >= ((int64_t)1 << 31) || row_span_o >= ((int64_t)1 << 31)) return false;
+    const int T_ = (d.flags & AFP_POSTTW) ? 1 << d.ftShift : 0;
+    const bool ftl = (d.flags & AFP_POSTTW) && ft_in_lds(d.ftN, d.ftShift);
+    const int64_t ft_entries = ftl ? (d.ftN + T_ - 1) / T_ + T_ : 0;
+    const int64_t lds = (int64_t)fast_ld(d.L, d.B) * d.B * (prec == PREC_F64 ? 16 : 8) +
+                        ((d.L <= 1024 ? d.L : 0) + ft_entries) * 16;
     if (lds > 160 * 1024) return false;
     return (int64_t)d.B * d.L / 16 <= 1024;
 }

@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--skip-extras", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="only the headline transform (steps launches) and a 32 MiB device copy of "
+                         "known size, for rocprofv3 kernel traces and counters")
     return ap.parse_args()
 
 
@@ -133,6 +136,17 @@ def main():
     x = synthetic.complex_normal(1, (batch, n))
     f32 = C2C(n, batch, _abi.F32)
     f32.set_io(x)
+    if args.profile:
+        for _ in range(warmup + steps):
+            f32.run()
+        src = torch.empty(32 * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(steps):
+            dst.copy_(src)  # reference kernel: reads and writes exactly 32 MiB
+        torch.cuda.synchronize()
+        print(json.dumps({"profile": "done", "transform_launches": f32.npasses * (warmup + steps),
+                          "copy_bytes_each_way": 32 * 2 ** 20}))
+        return
     ms = time_it(f32.run, steps, warmup)
     pass_ms = [time_it(lambda i=i: f32.run_pass(i), steps, warmup) for i in range(f32.npasses)]
     dom = int(np.argmax(pass_ms))
