@@ -57,88 +57,196 @@ __device__ __host__ constexpr int fast_ld(int L, int B) { return B > 1 ? fpad(L)
 
 // Stage twiddles come from an LDS copy of the W_L table when it is small
 // (L <= 1024: at most 16 KiB), otherwise from one FP64 load per butterfly and
-// FP64 powers.  Four-step twiddles come from FP64 recurrences (an LDS-table
-// variant exists below, disabled by kFtLdsEntries).
+// FP64 powers.  Four-step twiddles come from FP64 recurrences: one coarse x
+// fine table product per butterfly and a running FP64 product over its
+// outputs, each value rounded once to the working precision.  (LDS copies of
+// the four-step tables, two 16-byte LDS reads and a compensated product per
+// element, measured slower on MI355X.)
 template <int L> struct FastTables {
     static constexpr bool STAGE_LDS = L <= 1024;
     static constexpr int STAGE_ENTRIES = STAGE_LDS ? L : 0;
 };
-// Measured on MI355X: LDS copies of the four-step tables (two 16-byte LDS reads
-// and a compensated product per element) ran slower than one FP64 load pair per
-// butterfly plus FP64 recurrences, so the LDS variant is kept off.
-__host__ __device__ constexpr int kFtLdsEntries() { return 0; }
 
-__host__ __device__ inline bool ft_in_lds(int64_t ftN, int ftShift) {
-    const int64_t T = (int64_t)1 << ftShift, nc = (ftN + T - 1) / T;
-    return nc + T <= kFtLdsEntries();
+template <typename T, int L, int B>
+__host__ __device__ constexpr size_t fast_lds_bytes() {
+    return (size_t)fast_ld(L, B) * B * sizeof(typename Cx<T>::C) + (size_t)FastTables<L>::STAGE_ENTRIES * 16;
 }
 
-// W_N^(sign a) = coarse[a >> shift] * fine[a & mask]: FP32 product of two hi/lo
-// pairs with first-order lo corrections (about one ulp), FP64 plain product.
-__device__ __forceinline__ float2 ft_prod(const float4 *co, const float4 *fi, uint32_t a, int shift, int sign) {
-    const float4 c = co[a >> shift], f = fi[a & ((1u << shift) - 1)];
-    float re = fmaf(c.x, f.z, fmaf(-c.y, f.w, fmaf(c.z, f.x, -c.w * f.y)));
-    float im = fmaf(c.x, f.w, fmaf(c.y, f.z, fmaf(c.z, f.y, c.w * f.x)));
-    re = fmaf(c.x, f.x, fmaf(-c.y, f.y, re));
-    im = fmaf(c.x, f.y, fmaf(c.y, f.x, im));
-    return make_float2(re, sign > 0 ? -im : im);
+// Barrier for the LDS exchanges between stages: it orders this wave's LDS
+// operations only and leaves its HBM stores in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
-__device__ __forceinline__ double2 ft_prod(const double2 *co, const double2 *fi, uint32_t a, int shift, int sign) {
-    double2 w = cmul(co[a >> shift], fi[a & ((1u << shift) - 1)]);
-    if (sign > 0) w.y = -w.y;
+
+// HBM accesses of the power-of-two kernel go through buffer resources: a
+// uniform base and byte range in SGPRs and a 32-bit byte offset per lane.
+// The hardware checks every offset against the range (loads outside read 0,
+// stores outside are dropped), and the layouts keep distinct access widths:
+// interleaved elements move as one 8- or 16-byte access, planar ones as two.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc mk_rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
+}
+constexpr int kNT = 2;  // cache policy: nontemporal
+
+template <typename V, int AUX> __device__ __forceinline__ V bld(Rsrc r, uint32_t off) {
+    if constexpr (sizeof(V) == 4) return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, AUX));
+    else if constexpr (sizeof(V) == 8) return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX));
+    else return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, AUX));
+}
+constexpr int kStoreAux = 0;  // stores: default policy (streaming stores measured no faster end to end)
+template <typename V> __device__ __forceinline__ void bst(Rsrc r, uint32_t off, V v) {
+    if constexpr (sizeof(V) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, kStoreAux);
+    else if constexpr (sizeof(V) == 8) {
+        typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, off, 0, kStoreAux);
+    } else {
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, off, 0, kStoreAux);
+    }
+}
+
+// One tile (all its lines lie in one row: B divides the lines per row):
+// element (b, e) sits at byte offset b * ls + e * es from the tile's base,
+// both parts for interleaved data, the real part under re and the imaginary
+// part under im for planar data.
+struct FastRow {
+    Rsrc in_re, in_im, out_re, out_im;
+    bool in_planar, out_planar;
+    uint32_t line0;
+};
+
+// byte strides and byte span of one tile on the input (in = true) or output side
+template <typename T>
+__device__ __forceinline__ void fast_strides(const AfPass &d, bool in, uint32_t &es, uint32_t &ls, uint32_t &span) {
+    const bool planar = (in ? d.in_im : d.out_im) != nullptr;
+    const uint32_t S = planar ? (uint32_t)sizeof(T) : (uint32_t)(2 * sizeof(T));
+    es = (uint32_t)(in ? d.in_es : d.out_es) * S;
+    ls = (uint32_t)(in ? d.in_ls : d.out_ls) * S;
+    span = (uint32_t)(d.B - 1) * ls + (uint32_t)(d.L - 1) * es + S;
+}
+
+template <typename T, int B>
+__device__ __forceinline__ FastRow fast_row(const AfPass &d, int64_t tile) {
+    FastRow r;
+    const int64_t gl0 = tile * B;
+    const int64_t row = gl0 / d.nlines;
+    const int64_t line0 = gl0 - row * d.nlines;
+    const int64_t ib = row * d.in_bs + line0 * d.in_ls, ob = row * d.out_bs + line0 * d.out_ls;
+    uint32_t es, ls, span;
+    fast_strides<T>(d, true, es, ls, span);
+    r.in_planar = d.in_im != nullptr;
+    if (r.in_planar) {
+        r.in_re = mk_rsrc((const T *)d.in_re + ib, span);
+        r.in_im = mk_rsrc((const T *)d.in_im + ib, span);
+    } else {
+        r.in_re = r.in_im = mk_rsrc((const T *)d.in_re + 2 * ib, span);
+    }
+    fast_strides<T>(d, false, es, ls, span);
+    r.out_planar = d.out_im != nullptr;
+    if (r.out_planar) {
+        r.out_re = mk_rsrc((const T *)d.out_re + ob, span);
+        r.out_im = mk_rsrc((const T *)d.out_im + ob, span);
+    } else {
+        r.out_re = r.out_im = mk_rsrc((const T *)d.out_re + 2 * ob, span);
+    }
+    r.line0 = (uint32_t)line0;
+    return r;
+}
+
+// First stage inputs: one radix-16 butterfly per thread, straight from HBM.
+template <typename T, int L, int B>
+__device__ __forceinline__ void fast_load(const AfPass &d, const FastRow &r, typename Cx<T>::C (&v)[16]) {
+    using C = typename Cx<T>::C;
+    constexpr int NB = L / 16;
+    int b, i;
+    bf_map<B, NB>(threadIdx.x, d.in_lines_fast != 0, b, i);
+    uint32_t es, ls, span;
+    fast_strides<T>(d, true, es, ls, span);
+    const uint32_t o0 = (uint32_t)b * ls + (uint32_t)i * es, step = (uint32_t)NB * es;
+    // nontemporal loads only where the contiguous run of a tile row spans whole
+    // 128-byte lines (B lines side by side, or the elements of a row): shorter
+    // runs leave the rest of each line to neighbouring tiles, which then find it
+    // in L2
+    const bool nt = !d.in_lines_fast || B * (int)sizeof(T) >= 128;
+    if (r.in_planar && nt) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t off = o0 + (uint32_t)j * step;
+            v[j] = mk(bld<T, kNT>(r.in_re, off), bld<T, kNT>(r.in_im, off));
+        }
+    } else if (r.in_planar) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t off = o0 + (uint32_t)j * step;
+            v[j] = mk(bld<T, 0>(r.in_re, off), bld<T, 0>(r.in_im, off));
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = bld<C, 0>(r.in_re, o0 + (uint32_t)j * step);
+    }
+}
+
+// Last-stage outputs of one butterfly: element j goes to byte offset
+// o0 + j * step, after the four-step twiddle (mode 2: tw_e * tw_step^j, FP64,
+// rounded once), the scale (mode 1) or as they are (mode 0).
+template <typename T, bool PLANAR>
+__device__ __forceinline__ void fast_store(const FastRow &r, const int R, const typename Cx<T>::C *v, int mode, double2 tw_e,
+                                           const double2 tw_step, const T sc, const uint32_t o0, const uint32_t step) {
+    using C = typename Cx<T>::C;
+    auto put = [&](int j, C x) {
+        const uint32_t off = o0 + (uint32_t)j * step;
+        if (PLANAR) {
+            bst<T>(r.out_re, off, x.x);
+            bst<T>(r.out_im, off, x.y);
+        } else {
+            bst<C>(r.out_re, off, x);
+        }
+    };
+    if (mode == 2) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            put(j, cmul(v[j], mk(T(tw_e.x), T(tw_e.y))));
+            if (j + 1 < R) tw_e = cmul(tw_e, tw_step);
+        }
+    } else if (mode == 1) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) put(j, cscale(v[j], sc));
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) put(j, v[j]);
+    }
+}
+
+// W_N^(ftSign a) in FP64 from the coarse x fine tables, 32-bit indexing
+__device__ __forceinline__ double2 fast_ft(const AfPass &d, uint32_t a) {
+    const uint32_t mask = (1u << d.ftShift) - 1;
+    const double2 c = *(const double2 *)((const char *)d.ft_coarse_d + (size_t)((a >> d.ftShift) * 16u));
+    const double2 f = *(const double2 *)((const char *)d.ft_fine_d + (size_t)((a & mask) * 16u));
+    double2 w = cmul(c, f);
+    if (d.ftSign > 0) w.y = -w.y;
     return w;
 }
 
-template <typename T, int L, int B>
-__host__ __device__ constexpr size_t fast_lds_bytes(bool ft_lds, int ft_entries) {
-    return (size_t)fast_ld(L, B) * B * sizeof(typename Cx<T>::C) +
-           (size_t)(FastTables<L>::STAGE_ENTRIES + (ft_lds ? ft_entries : 0)) * 16;
-}
-
-template <typename T, int L, int B>
-__global__ void __launch_bounds__(B * L / 16)
-k_fast(const AfPass d) {
+// All stages of one tile whose first-stage inputs are in v; the last stage
+// stores to HBM.  SG is the transform sign, a
+// compile-time constant so that the products by +-i and the conjugations in
+// the butterflies and twiddles fold away.
+template <typename T, int L, int B, int SG>
+__device__ __forceinline__ void fast_stages(const AfPass &d, const FastRow &r, typename Cx<T>::C (&v)[16],
+                                            typename Cx<T>::C *lds, const typename Cx<T>::W *stab) {
     using C = typename Cx<T>::C;
-    using W = typename Cx<T>::W;
     using PL = P2Plan<L>;
     using FT = FastTables<L>;
     constexpr int NT = B * L / 16;
     constexpr int LD = fast_ld(L, B);
-    extern __shared__ __align__(16) unsigned char smem_fast[];
-    C *lds = (C *)smem_fast;
-    W *stab = (W *)(smem_fast + (size_t)LD * B * sizeof(C));
-    W *ftc = stab + FT::STAGE_ENTRIES;
     const int t = threadIdx.x;
-    const bool plain = (d.flags & AFP_PLAIN_TW) != 0;
     const bool posttw = (d.flags & AFP_POSTTW) != 0;
-    const bool ftl = posttw && ft_in_lds(d.ftN, d.ftShift);
-    const int ftT = 1 << d.ftShift;
-    const int ftNc = (int)((d.ftN + ftT - 1) >> d.ftShift);
-    W *ftf = ftc + (ftl ? ftNc : 0);
-    const T s = T(d.sign);
-    // tables into LDS; the first barrier below orders them before any use
-    if (FT::STAGE_LDS)
-        for (int i = t; i < L; i += NT) stab[i] = ((const W *)d.tw)[i];
-    if (ftl) {
-        for (int i = t; i < ftNc; i += NT) ftc[i] = ((const W *)d.ft_coarse)[i];
-        for (int i = t; i < ftT; i += NT) ftf[i] = ((const W *)d.ft_fine)[i];
-    }
-    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int64_t gl0 = tile * B;
-    // all lines of a tile lie in one row (B divides the lines per row)
-    const int64_t row = gl0 / d.nlines;
-    const int64_t line0 = gl0 - row * d.nlines;
-    const int64_t ib = row * d.in_bs + line0 * d.in_ls, ob = row * d.out_bs + line0 * d.out_ls;
-    const int ies = (int)d.in_es, oes = (int)d.out_es, ils = (int)d.in_ls, ols = (int)d.out_ls;
-    // row base pointers; element offsets below stay 32-bit (b * ls + e * es < 2^31)
-    const T *in_re = (const T *)d.in_re, *in_im = (const T *)d.in_im;
-    const C *in_c = (const C *)d.in_re;
-    T *out_re = (T *)d.out_re, *out_im = (T *)d.out_im;
-    C *out_c = (C *)d.out_re;
-    if (in_im) { in_re += ib; in_im += ib; } else { in_c += ib; }
-    if (out_im) { out_re += ob; out_im += ob; } else { out_c += ob; }
-
-    C v[16];
+    constexpr T s = T(SG);
+    uint32_t oes, ols, ospan;
+    fast_strides<T>(d, false, oes, ols, ospan);
 #pragma unroll
     for (int st = 0; st < PL::NST; ++st) {
         const int R = PL::radix(st);
@@ -147,28 +255,20 @@ k_fast(const AfPass d) {
         const int PER = 16 / R;        // butterflies per thread
         const bool first = st == 0, last = st == PL::NST - 1;
         const bool lf = first ? d.in_lines_fast != 0 : (last ? d.out_lines_fast != 0 : false);
-        // ---- gather inputs ----
+        // ---- gather inputs (the first stage's are already in v) ----
+        if (!first) {
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            int b, i;
-            if (R == 16) bf_map<B, L / 16>(t + q * NT, lf, b, i);
-            else if (R == 8) bf_map<B, L / 8>(t + q * NT, lf, b, i);
-            else if (R == 4) bf_map<B, L / 4>(t + q * NT, lf, b, i);
-            else bf_map<B, L / 2>(t + q * NT, lf, b, i);
+            for (int q = 0; q < PER; ++q) {
+                int b, i;
+                if (R == 16) bf_map<B, L / 16>(t + q * NT, lf, b, i);
+                else if (R == 8) bf_map<B, L / 8>(t + q * NT, lf, b, i);
+                else if (R == 4) bf_map<B, L / 4>(t + q * NT, lf, b, i);
+                else bf_map<B, L / 2>(t + q * NT, lf, b, i);
 #pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const int e = i + j * NB;
-                if (first) {
-                    const int off = b * ils + e * ies;
-                    if (in_im) v[q * R + j] = mk(__builtin_nontemporal_load(in_re + off),
-                                                 __builtin_nontemporal_load(in_im + off));
-                    else v[q * R + j] = in_c[off];
-                } else {
-                    v[q * R + j] = lds[b * LD + fpad(e)];
-                }
+                for (int j = 0; j < R; ++j) v[q * R + j] = lds[b * LD + fpad(i + j * NB)];
             }
+            lds_barrier();
         }
-        if (!first) __syncthreads();
         // ---- twiddle, butterfly, scatter ----
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
@@ -182,15 +282,15 @@ k_fast(const AfPass d) {
                 const int tws = L / (p * R);
                 if (FT::STAGE_LDS) {
 #pragma unroll
-                    for (int j = 1; j < R; ++j) v[q * R + j] = twmul(v[q * R + j], stab[j * k * tws], s, plain);
+                    for (int j = 1; j < R; ++j) v[q * R + j] = twmul(v[q * R + j], stab[j * k * tws], s, false);
                 } else {
                     // w^j for j = 1..R-1 from one FP64 load and FP64 powers
                     double2 w1 = ((const double2 *)d.tw_d)[k * tws];
-                    if (d.sign > 0) w1.y = -w1.y;
+                    if (SG > 0) w1.y = -w1.y;
                     double2 wj = w1;
 #pragma unroll
                     for (int j = 1; j < R; ++j) {
-                        v[q * R + j] = dmul(v[q * R + j], wj, plain);
+                        v[q * R + j] = dmul(v[q * R + j], wj, false);
                         if (j + 1 < R) wj = cmul(wj, w1);
                     }
                 }
@@ -204,56 +304,62 @@ k_fast(const AfPass d) {
 #pragma unroll
                 for (int j = 0; j < R; ++j) lds[b * LD + fpad(o + j * p)] = v[q * R + j];
             } else {
-                const uint32_t l = (uint32_t)(line0 + b);
-                // four-step twiddles W_N^(l e), e = i + j NB (l * e < N <= 2^30)
+                const uint32_t l = r.line0 + (uint32_t)b;
+                const uint32_t o0 = (uint32_t)b * ols + (uint32_t)i * oes, ostep = (uint32_t)NB * oes;
                 double2 tw_e = make_double2(1.0, 0.0), tw_step = tw_e;
-                const uint32_t mask = (uint32_t)(d.ftN - 1);
-                if (posttw && !ftl) {
-                    tw_e = ft_twiddle_d(d, (l * (uint32_t)i) & mask);
-                    tw_step = ft_twiddle_d(d, (l * (uint32_t)NB) & mask);
+                int mode = (d.flags & AFP_SCALE) ? 1 : 0;
+                if (posttw) {
+                    // four-step twiddles W_N^(l e), e = i + j NB (l * e < N <= 2^30), with the
+                    // output scale folded into the FP64 start value
+                    const uint32_t mask = (uint32_t)(d.ftN - 1);
+                    tw_e = fast_ft(d, (l * (uint32_t)i) & mask);
+                    tw_step = fast_ft(d, (l * (uint32_t)NB) & mask);
+                    if (mode) tw_e = make_double2(tw_e.x * d.scale, tw_e.y * d.scale);
+                    mode = 2;
                 }
-#pragma unroll
-                for (int j = 0; j < R; ++j) {
-                    const int e = i + j * NB;
-                    C x = v[q * R + j];
-                    if (posttw) {
-                        if (ftl) {
-                            x = cmul(x, ft_prod(ftc, ftf, (l * (uint32_t)e) & mask, d.ftShift, d.ftSign));
-                        } else {
-                            // the FP64 value rounded once to the working precision
-                            x = cmul(x, mk(T(tw_e.x), T(tw_e.y)));
-                            if (j + 1 < R) tw_e = cmul(tw_e, tw_step);
-                        }
-                    }
-                    if (d.flags & AFP_SCALE) x = cscale(x, T(d.scale));
-                    const int off = b * ols + e * oes;
-                    if (out_im) {
-                        out_re[off] = x.x;
-                        out_im[off] = x.y;
-                    } else {
-                        out_c[off] = x;
-                    }
-                }
+                if (r.out_planar) fast_store<T, true>(r, R, &v[q * R], mode, tw_e, tw_step, T(d.scale), o0, ostep);
+                else fast_store<T, false>(r, R, &v[q * R], mode, tw_e, tw_step, T(d.scale), o0, ostep);
             }
         }
-        if (!last) __syncthreads();
+        if (!last) lds_barrier();
     }
+}
+
+// One launch of a power-of-two pass: one workgroup per tile.
+template <typename T, int L, int B, int SG>
+__global__ void __launch_bounds__(B * L / 16)
+k_fast(const AfPass d) {
+    using C = typename Cx<T>::C;
+    using W = typename Cx<T>::W;
+    using FT = FastTables<L>;
+    constexpr int NT = B * L / 16;
+    constexpr int LD = fast_ld(L, B);
+    extern __shared__ __align__(16) unsigned char smem_fast[];
+    C *lds = (C *)smem_fast;
+    W *stab = (W *)(smem_fast + (size_t)LD * B * sizeof(C));
+    // stage table into LDS; the first barrier of the stages orders it before any use
+    if (FT::STAGE_LDS)
+        for (int i = threadIdx.x; i < L; i += NT) stab[i] = ((const W *)d.tw)[i];
+    const FastRow r = fast_row<T, B>(d, xcd_tile(blockIdx.x, gridDim.x));
+    C v[16];
+    fast_load<T, L, B>(d, r, v);
+    fast_stages<T, L, B, SG>(d, r, v, lds, stab);
 }
 
 template <typename T, int L, int B>
 static bool try_launch(const AfPass &d, hipStream_t s, hipError_t &err) {
     if (d.L != L || d.B != B) return false;
-    const bool ftl = (d.flags & AFP_POSTTW) && ft_in_lds(d.ftN, d.ftShift);
-    const int T_ = 1 << d.ftShift;
-    const int ft_entries = ftl ? (int)((d.ftN + T_ - 1) / T_) + T_ : 0;
-    const size_t lds = fast_lds_bytes<T, L, B>(ftl, ft_entries);
-    hipLaunchKernelGGL((k_fast<T, L, B>), dim3((unsigned)d.ntiles), dim3(B * L / 16), lds, s, d);
+    const unsigned g = (unsigned)d.ntiles;
+    const size_t lds = fast_lds_bytes<T, L, B>();
+    if (d.sign < 0) hipLaunchKernelGGL((k_fast<T, L, B, -1>), dim3(g), dim3(B * L / 16), lds, s, d);
+    else hipLaunchKernelGGL((k_fast<T, L, B, 1>), dim3(g), dim3(B * L / 16), lds, s, d);
     err = hipGetLastError();
     return true;
 }
 
-// Flags the fast kernel implements; anything else goes to the generic pass.
-static const int32_t kFastFlags = AFP_POSTTW | AFP_SCALE | AFP_PLAIN_TW;
+// Flags the fast kernel implements; anything else (plain FP32 twiddles
+// included) goes to the generic pass.
+static const int32_t kFastFlags = AFP_POSTTW | AFP_SCALE;
 
 bool fast_pass_eligible(int prec, const AfPass &d) {
     if (d.flags & ~kFastFlags) return false;
@@ -264,14 +370,13 @@ bool fast_pass_eligible(int prec, const AfPass &d) {
     if ((d.flags & AFP_POSTTW) && (d.ftN & (d.ftN - 1))) return false;
     if (d.nlines % d.B) return false;
     if (d.ntiles >= ((int64_t)1 << 31)) return false;
-    const int64_t row_span = (int64_t)(d.B - 1) * d.in_ls + (int64_t)(d.L - 1) * d.in_es;
-    const int64_t row_span_o = (int64_t)(d.B - 1) * d.out_ls + (int64_t)(d.L - 1) * d.out_es;
-    if (row_span >= ((int64_t)1 << 31) || row_span_o >= ((int64_t)1 << 31)) return false;
-    const int T_ = (d.flags & AFP_POSTTW) ? 1 << d.ftShift : 0;
-    const bool ftl = (d.flags & AFP_POSTTW) && ft_in_lds(d.ftN, d.ftShift);
-    const int64_t ft_entries = ftl ? (d.ftN + T_ - 1) / T_ + T_ : 0;
-    const int64_t lds = (int64_t)fast_ld(d.L, d.B) * d.B * (prec == PREC_F64 ? 16 : 8) +
-                        ((d.L <= 1024 ? d.L : 0) + ft_entries) * 16;
+    // byte offsets inside a tile and into the FP64 four-step tables stay 32-bit
+    const int64_t eb = prec == PREC_F64 ? 8 : 4;
+    const int64_t span_i = ((int64_t)(d.B - 1) * d.in_ls + (int64_t)(d.L - 1) * d.in_es) * (d.in_im ? eb : 2 * eb) + eb;
+    const int64_t span_o = ((int64_t)(d.B - 1) * d.out_ls + (int64_t)(d.L - 1) * d.out_es) * (d.out_im ? eb : 2 * eb) + eb;
+    if (span_i >= ((int64_t)1 << 32) || span_o >= ((int64_t)1 << 32)) return false;
+    if ((d.flags & AFP_POSTTW) && (d.ftN >> d.ftShift) * 16 + 16 >= ((int64_t)1 << 32)) return false;
+    const int64_t lds = (int64_t)fast_ld(d.L, d.B) * d.B * (prec == PREC_F64 ? 16 : 8) + (d.L <= 1024 ? d.L : 0) * 16;
     if (lds > 160 * 1024) return false;
     return (int64_t)d.B * d.L / 16 <= 1024;
 }
@@ -297,7 +402,8 @@ hipError_t launch_fast(int prec, const AfPass &d, hipStream_t s, bool &done) {
 
 template <typename T, int L, int B>
 static void set_attr() {
-    (void)hipFuncSetAttribute((const void *)k_fast<T, L, B>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void *)k_fast<T, L, B, -1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void *)k_fast<T, L, B, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
 
 void init_fast_attributes() {
