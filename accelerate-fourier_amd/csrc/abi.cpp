@@ -163,7 +163,7 @@ int choose_B_rows(int prec, int64_t L, int64_t nlines_total) {
     while (B < 64 && (2 * B) * L <= target && tile_fits(prec, (int)L, 2 * B) &&
            (nlines_total + 2 * B - 1) / (2 * B) >= 512)
         B *= 2;
-    return B;
+    return fast_tile_lines(prec, L, B);
 }
 
 // Lines per tile for passes whose loads or stores are strided: runs of 128
@@ -171,7 +171,7 @@ int choose_B_rows(int prec, int64_t L, int64_t nlines_total) {
 int choose_B_cols(int prec, int64_t L, bool planar = false) {
     int B = (prec == AF_F64 ? 8 : 16) * (planar ? 2 : 1);
     while (B > 1 && !tile_fits(prec, (int)L, B)) B /= 2;
-    return B;
+    return fast_tile_lines(prec, L, B);
 }
 
 struct Endpoint {

@@ -381,22 +381,42 @@ bool fast_pass_eligible(int prec, const AfPass &d) {
     return (int64_t)d.B * d.L / 16 <= 1024;
 }
 
+// Tile shapes with a power-of-two kernel: lengths 256 ... 4096 and line
+// counts B from one wavefront (B * L / 16 >= 64 threads) up to what one tile
+// holds in LDS.  X(type, L, B).
+#define AF_FAST_SHAPES(X)                                                                          \
+    X(float, 256, 4) X(float, 256, 8) X(float, 256, 16) X(float, 256, 32)                          \
+    X(float, 512, 2) X(float, 512, 4) X(float, 512, 8) X(float, 512, 16) X(float, 512, 32)        \
+    X(float, 1024, 1) X(float, 1024, 2) X(float, 1024, 4) X(float, 1024, 8) X(float, 1024, 16)     \
+    X(float, 2048, 1) X(float, 2048, 2) X(float, 2048, 4) X(float, 2048, 8)                        \
+    X(float, 4096, 1) X(float, 4096, 2) X(float, 4096, 4)                                          \
+    X(double, 256, 4) X(double, 256, 8) X(double, 256, 16)                                         \
+    X(double, 512, 2) X(double, 512, 4) X(double, 512, 8) X(double, 512, 16)                       \
+    X(double, 1024, 1) X(double, 1024, 2) X(double, 1024, 4) X(double, 1024, 8)                    \
+    X(double, 2048, 1) X(double, 2048, 2) X(double, 2048, 4)                                       \
+    X(double, 4096, 1) X(double, 4096, 2)
+
+static bool fast_has_shape(int prec, int L, int B) {
+#define AF_HAS(T, L_, B_) if ((prec == PREC_F64) == (sizeof(T) == 8) && L == L_ && B == B_) return true;
+    AF_FAST_SHAPES(AF_HAS)
+#undef AF_HAS
+    return false;
+}
+
+int fast_tile_lines(int prec, int64_t L, int B) {
+    if (L < 256 || L > 4096 || (L & (L - 1))) return B;
+    for (int b = B; b >= 1; b /= 2)
+        if (fast_has_shape(prec, (int)L, b)) return b;
+    return B;
+}
+
 hipError_t launch_fast(int prec, const AfPass &d, hipStream_t s, bool &done) {
     hipError_t err = hipSuccess;
     done = false;
     if (!fast_pass_eligible(prec, d)) return hipSuccess;
-    if (prec == PREC_F32) {
-        done = try_launch<float, 256, 16>(d, s, err) || try_launch<float, 256, 32>(d, s, err) || try_launch<float, 256, 4>(d, s, err) ||
-               try_launch<float, 256, 1>(d, s, err) || try_launch<float, 512, 8>(d, s, err) ||
-               try_launch<float, 1024, 4>(d, s, err) || try_launch<float, 1024, 1>(d, s, err) ||
-               try_launch<float, 2048, 2>(d, s, err) || try_launch<float, 4096, 4>(d, s, err) ||
-               try_launch<float, 4096, 1>(d, s, err);
-    } else {
-        done = try_launch<double, 256, 8>(d, s, err) || try_launch<double, 256, 2>(d, s, err) ||
-               try_launch<double, 256, 1>(d, s, err) || try_launch<double, 512, 4>(d, s, err) ||
-               try_launch<double, 1024, 2>(d, s, err) || try_launch<double, 1024, 1>(d, s, err) ||
-               try_launch<double, 2048, 1>(d, s, err) || try_launch<double, 4096, 2>(d, s, err);
-    }
+#define AF_TRY(T, L_, B_) if (!done && (prec == PREC_F64) == (sizeof(T) == 8)) done = try_launch<T, L_, B_>(d, s, err);
+    AF_FAST_SHAPES(AF_TRY)
+#undef AF_TRY
     return err;
 }
 
@@ -407,11 +427,9 @@ static void set_attr() {
 }
 
 void init_fast_attributes() {
-    set_attr<float, 256, 16>(); set_attr<float, 256, 32>(); set_attr<float, 256, 4>(); set_attr<float, 256, 1>(); set_attr<float, 512, 8>();
-    set_attr<float, 1024, 4>(); set_attr<float, 1024, 1>(); set_attr<float, 2048, 2>(); set_attr<float, 4096, 4>();
-    set_attr<float, 4096, 1>();
-    set_attr<double, 256, 8>(); set_attr<double, 256, 2>(); set_attr<double, 256, 1>(); set_attr<double, 512, 4>();
-    set_attr<double, 1024, 2>(); set_attr<double, 1024, 1>(); set_attr<double, 2048, 1>(); set_attr<double, 4096, 2>();
+#define AF_ATTR(T, L_, B_) set_attr<T, L_, B_>();
+    AF_FAST_SHAPES(AF_ATTR)
+#undef AF_ATTR
 }
 
 }  // namespace af

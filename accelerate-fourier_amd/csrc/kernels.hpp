@@ -35,6 +35,9 @@ void init_karatsuba_attributes();
 void init_fast_attributes();
 // Power-of-two passes with compile-time stages (fast_pass.hip); done = false when not eligible.
 hipError_t launch_fast(int prec, const AfPass &d, hipStream_t s, bool &done);
+// the largest line count <= B with a power-of-two kernel for lines of length L
+// (B itself when L has none)
+int fast_tile_lines(int prec, int64_t L, int B);
 
 // Karatsuba (karatsuba.hip)
 hipError_t launch_karatsuba(const int64_t *x, const int64_t *y, int64_t *z, const int64_t *dev_meta, int64_t npairs,

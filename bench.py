@@ -28,6 +28,9 @@ for _p in (os.path.join(REPO, "accelerate-fourier_amd"), os.path.join(REPO, "ora
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+# the two launches of the 64 x 65536 four-step transform (65536 = 256 x 256)
+PASS_NAMES = {0: "pass 0: 256-point column FFTs + four-step twiddles",
+              1: "pass 1: 256-point row FFTs, transposed store"}
 METRIC = "FP32 batched FFT GFLOP/s(5NlgN) & %HBM peak vs FP64 plain; fwd+bwd conv/s"
 
 
@@ -110,6 +113,21 @@ def time_it(fn, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
+def profiled_traffic(pass_index):
+    """HBM bytes per launch of one pass of the headline transform, from the
+    committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, scaled by
+    what the same counters read for a copy of known size); None if absent."""
+    path = os.path.join(REPO, "profiles", "c2a_64x65536_f32_hbm_counters.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(f"pass {pass_index} traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def rel_l2(a, b):
     import numpy as np
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
@@ -148,6 +166,8 @@ def main():
                           "copy_bytes_each_way": 32 * 2 ** 20}))
         return
     ms = time_it(f32.run, steps, warmup)
+    # average launch duration of each pass: HIP events around `steps` launches of
+    # that pass alone, on the stream the kernels run on
     pass_ms = [time_it(lambda i=i: f32.run_pass(i), steps, warmup) for i in range(f32.npasses)]
     dom = int(np.argmax(pass_ms))
     # every pass of the four-step transform reads and writes the whole array once
@@ -170,8 +190,9 @@ def main():
         "hbm_GBps_end_to_end": alg_bytes / (ms * 1e-3) / 1e9,
         "hbm_frac_end_to_end": alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_pass<float> pass {dom} of {f32.npasses}",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom),
+                     "algorithmic_bytes": alg_bytes,
+                     "kernel": "k_fast<float, 256, 32, -1> " + PASS_NAMES.get(dom, f"pass {dom}"),
                      "pass_ms": pass_ms},
     }
     y32 = f32.result()
