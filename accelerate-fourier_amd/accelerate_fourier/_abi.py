@@ -35,9 +35,9 @@ class PlanDesc(ctypes.Structure):
 
 
 class AfError(RuntimeError):
-    def __init__(self, status: int, what: str):
+    def __init__(self, status: int, what: str, detail: str = ""):
         self.status = status
-        super().__init__(f"{what}: {status_string(status)} (status {status})")
+        super().__init__(f"{what}: {status_string(status)} (status {status})" + (f" [{detail}]" if detail else ""))
 
 
 _lib = None
@@ -49,6 +49,7 @@ _p64 = ctypes.POINTER(ctypes.c_int64)
 _SIGNATURES = {
     "af_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "af_version": (ctypes.c_char_p, []),
+    "af_last_error": (ctypes.c_char_p, []),
     "af_plan_create": (ctypes.c_int, [ctypes.POINTER(PlanDesc), ctypes.POINTER(_v)]),
     "af_plan_destroy": (ctypes.c_int, [_v]),
     "af_plan_describe": (ctypes.c_int, [ctypes.POINTER(PlanDesc), ctypes.c_char_p, ctypes.c_size_t]),
@@ -106,7 +107,8 @@ def status_string(status: int) -> str:
 
 def check(status: int, what: str) -> None:
     if status != SUCCESS:
-        raise AfError(status, what)
+        detail = lib().af_last_error().decode() if status in (HIP_ERROR, OUT_OF_MEMORY) else ""
+        raise AfError(status, what, detail)
 
 
 def int64_array(values):

@@ -24,13 +24,22 @@ using namespace af;
 
 namespace {
 
-af_status hip_status(hipError_t e) { return e == hipErrorOutOfMemory ? AF_OUT_OF_MEMORY : AF_HIP_ERROR; }
+// text of the last failed HIP call on this thread (af_last_error)
+thread_local char g_last_error[256] = "";
 
-#define HIPCHK(x)                                       \
-    do {                                                \
-        hipError_t e__ = (x);                           \
-        if (e__ != hipSuccess) return hip_status(e__);  \
+af_status hip_status(hipError_t e, const char *call, int line) {
+    std::snprintf(g_last_error, sizeof g_last_error, "%s failed at abi.cpp:%d: %s", call, line, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? AF_OUT_OF_MEMORY : AF_HIP_ERROR;
+}
+
+#define HIPCHK(x)                                                 \
+    do {                                                          \
+        hipError_t e__ = (x);                                     \
+        if (e__ != hipSuccess) return hip_status(e__, #x, __LINE__); \
     } while (0)
+// status of a HIP call, recording the failed call's text
+af_status hip_try(hipError_t e, const char *call, int line) { return e == hipSuccess ? AF_SUCCESS : hip_status(e, call, line); }
+#define HIPST(x) hip_try((x), #x, __LINE__)
 #define AFCHK(x)                                 \
     do {                                         \
         af_status s__ = (x);                     \
@@ -470,7 +479,7 @@ af_status fft_f64_forward(int64_t N, const void *in, void *out, hipStream_t s) {
     af_plan p = nullptr;
     AFCHK(af_plan_create(&d, &p));
     af_status st = af_execute(p, 1, in, nullptr, out, nullptr, nullptr, s);
-    if (st == AF_SUCCESS) st = hipStreamSynchronize(s) == hipSuccess ? AF_SUCCESS : AF_HIP_ERROR;
+    if (st == AF_SUCCESS) st = HIPST(hipStreamSynchronize(s));
     af_plan_destroy(p);
     return st;
 }
@@ -494,16 +503,15 @@ af_status build_rader(af_plan P, hipStream_t s) {
     HIPCHK(hipMalloc(&kb, L * 16));
     HIPCHK(hipMalloc(&kf, L * 16));
     af_status st = AF_SUCCESS;
-    if (gen_rader_kernel_d(kb, P->out_map, p, L, P->desc.sign, s) != hipSuccess) st = AF_HIP_ERROR;
+    { const af_status h__ = HIPST(gen_rader_kernel_d(kb, P->out_map, p, L, P->desc.sign, s)); if (st == AF_SUCCESS) st = h__; }
     if (st == AF_SUCCESS) st = fft_f64_forward(L, kb, kf, s);
     if (st == AF_SUCCESS) st = P->own.alloc(&P->spec, L * 16);
     if (st == AF_SUCCESS) {
         const bool four = P->core.sh.kind == CORE_FOUR;
-        if (convert_to_w(P->prec, P->spec, kf, L, 1.0 / (double)L, four ? P->core.sh.N1 : 0,
-                         four ? P->core.sh.N2 : 0, s) != hipSuccess)
-            st = AF_HIP_ERROR;
+        st = HIPST(convert_to_w(P->prec, P->spec, kf, L, 1.0 / (double)L, four ? P->core.sh.N1 : 0,
+                                four ? P->core.sh.N2 : 0, s));
     }
-    if (hipStreamSynchronize(s) != hipSuccess && st == AF_SUCCESS) st = AF_HIP_ERROR;
+    { const af_status h__ = HIPST(hipStreamSynchronize(s)); if (st == AF_SUCCESS) st = h__; }
     (void)hipFree(kb);
     (void)hipFree(kf);
     return st;
@@ -517,19 +525,18 @@ af_status build_bluestein(af_plan P, hipStream_t s) {
     HIPCHK(hipMalloc(&hf, M * 16));
     af_status st = AF_SUCCESS;
     // chirp c_k = exp(sign i pi k^2 / n), k < n; kernel h = conj(c) wrapped over M
-    if (gen_chirp_d(cd, n, n, P->desc.sign, 0, 0, s) != hipSuccess) st = AF_HIP_ERROR;
+    { const af_status h__ = HIPST(gen_chirp_d(cd, n, n, P->desc.sign, 0, 0, s)); if (st == AF_SUCCESS) st = h__; }
     if (st == AF_SUCCESS) st = P->own.alloc(&P->chirp, n * 16);
-    if (st == AF_SUCCESS && convert_to_w(P->prec, P->chirp, cd, n, 1.0, 0, 0, s) != hipSuccess) st = AF_HIP_ERROR;
-    if (st == AF_SUCCESS && gen_chirp_d(hb, n, M, P->desc.sign, M, 1, s) != hipSuccess) st = AF_HIP_ERROR;
+    if (st == AF_SUCCESS) st = HIPST(convert_to_w(P->prec, P->chirp, cd, n, 1.0, 0, 0, s));
+    if (st == AF_SUCCESS) st = HIPST(gen_chirp_d(hb, n, M, P->desc.sign, M, 1, s));
     if (st == AF_SUCCESS) st = fft_f64_forward(M, hb, hf, s);
     if (st == AF_SUCCESS) st = P->own.alloc(&P->spec, M * 16);
     if (st == AF_SUCCESS) {
         const bool four = P->core.sh.kind == CORE_FOUR;
-        if (convert_to_w(P->prec, P->spec, hf, M, 1.0 / (double)M, four ? P->core.sh.N1 : 0,
-                         four ? P->core.sh.N2 : 0, s) != hipSuccess)
-            st = AF_HIP_ERROR;
+        st = HIPST(convert_to_w(P->prec, P->spec, hf, M, 1.0 / (double)M, four ? P->core.sh.N1 : 0,
+                                four ? P->core.sh.N2 : 0, s));
     }
-    if (hipStreamSynchronize(s) != hipSuccess && st == AF_SUCCESS) st = AF_HIP_ERROR;
+    { const af_status h__ = HIPST(hipStreamSynchronize(s)); if (st == AF_SUCCESS) st = h__; }
     (void)hipFree(cd);
     (void)hipFree(hb);
     (void)hipFree(hf);
@@ -558,6 +565,8 @@ const char *af_status_string(af_status s) {
 }
 
 const char *af_version(void) { return "accelerate-fourier-amd 0.1 (gfx950)"; }
+
+const char *af_last_error(void) { return g_last_error; }
 
 af_status af_plan_describe(const af_plan_desc *desc, char *buf, size_t buflen) {
     AFCHK(validate_desc(desc));
@@ -640,10 +649,9 @@ static af_status plan_create_impl(const af_plan_desc *desc, af_plan *plan, bool 
     if (st == AF_SUCCESS &&
         (desc->kind == AF_R2C_FULL || desc->kind == AF_C2R_FULL || desc->kind == AF_REAL_CONV) && sh.n >= 1) {
         st = P->own.alloc(&P->aux_tw, (size_t)sh.n * 16);
-        if (st == AF_SUCCESS && gen_twiddles(P->prec, P->aux_tw, 2 * sh.n, sh.n, 1, s) != hipSuccess)
-            st = AF_HIP_ERROR;
+        if (st == AF_SUCCESS) st = HIPST(gen_twiddles(P->prec, P->aux_tw, 2 * sh.n, sh.n, 1, s));
     }
-    if (hipStreamSynchronize(s) != hipSuccess && st == AF_SUCCESS) st = AF_HIP_ERROR;
+    { const af_status h__ = HIPST(hipStreamSynchronize(s)); if (st == AF_SUCCESS) st = h__; }
     (void)hipStreamDestroy(s);
     if (st != AF_SUCCESS) return st;
     *plan = P.release();
@@ -1035,11 +1043,10 @@ af_status karatsuba_run(const std::vector<int64_t> &meta_in, int64_t npairs, int
     void *dm = nullptr;
     HIPCHK(hipMallocAsync(&dm, meta.size() * 8, s));
     af_status st = AF_SUCCESS;
-    if (hipMemcpyAsync(dm, meta.data(), meta.size() * 8, hipMemcpyHostToDevice, s) != hipSuccess) st = AF_HIP_ERROR;
+    { const af_status h__ = HIPST(hipMemcpyAsync(dm, meta.data(), meta.size() * 8, hipMemcpyHostToDevice, s)); if (st == AF_SUCCESS) st = h__; }
     // the host vector dies with this frame: wait for the copy before returning
-    if (st == AF_SUCCESS && hipStreamSynchronize(s) != hipSuccess) st = AF_HIP_ERROR;
-    if (st == AF_SUCCESS && launch_karatsuba(x, y, z, (const int64_t *)dm, npairs, maxP, cyclic, s) != hipSuccess)
-        st = AF_HIP_ERROR;
+    if (st == AF_SUCCESS) st = HIPST(hipStreamSynchronize(s));
+    if (st == AF_SUCCESS) st = HIPST(launch_karatsuba(x, y, z, (const int64_t *)dm, npairs, maxP, cyclic, s));
     (void)hipFreeAsync(dm, s);
     return st;
 }
@@ -1280,15 +1287,14 @@ af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign, i
     hipStream_t s = nullptr;
     HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     af_status st = AF_SUCCESS;
-    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) st = AF_HIP_ERROR;
+    { const af_status h__ = HIPST(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)); if (st == AF_SUCCESS) st = h__; }
     for (const auto &g : B->groups) {
         if (st != AF_SUCCESS) break;
-        if (launch_pass_grouped(B->prec, B->d_descs + g.desc_off, B->d_prefix + g.prefix_off, (int)g.rows.size(),
-                                g.total_tiles, g.nthreads, g.lds, s) != hipSuccess)
-            st = AF_HIP_ERROR;
+        st = HIPST(launch_pass_grouped(B->prec, B->d_descs + g.desc_off, B->d_prefix + g.prefix_off,
+                                       (int)g.rows.size(), g.total_tiles, g.nthreads, g.lds, s));
     }
     hipGraph_t graph = nullptr;
-    if (hipStreamEndCapture(s, &graph) != hipSuccess && st == AF_SUCCESS) st = AF_HIP_ERROR;
+    { const af_status h__ = HIPST(hipStreamEndCapture(s, &graph)); if (st == AF_SUCCESS) st = h__; }
     (void)hipStreamDestroy(s);
     if (st != AF_SUCCESS) {
         if (graph) (void)hipGraphDestroy(graph);

@@ -321,9 +321,18 @@ void init_kernel_attributes() {
 
 static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// LDS of one workgroup of k_pass / k_pass_grouped: the dynamic tile plus the
+// static line table and Rader slots.  A launch above the 160 KiB a workgroup
+// can hold is refused here instead of reaching the GPU.
+static bool pass_lds_fits(int prec, int64_t dynamic_bytes) {
+    const int64_t stat = (int64_t)sizeof(TileLines) + 2 * 64 * (prec == PREC_F64 ? 16 : 8);
+    return dynamic_bytes >= 0 && dynamic_bytes + stat <= 160 * 1024;
+}
+
 template <typename T>
 static hipError_t launch_pass_t(const AfPass &d, hipStream_t s) {
     if (d.ntiles <= 0) return hipSuccess;
+    if (!pass_lds_fits(sizeof(T) == 8 ? PREC_F64 : PREC_F32, d.lds_bytes) || d.B > 64) return hipErrorInvalidValue;
     const int64_t g = d.ntiles < (int64_t)1 << 30 ? d.ntiles : (int64_t)1 << 30;
     hipLaunchKernelGGL(k_pass<T>, dim3((unsigned)g), dim3(d.nthreads), d.lds_bytes, s, d);
     return hipGetLastError();
@@ -340,6 +349,7 @@ hipError_t launch_pass(int prec, const AfPass &d, hipStream_t s) {
 hipError_t launch_pass_grouped(int prec, const AfPass *descs, const int64_t *prefix, int ndesc,
                                int64_t total_tiles, int nthreads, int lds_bytes, hipStream_t s) {
     if (total_tiles <= 0) return hipSuccess;
+    if (!pass_lds_fits(prec, lds_bytes)) return hipErrorInvalidValue;
     const int64_t g = total_tiles < (int64_t)1 << 30 ? total_tiles : (int64_t)1 << 30;
     if (prec == PREC_F64)
         hipLaunchKernelGGL(k_pass_grouped<double>, dim3((unsigned)g), dim3(nthreads), lds_bytes, s, descs, prefix, ndesc);

@@ -107,7 +107,10 @@ bool stockham_radices(int64_t n, std::vector<int> &radices) {
 
 int elem_bytes(int prec) { return prec == AF_F64 ? 16 : 8; }
 int64_t lds_max_len(int prec) { return prec == AF_F64 ? 8192 : 16384; }
-int lds_tile_bytes_cap() { return 160 * 1024; }
+// Dynamic LDS budget of one tile of the generic pass: 160 KiB per workgroup
+// minus 4 KiB for that kernel's static arrays (its line table and Rader slots,
+// 3328 bytes in FP64).
+int lds_tile_bytes_cap() { return 156 * 1024; }
 
 CoreShape core_shape(int prec, int64_t N) {
     CoreShape c;

@@ -80,6 +80,11 @@ typedef struct af_batch_s *af_batch;
 
 const char *af_status_string(af_status s);
 const char *af_version(void);
+/* The failed HIP call behind the last AF_HIP_ERROR / AF_OUT_OF_MEMORY returned on
+ * the calling thread ("" if none): the call, its source line and HIP's error
+ * string.  The reference reports errors as Accelerate runtime exceptions; this
+ * is the ABI's text for them. */
+const char *af_last_error(void);
 
 /*
  * Plan creation: host-side planning (Planned.plan, Planned.hs:201-220) plus

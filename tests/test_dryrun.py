@@ -1,0 +1,198 @@
+"""Launch checks without a GPU.
+
+The library is built host-only against tests/dryrun/fake_hip.cpp: kernels do
+not run, but every launch's memory footprint (from its pass descriptor or
+scalar arguments) is checked against the live allocations, and its launch shape
+against what the kernel assumes (thread count, LDS size, tile geometry).  Plans
+are created and executed for lengths across every algorithm and for the
+4096-row uneven batch, in both precisions.
+"""
+import ctypes
+import importlib.util
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+BUILD = os.path.join(HERE, "dryrun", "_build")
+LIB = os.path.join(BUILD, "libaf_dryrun.so")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")),
+                                reason="hipcc is needed to build the host-only library")
+
+V = ctypes.c_void_p
+C2C, R2C_FULL, C2R_FULL, CONV_CYCLIC, REAL_CONV = range(5)
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [("length", ctypes.c_int64), ("sign", ctypes.c_int32), ("kind", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("algorithm", ctypes.c_int32), ("flags", ctypes.c_uint32)]
+
+
+@pytest.fixture(scope="module")
+def dry():
+    r = subprocess.run(["bash", os.path.join(HERE, "dryrun", "build.sh"), BUILD], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lib = ctypes.CDLL(LIB)
+    lib.dry_problem.restype = ctypes.c_char_p
+    lib.dry_problem_count.restype = ctypes.c_long
+    lib.dry_launches.restype = ctypes.c_long
+    lib.dry_kernel_summary.restype = ctypes.c_char_p
+    lib.hipMalloc.argtypes = [ctypes.POINTER(V), ctypes.c_size_t]
+    lib.hipFree.argtypes = [V]
+    lib.af_plan_create.argtypes = [ctypes.POINTER(PlanDesc), ctypes.POINTER(V)]
+    lib.af_plan_destroy.argtypes = [V]
+    lib.af_execute.argtypes = [V, ctypes.c_int64, V, V, V, V, V, V]
+    lib.af_vjp_execute.argtypes = [V, ctypes.c_int64, V, V, V, V, V, V]
+    lib.af_to_spectrum.argtypes = [V, ctypes.c_int64, V, V, V, V, V]
+    lib.af_from_spectrum.argtypes = [V, ctypes.c_int64, V, V, V, V, V]
+    lib.af_convolve_cyclic.argtypes = [V, ctypes.c_int64, V, V, V, V, V, V, V, V]
+    lib.af_real_convolve.argtypes = [V, ctypes.c_int64, V, V, V, V]
+    lib.af_vjp_real_convolve.argtypes = [V, ctypes.c_int64, V, V, V, V, V, V]
+    lib.af_batch_create.argtypes = [ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.POINTER(V)]
+    lib.af_batch_execute.argtypes = [V, V, V, V, V, V]
+    lib.af_batch_destroy.argtypes = [V]
+    return lib
+
+
+class Bufs:
+    def __init__(self, lib):
+        self.lib, self.ptrs = lib, []
+
+    def __call__(self, nbytes):
+        p = V()
+        assert self.lib.hipMalloc(ctypes.byref(p), max(int(nbytes), 1)) == 0
+        self.ptrs.append(p)
+        return p
+
+    def free(self):
+        for p in self.ptrs:
+            self.lib.hipFree(p)
+        self.ptrs = []
+
+
+def problems(lib, start=0):
+    n = lib.dry_problem_count()
+    return [lib.dry_problem(i).decode() for i in range(start, n)]
+
+
+def run_c2c(lib, n, sign, prec, batch, planar, alg=0):
+    e = 16 if prec == 0 else 8
+    d = PlanDesc(n, sign, C2C, prec, alg, 0)
+    p = V()
+    st = lib.af_plan_create(ctypes.byref(d), ctypes.byref(p))
+    if st != 0:
+        return st
+    b = Bufs(lib)
+    h = e // 2
+    if planar:
+        xr, xi, yr, yi = b(batch * n * h), b(batch * n * h), b(batch * n * h), b(batch * n * h)
+    else:
+        xr, xi, yr, yi = b(batch * n * e), None, b(batch * n * e), None
+    st = lib.af_execute(p, batch, xr, xi, yr, yi, None, None)
+    if st == 0:
+        st = lib.af_vjp_execute(p, batch, xr, xi, yr, yi, None, None)
+    b.free()
+    lib.af_plan_destroy(p)
+    return st
+
+
+LENGTHS = sorted(set(list(range(0, 130)) + [255, 256, 257, 360, 1000, 1024, 2048, 4096, 4099, 5000, 8192, 16384,
+                                            16385, 20000, 32768, 65521, 65536, 100003, 114689, 2 * 65521,
+                                            1 << 20, 1200 * 1200, 997637]))
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_c2c_lengths(dry, prec):
+    before = dry.dry_problem_count()
+    for n in LENGTHS:
+        for sign in (-1, 1):
+            for batch, planar in ((1, False), (3, True), (64, False)):
+                if n * batch > 1 << 22 and batch > 1:
+                    continue
+                st = run_c2c(dry, n, sign, prec, batch, planar)
+                assert st in (0, 2), (n, sign, batch, planar, st)
+    assert problems(dry, before) == []
+
+
+def test_large_power_of_two(dry):
+    before = dry.dry_problem_count()
+    for n, batch in ((65536, 64), (1 << 24, 1), (1024, 256)):
+        for prec in (0, 1):
+            for planar in (False, True):
+                assert run_c2c(dry, n, -1, prec, batch, planar) == 0
+    assert problems(dry, before) == []
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_real_and_convolution(dry, prec):
+    before = dry.dry_problem_count()
+    e = 16 if prec == 0 else 8
+    h = e // 2
+    for n in (2, 4, 6, 10, 64, 86, 1000, 8192, 16384, 32768, 40000, 2 * 65521):
+        for batch in (1, 5):
+            b = Bufs(dry)
+            for kind, fn in ((R2C_FULL, "to"), (C2R_FULL, "from")):
+                d = PlanDesc(n, -1 if kind == R2C_FULL else 1, kind, prec, 0, 0)
+                p = V()
+                st = dry.af_plan_create(ctypes.byref(d), ctypes.byref(p))
+                assert st in (0, 2), (n, kind, st)
+                if st:
+                    continue
+                if fn == "to":
+                    assert dry.af_to_spectrum(p, batch, b(batch * n * h), b(batch * n * e), None, None, None) == 0
+                else:
+                    assert dry.af_from_spectrum(p, batch, b(batch * n * e), None, b(batch * n * h), None, None) == 0
+                dry.af_plan_destroy(p)
+            d = PlanDesc(n, -1, REAL_CONV, prec, 0, 0)
+            p = V()
+            st = dry.af_plan_create(ctypes.byref(d), ctypes.byref(p))
+            assert st in (0, 2), (n, st)
+            if st == 0:
+                x, k, y = b(batch * n * h), b(batch * n * h), b(batch * n * h)
+                assert dry.af_real_convolve(p, batch, x, k, y, None) == 0
+                assert dry.af_vjp_real_convolve(p, batch, x, k, b(batch * n * h), b(batch * n * h),
+                                                b(batch * n * h), None) == 0
+                dry.af_plan_destroy(p)
+            b.free()
+    for n in (1, 7, 97, 1000, 4099, 65521, 100003):
+        b = Bufs(dry)
+        d = PlanDesc(n, -1, CONV_CYCLIC, prec, 0, 0)
+        p = V()
+        assert dry.af_plan_create(ctypes.byref(d), ctypes.byref(p)) == 0
+        assert dry.af_convolve_cyclic(p, 2, b(2 * n * e), None, b(2 * n * e), None, b(2 * n * e), None, None,
+                                      None) == 0
+        dry.af_plan_destroy(p)
+        b.free()
+    assert problems(dry, before) == []
+
+
+def _uneven_lengths():
+    spec = importlib.util.spec_from_file_location(
+        "synthetic", os.path.join(REPO, "accelerate-fourier_amd", "accelerate_fourier", "synthetic.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m.uneven_lengths(2)[0]
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_uneven_batch_4096_rows(dry, prec):
+    """The 4096-row uneven batch: every plan (with its plan-time FP64 transforms)
+    and the captured grouped launches."""
+    before = dry.dry_problem_count()
+    lens = _uneven_lengths()
+    arr = (ctypes.c_int64 * len(lens))(*[int(x) for x in lens])
+    h = V()
+    assert dry.af_batch_create(len(lens), arr, -1, prec, ctypes.byref(h)) == 0
+    e = 16 if prec == 0 else 8
+    b = Bufs(dry)
+    tot = int(np.sum(lens))
+    assert dry.af_batch_execute(h, b(tot * e), None, b(tot * e), None, None) == 0
+    dry.af_batch_destroy(h)
+    b.free()
+    assert problems(dry, before) == []
