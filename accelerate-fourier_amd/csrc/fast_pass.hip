@@ -357,6 +357,229 @@ static bool try_launch(const AfPass &d, hipStream_t s, hipError_t &err) {
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Fused real convolution and its adjoint for power-of-two half lengths
+// L = 256 ... 4096 (Real.hs:49-107 around the pointwise product of
+// Planned.hs:816-821): a workgroup takes B rows of 2L reals; both signals go
+// from HBM straight into first-stage registers as L complex pairs, their
+// transforms end in LDS, the untangle -> product -> entangle step runs in LDS
+// and the inverse transform of the entangled spectrum goes back to HBM as
+// real pairs.
+// ---------------------------------------------------------------------------
+
+// All stages of B lines (one radix-16 first-stage butterfly per thread, its
+// inputs in v); each line ends in lds in natural order.  Consecutive threads
+// take consecutive elements of a line in every stage.  Ends with a barrier.
+template <typename T, int L, int B, int SG>
+__device__ __forceinline__ void fast_fft_lds(const AfPass &d, typename Cx<T>::C (&v)[16], typename Cx<T>::C *lds,
+                                             const typename Cx<T>::W *stab) {
+    using PL = P2Plan<L>;
+    using FT = FastTables<L>;
+    constexpr int NT = B * L / 16;
+    constexpr int LD = fast_ld(L, B);
+    constexpr T s = T(SG);
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int st = 0; st < PL::NST; ++st) {
+        const int R = PL::radix(st);
+        const int p = PL::p(st);
+        const int NB = L / R;
+        const int PER = 16 / R;
+        if (st > 0) {
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                int b, i;
+                if (R == 16) bf_map<B, L / 16>(t + q * NT, false, b, i);
+                else if (R == 8) bf_map<B, L / 8>(t + q * NT, false, b, i);
+                else if (R == 4) bf_map<B, L / 4>(t + q * NT, false, b, i);
+                else bf_map<B, L / 2>(t + q * NT, false, b, i);
+#pragma unroll
+                for (int j = 0; j < R; ++j) v[q * R + j] = lds[b * LD + fpad(i + j * NB)];
+            }
+            lds_barrier();
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            int b, i;
+            if (R == 16) bf_map<B, L / 16>(t + q * NT, false, b, i);
+            else if (R == 8) bf_map<B, L / 8>(t + q * NT, false, b, i);
+            else if (R == 4) bf_map<B, L / 4>(t + q * NT, false, b, i);
+            else bf_map<B, L / 2>(t + q * NT, false, b, i);
+            const int k = i % p;
+            if (p > 1) {
+                const int tws = L / (p * R);
+                if (FT::STAGE_LDS) {
+#pragma unroll
+                    for (int j = 1; j < R; ++j) v[q * R + j] = twmul(v[q * R + j], stab[j * k * tws], s, false);
+                } else {
+                    double2 w1 = ((const double2 *)d.tw_d)[k * tws];
+                    if (SG > 0) w1.y = -w1.y;
+                    double2 wj = w1;
+#pragma unroll
+                    for (int j = 1; j < R; ++j) {
+                        v[q * R + j] = dmul(v[q * R + j], wj, false);
+                        if (j + 1 < R) wj = cmul(wj, w1);
+                    }
+                }
+            }
+            if (R == 16) dft<16>(&v[q * R], s);
+            else if (R == 8) dft<8>(&v[q * R], s);
+            else if (R == 4) dft<4>(&v[q * R], s);
+            else dft<2>(&v[q * R], s);
+            const int o = (i - k) * R + k;  // the last stage (p = L / R) writes o + j p = i + j L/R
+#pragma unroll
+            for (int j = 0; j < R; ++j) lds[b * LD + fpad(o + j * p)] = v[q * R + j];
+        }
+        lds_barrier();
+    }
+}
+
+// B rows of 2L reals from row0 on, as L complex pairs, into first-stage registers.
+template <typename T, int L, int B>
+__device__ __forceinline__ void fast_rows_load(const T *src, int64_t row0, typename Cx<T>::C (&v)[16]) {
+    using C = typename Cx<T>::C;
+    constexpr int NB = L / 16;
+    int b, i;
+    bf_map<B, NB>(threadIdx.x, false, b, i);
+    const Rsrc r = mk_rsrc(src + row0 * 2 * L, (uint32_t)(B * L * sizeof(C)));
+    const uint32_t o0 = (uint32_t)((b * L + i) * sizeof(C));
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = bld<C, kNT>(r, o0 + (uint32_t)(j * NB * sizeof(C)));
+}
+
+// Inverse transform of the B lines in buf (natural order), stored to HBM as
+// real pairs from row0 on.  buf is free again when it returns.
+template <typename T, int L, int B>
+__device__ __forceinline__ void fast_inverse_store(const AfPass &d, typename Cx<T>::C *buf,
+                                                   const typename Cx<T>::W *stab, T *dst, int64_t row0) {
+    using C = typename Cx<T>::C;
+    constexpr int NB = L / 16;
+    constexpr int LD = fast_ld(L, B);
+    int b, i;
+    bf_map<B, NB>(threadIdx.x, false, b, i);
+    C v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = buf[b * LD + fpad(i + j * NB)];
+    lds_barrier();
+    fast_fft_lds<T, L, B, 1>(d, v, buf, stab);
+    const Rsrc r = mk_rsrc(dst + row0 * 2 * L, (uint32_t)(B * L * sizeof(C)));
+    const uint32_t o0 = (uint32_t)((b * L + i) * sizeof(C));
+#pragma unroll
+    for (int j = 0; j < 16; ++j) bst<C>(r, o0 + (uint32_t)(j * NB * sizeof(C)), buf[b * LD + fpad(i + j * NB)]);
+    lds_barrier();
+}
+
+// Per line: untangle both half-length spectra to full spectra, multiply (by
+// the conjugate of the second for the adjoint's correlations) and entangle the
+// product back to the half-length spectrum of the inverse.  One thread takes a
+// pair (e, L - e), so dst may be A or Bb.
+template <typename T, int L, int B>
+__device__ __forceinline__ void fast_spec_product(const typename Cx<T>::C *A, const typename Cx<T>::C *Bb,
+                                                  typename Cx<T>::C *dst, const typename Cx<T>::W *aux, bool conj_b) {
+    using C = typename Cx<T>::C;
+    constexpr int NT = B * L / 16;
+    constexpr int LD = fast_ld(L, B);
+    constexpr int H = L / 2 + 1;
+    for (int idx = threadIdx.x; idx < B * H; idx += NT) {
+        const int b = idx / H, e = idx - b * H;
+        const int er = (L - e) & (L - 1);
+        const int ie = b * LD + fpad(e), ir = b * LD + fpad(er);
+        const C a0 = A[ie], a1 = A[ir], b0 = Bb[ie], b1 = Bb[ir];
+        C sa_lo, sa_hi, sb_lo, sb_hi;
+        untangle<T>(a0, a1, aux[e], false, sa_lo, sa_hi);
+        untangle<T>(b0, b1, aux[e], false, sb_lo, sb_hi);
+        if (conj_b) { sb_lo = cconj(sb_lo); sb_hi = cconj(sb_hi); }
+        const C z0 = entangle<T>(cmul(sa_lo, sb_lo), cmul(sa_hi, sb_hi), aux[e], false);
+        C z1 = z0;
+        if (er != e) {
+            C ta_lo, ta_hi, tb_lo, tb_hi;
+            untangle<T>(a1, a0, aux[er], false, ta_lo, ta_hi);
+            untangle<T>(b1, b0, aux[er], false, tb_lo, tb_hi);
+            if (conj_b) { tb_lo = cconj(tb_lo); tb_hi = cconj(tb_hi); }
+            z1 = entangle<T>(cmul(ta_lo, tb_lo), cmul(ta_hi, tb_hi), aux[er], false);
+        }
+        dst[ie] = z0;
+        if (er != e) dst[ir] = z1;
+    }
+}
+
+template <typename T, int L, int B>
+__host__ __device__ constexpr size_t fast_rconv_lds_bytes() {
+    return 2 * (size_t)fast_ld(L, B) * B * sizeof(typename Cx<T>::C) + (size_t)FastTables<L>::STAGE_ENTRIES * 16;
+}
+
+// Forward (VJP = false): out0 = fromSpectrum(toSpectrum in0 * toSpectrum in1), in0 = x, in1 = k.
+// Adjoint (VJP = true) with cotangent in0 = w: out0 = dk from in1 = x and out1 = dx from
+// in2 = k, each fromSpectrum(S_w conj S_other).
+template <typename T, int L, int B, bool VJP>
+__global__ void __launch_bounds__(B * L / 16)
+k_fast_rconv(const AfPass d, const T *__restrict__ in0, const T *__restrict__ in1, const T *__restrict__ in2,
+             T *__restrict__ out0, T *__restrict__ out1) {
+    using C = typename Cx<T>::C;
+    using W = typename Cx<T>::W;
+    using FT = FastTables<L>;
+    constexpr int NT = B * L / 16;
+    constexpr int LD = fast_ld(L, B);
+    extern __shared__ __align__(16) unsigned char smem_rc[];
+    C *A = (C *)smem_rc;
+    C *Bb = A + LD * B;
+    W *stab = (W *)(Bb + LD * B);
+    if (FT::STAGE_LDS)
+        for (int i = threadIdx.x; i < L; i += NT) stab[i] = ((const W *)d.tw)[i];
+    const W *aux = (const W *)d.aux_tw;
+    const int64_t row0 = (int64_t)blockIdx.x * B;
+    C v[16];
+    fast_rows_load<T, L, B>(in0, row0, v);
+    fast_fft_lds<T, L, B, -1>(d, v, A, stab);
+    fast_rows_load<T, L, B>(in1, row0, v);
+    fast_fft_lds<T, L, B, -1>(d, v, Bb, stab);
+    fast_spec_product<T, L, B>(A, Bb, Bb, aux, VJP);
+    lds_barrier();
+    fast_inverse_store<T, L, B>(d, Bb, stab, out0, row0);
+    if (VJP) {
+        fast_rows_load<T, L, B>(in2, row0, v);
+        fast_fft_lds<T, L, B, -1>(d, v, Bb, stab);
+        fast_spec_product<T, L, B>(A, Bb, Bb, aux, true);
+        lds_barrier();
+        fast_inverse_store<T, L, B>(d, Bb, stab, out1, row0);
+    }
+}
+
+// X(type, L, rows per workgroup)
+#define AF_RCONV_SHAPES(X)                                                                           \
+    X(float, 256, 16) X(float, 512, 8) X(float, 1024, 4) X(float, 2048, 2) X(float, 4096, 1)        \
+    X(double, 256, 8) X(double, 512, 4) X(double, 1024, 2) X(double, 2048, 1) X(double, 4096, 1)
+
+template <typename T, int L, int B>
+static bool try_rconv(const AfPass &d, const void *in0, const void *in1, const void *in2, void *out0, void *out1,
+                      bool vjp, hipStream_t s, hipError_t &err) {
+    if (d.L != L || d.nrows % B) return false;
+    const size_t lds = fast_rconv_lds_bytes<T, L, B>();
+    const dim3 g((unsigned)(d.nrows / B)), bl(B * L / 16);
+    if (vjp)
+        hipLaunchKernelGGL((k_fast_rconv<T, L, B, true>), g, bl, lds, s, d, (const T *)in0, (const T *)in1,
+                           (const T *)in2, (T *)out0, (T *)out1);
+    else
+        hipLaunchKernelGGL((k_fast_rconv<T, L, B, false>), g, bl, lds, s, d, (const T *)in0, (const T *)in1,
+                           (const T *)nullptr, (T *)out0, (T *)nullptr);
+    err = hipGetLastError();
+    return true;
+}
+
+hipError_t launch_fast_rconv(int prec, const AfPass &d, const void *in0, const void *in1, const void *in2, void *out0,
+                             void *out1, bool vjp, hipStream_t s, bool &done) {
+    hipError_t err = hipSuccess;
+    done = false;
+    if (d.flags & AFP_PLAIN_TW) return hipSuccess;
+    if (!d.tw || !d.tw_d || !d.aux_tw || d.nrows <= 0 || d.nrows >= ((int64_t)1 << 31)) return hipSuccess;
+#define AF_TRY_RC(T, L_, B_)                                                  \
+    if (!done && (prec == PREC_F64) == (sizeof(T) == 8))                      \
+        done = try_rconv<T, L_, B_>(d, in0, in1, in2, out0, out1, vjp, s, err);
+    AF_RCONV_SHAPES(AF_TRY_RC)
+#undef AF_TRY_RC
+    return err;
+}
+
 // Flags the fast kernel implements; anything else (plain FP32 twiddles
 // included) goes to the generic pass.
 static const int32_t kFastFlags = AFP_POSTTW | AFP_SCALE;
@@ -426,10 +649,21 @@ static void set_attr() {
     (void)hipFuncSetAttribute((const void *)k_fast<T, L, B, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
 
+template <typename T, int L, int B>
+static void set_attr_rc() {
+    (void)hipFuncSetAttribute((const void *)k_fast_rconv<T, L, B, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    (void)hipFuncSetAttribute((const void *)k_fast_rconv<T, L, B, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+}
+
 void init_fast_attributes() {
 #define AF_ATTR(T, L_, B_) set_attr<T, L_, B_>();
     AF_FAST_SHAPES(AF_ATTR)
 #undef AF_ATTR
+#define AF_ATTR_RC(T, L_, B_) set_attr_rc<T, L_, B_>();
+    AF_RCONV_SHAPES(AF_ATTR_RC)
+#undef AF_ATTR_RC
 }
 
 }  // namespace af

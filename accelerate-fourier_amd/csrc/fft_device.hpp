@@ -343,6 +343,29 @@ struct TileLines {
     int valid[64];
 };
 
+// Real-signal packing (Real.hs:49-107).
+// Untangled full spectrum at index e of the half-length spectrum Z:
+// S_e = ev - od, S_{e+L} = ev + od (toSpectrum, Real.hs:53-70).
+template <typename T>
+__device__ __forceinline__ void untangle(typename Cx<T>::C zk, typename Cx<T>::C zr, typename Cx<T>::W w,
+                                         bool plain, typename Cx<T>::C &lo, typename Cx<T>::C &hi) {
+    using C = typename Cx<T>::C;
+    const C xp = cscale(cadd(zk, zr), T(0.5)), xm = cscale(csub(zk, zr), T(0.5));
+    const C ev = mk(xp.x, xm.y);
+    const C t = twmul(mk(xm.x, xp.y), w, T(-1), plain);
+    const C od = mk(-t.y, t.x);
+    lo = csub(ev, od);
+    hi = cadd(ev, od);
+}
+
+// fromSpectrum's entangling (Real.hs:96-107): z_e = (p0 + p1) + i conj(W_{2L}^e) (p0 - p1)
+template <typename T>
+__device__ __forceinline__ typename Cx<T>::C entangle(typename Cx<T>::C p0, typename Cx<T>::C p1,
+                                                      typename Cx<T>::W w, bool plain) {
+    const auto t = twmul(csub(p0, p1), w, T(1), plain);
+    return cadd(cadd(p0, p1), mk(-t.y, t.x));
+}
+
 // ---------------------------------------------------------------------------
 // One tile of a pass: load -> stages -> [spectrum product -> inverse stages]
 // -> store.

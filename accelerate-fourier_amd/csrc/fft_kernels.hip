@@ -55,28 +55,6 @@ k_pass_grouped(const AfPass *__restrict__ descs, const int64_t *__restrict__ til
 // with the pointwise product of Planned.hs:816-821).
 // ---------------------------------------------------------------------------
 
-// Untangled full spectrum at index e of the half-length spectrum Z:
-// S_e = ev - od, S_{e+L} = ev + od (toSpectrum, Real.hs:53-70).
-template <typename T>
-__device__ __forceinline__ void untangle(typename Cx<T>::C zk, typename Cx<T>::C zr, typename Cx<T>::W w,
-                                         bool plain, typename Cx<T>::C &lo, typename Cx<T>::C &hi) {
-    using C = typename Cx<T>::C;
-    const C xp = cscale(cadd(zk, zr), T(0.5)), xm = cscale(csub(zk, zr), T(0.5));
-    const C ev = mk(xp.x, xm.y);
-    const C t = twmul(mk(xm.x, xp.y), w, T(-1), plain);
-    const C od = mk(-t.y, t.x);
-    lo = csub(ev, od);
-    hi = cadd(ev, od);
-}
-
-// fromSpectrum's entangling (Real.hs:96-107): z_e = (p0 + p1) + i conj(W_{2L}^e) (p0 - p1)
-template <typename T>
-__device__ __forceinline__ typename Cx<T>::C entangle(typename Cx<T>::C p0, typename Cx<T>::C p1,
-                                                      typename Cx<T>::W w, bool plain) {
-    const auto t = twmul(csub(p0, p1), w, T(1), plain);
-    return cadd(cadd(p0, p1), mk(-t.y, t.x));
-}
-
 template <typename T>
 __device__ __forceinline__ void load_real_row(typename Cx<T>::C *buf, const AfPass &d, const T *src) {
     const auto *s2 = (const typename Cx<T>::C *)src;
@@ -360,6 +338,9 @@ hipError_t launch_pass_grouped(int prec, const AfPass *descs, const int64_t *pre
 
 hipError_t launch_real_conv(int prec, const AfPass &d, const void *x, const void *k, void *y, hipStream_t s) {
     if (d.nrows <= 0) return hipSuccess;
+    bool done = false;
+    const hipError_t e = launch_fast_rconv(prec, d, x, k, nullptr, y, nullptr, false, s, done);
+    if (done || e != hipSuccess) return e;
     if (prec == PREC_F64)
         hipLaunchKernelGGL(k_real_conv<double>, dim3((unsigned)d.nrows), dim3(d.nthreads), d.lds_bytes, s, d,
                            (const double *)x, (const double *)k, (double *)y);
@@ -372,6 +353,9 @@ hipError_t launch_real_conv(int prec, const AfPass &d, const void *x, const void
 hipError_t launch_real_conv_vjp(int prec, const AfPass &d, const void *x, const void *k, const void *w, void *dx,
                                 void *dk, hipStream_t s) {
     if (d.nrows <= 0) return hipSuccess;
+    bool done = false;
+    const hipError_t e = launch_fast_rconv(prec, d, w, x, k, dk, dx, true, s, done);
+    if (done || e != hipSuccess) return e;
     if (prec == PREC_F64)
         hipLaunchKernelGGL(k_real_conv_vjp<double>, dim3((unsigned)d.nrows), dim3(d.nthreads), d.lds_bytes, s, d,
                            (const double *)x, (const double *)k, (const double *)w, (double *)dx, (double *)dk);

@@ -38,6 +38,11 @@ hipError_t launch_fast(int prec, const AfPass &d, hipStream_t s, bool &done);
 // the largest line count <= B with a power-of-two kernel for lines of length L
 // (B itself when L has none)
 int fast_tile_lines(int prec, int64_t L, int B);
+// fused real convolution (vjp = false: out0 = conv(in0, in1)) or its adjoint
+// (vjp = true: cotangent in0, out0 = dk from in1 = x, out1 = dx from in2 = k)
+// for power-of-two half lengths; done = false when the shape has no kernel
+hipError_t launch_fast_rconv(int prec, const AfPass &d, const void *in0, const void *in1, const void *in2, void *out0,
+                             void *out1, bool vjp, hipStream_t s, bool &done);
 
 // Karatsuba (karatsuba.hip)
 hipError_t launch_karatsuba(const int64_t *x, const int64_t *y, int64_t *z, const int64_t *dev_meta, int64_t npairs,

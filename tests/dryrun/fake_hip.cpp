@@ -301,6 +301,30 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
             }
         };
     }
+    if (name.find("af::k_fast_rconv<") == 0) {
+        const AfPass d = *(const AfPass *)args[0];
+        const void *in[3] = {*(const void *const *)args[1], *(const void *const *)args[2], *(const void *const *)args[3]};
+        void *out[2] = {*(void **)args[4], *(void **)args[5]};
+        int L = 0, B = 0;
+        const bool vjp = name.find("true>") != std::string::npos;
+        const size_t a = name.find('<');
+        const char *p = name.c_str() + a + 1;
+        while (*p && *p != ',') ++p;
+        if (std::sscanf(p, ", %d, %d", &L, &B) != 2) return [=] { problem(name + ": cannot parse the instantiation"); };
+        return [=] {
+            const int64_t row = 2 * (int64_t)L * tsz;
+            if (d.L != L) problem(name + ": descriptor L differs from the instantiation");
+            if ((int64_t)grid.x * B != d.nrows) problem(name + ": grid * B differs from the rows");
+            if ((int64_t)block.x != (int64_t)B * L / 16) problem(name + ": block differs from B*L/16");
+            const int64_t need = 2 * (int64_t)fast_ld_h(L, B) * B * 2 * tsz + (L <= 1024 ? L : 0) * 16;
+            if ((int64_t)shmem != need) problem(name + ": LDS size differs from the kernel's layout");
+            for (int i = 0; i < (vjp ? 3 : 2); ++i) in_alloc(in[i], 0, d.nrows * row, "input", name);
+            for (int i = 0; i < (vjp ? 2 : 1); ++i) in_alloc(out[i], 0, d.nrows * row, "output", name);
+            in_alloc(d.aux_tw, 0, (int64_t)L * 16, "aux_tw", name);
+            if (L <= 1024) in_alloc(d.tw, 0, (int64_t)L * 16, "tw", name);
+            else in_alloc(d.tw_d, 0, (int64_t)L * 16, "tw_d", name);
+        };
+    }
     if (name.find("af::k_gen_twiddles<") == 0) {
         void *out = *(void **)args[0];
         const int64_t count = *(const int64_t *)args[2];
