@@ -100,28 +100,48 @@ struct Core {
     void *tw1 = nullptr, *tw2 = nullptr;  // four-step: W_N1, W_N2
     void *ft_coarse = nullptr, *ft_fine = nullptr;  // four-step twiddles W_N, table format
     int ftShift = 0;
+    // three passes: W_N3 stage table and the second pass's twiddles W_{N1 N2}
+    void *tw3 = nullptr, *ft2_coarse = nullptr, *ft2_fine = nullptr;
+    int ft2Shift = 0;
     // FP64 copies (the same tables for FP64 plans)
     void *tw_d = nullptr, *tw1_d = nullptr, *tw2_d = nullptr, *ft_coarse_d = nullptr, *ft_fine_d = nullptr;
+    void *tw3_d = nullptr, *ft2_coarse_d = nullptr, *ft2_fine_d = nullptr;
 };
+
+// coarse x fine split of the twiddles W_N^a: 4^shift >= N
+int ft_shift_for(int64_t N) {
+    int shift = 0;
+    while (((int64_t)1 << (2 * shift)) < N) ++shift;
+    return shift;
+}
 
 af_status build_core(int prec, const CoreShape &sh, Core &c, Owner &own, hipStream_t s) {
     c.sh = sh;
     if (sh.kind == CORE_SINGLE) {
         AFCHK(own.alloc(&c.tw, (size_t)sh.N * tw_bytes(prec)));
         HIPCHK(gen_twiddles(prec, c.tw, sh.N, sh.N, 1, s));
-    } else if (sh.kind == CORE_FOUR) {
+    } else if (sh.kind == CORE_FOUR || sh.kind == CORE_THREE) {
         AFCHK(own.alloc(&c.tw1, (size_t)sh.N1 * tw_bytes(prec)));
         AFCHK(own.alloc(&c.tw2, (size_t)sh.N2 * tw_bytes(prec)));
         HIPCHK(gen_twiddles(prec, c.tw1, sh.N1, sh.N1, 1, s));
         HIPCHK(gen_twiddles(prec, c.tw2, sh.N2, sh.N2, 1, s));
-        int shift = 0;
-        while (((int64_t)1 << (2 * shift)) < sh.N) ++shift;
-        c.ftShift = shift;
-        const int64_t T = (int64_t)1 << shift, nc = (sh.N + T - 1) / T;
+        c.ftShift = ft_shift_for(sh.N);
+        const int64_t T = (int64_t)1 << c.ftShift, nc = (sh.N + T - 1) / T;
         AFCHK(own.alloc(&c.ft_coarse, (size_t)nc * 16));
         AFCHK(own.alloc(&c.ft_fine, (size_t)T * 16));
         HIPCHK(gen_twiddles(prec, c.ft_coarse, sh.N, nc, T, s));
         HIPCHK(gen_twiddles(prec, c.ft_fine, sh.N, T, 1, s));
+        if (sh.kind == CORE_THREE) {
+            AFCHK(own.alloc(&c.tw3, (size_t)sh.N3 * tw_bytes(prec)));
+            HIPCHK(gen_twiddles(prec, c.tw3, sh.N3, sh.N3, 1, s));
+            const int64_t N12 = sh.N1 * sh.N2;
+            c.ft2Shift = ft_shift_for(N12);
+            const int64_t T2 = (int64_t)1 << c.ft2Shift, nc2 = (N12 + T2 - 1) / T2;
+            AFCHK(own.alloc(&c.ft2_coarse, (size_t)nc2 * 16));
+            AFCHK(own.alloc(&c.ft2_fine, (size_t)T2 * 16));
+            HIPCHK(gen_twiddles(prec, c.ft2_coarse, N12, nc2, T2, s));
+            HIPCHK(gen_twiddles(prec, c.ft2_fine, N12, T2, 1, s));
+        }
     }
     if (prec == PREC_F64) {
         c.tw_d = c.tw;
@@ -129,6 +149,9 @@ af_status build_core(int prec, const CoreShape &sh, Core &c, Owner &own, hipStre
         c.tw2_d = c.tw2;
         c.ft_coarse_d = c.ft_coarse;
         c.ft_fine_d = c.ft_fine;
+        c.tw3_d = c.tw3;
+        c.ft2_coarse_d = c.ft2_coarse;
+        c.ft2_fine_d = c.ft2_fine;
         return AF_SUCCESS;
     }
     // FP32 plans keep FP64 copies for the power-of-two kernel's twiddle recurrences
@@ -139,12 +162,18 @@ af_status build_core(int prec, const CoreShape &sh, Core &c, Owner &own, hipStre
     };
     if (sh.kind == CORE_SINGLE) {
         AFCHK(dtab(&c.tw_d, sh.N, sh.N, 1));
-    } else if (sh.kind == CORE_FOUR) {
+    } else if (sh.kind == CORE_FOUR || sh.kind == CORE_THREE) {
         const int64_t T = (int64_t)1 << c.ftShift, nc = (sh.N + T - 1) / T;
         AFCHK(dtab(&c.tw1_d, sh.N1, sh.N1, 1));
         AFCHK(dtab(&c.tw2_d, sh.N2, sh.N2, 1));
         AFCHK(dtab(&c.ft_coarse_d, sh.N, nc, T));
         AFCHK(dtab(&c.ft_fine_d, sh.N, T, 1));
+        if (sh.kind == CORE_THREE) {
+            const int64_t N12 = sh.N1 * sh.N2, T2 = (int64_t)1 << c.ft2Shift, nc2 = (N12 + T2 - 1) / T2;
+            AFCHK(dtab(&c.tw3_d, sh.N3, sh.N3, 1));
+            AFCHK(dtab(&c.ft2_coarse_d, N12, nc2, T2));
+            AFCHK(dtab(&c.ft2_fine_d, N12, T2, 1));
+        }
     }
     return AF_SUCCESS;
 }
@@ -248,6 +277,63 @@ void set_ft(AfPass &d, const Core &c, int sign) {
     d.ft_fine_d = c.ft_fine_d;
 }
 
+// Three passes of N = Na Nb Nc (Na = N1, Nb = N2, Nc = N3), input index
+// n = a + Na b + Na Nb c, output index k = kc + Nc kb + Nc Nb ka, from
+// W_N^{nk} = W_Nc^{c kc} W_N^{(a + Na b) kc} W_Nb^{b kb} W_{Na Nb}^{a kb} W_Na^{a ka}:
+//  1. Nc-point transforms over c (stride Na Nb), lines l = a + Na b, then W_N^{l kc};
+//  2. Nb-point transforms over b (stride Na) in each block kc, lines a, then
+//     W_{Na Nb}^{a kb}, in place;
+//  3. Na-point transforms of the contiguous rows, stored transposed to
+//     kc + Nc kb + Nc Nb ka (one launch per transform of the batch).
+// Every pass has lines of one factor and tiles of neighbouring lines.
+void three_passes(int prec, const Core &c, int64_t nrows, int sign, bool plain, const Endpoint &in,
+                  const Endpoint &out, void *scratch, std::vector<AfPass> &ps) {
+    const int64_t N = c.sh.N, Na = c.sh.N1, Nb = c.sh.N2, Nc = c.sh.N3, Nab = Na * Nb;
+    const size_t eb = (size_t)elem_bytes(prec);
+    Endpoint sc;
+    sc.re = scratch;
+    sc.bs = N;
+    AfPass p1 = base_pass(prec, (int)Nc, c.sh.r3, c.tw3, c.tw3_d, choose_B_cols(prec, Nc, in.im != nullptr), Nab, nrows,
+                          sign, plain);
+    set_in(p1, in, Nab, 1);
+    set_out(p1, sc, Nab, 1);
+    p1.in_lines_fast = p1.out_lines_fast = 1;
+    set_ft(p1, c, sign);
+    ps.push_back(p1);
+    Endpoint sb;
+    sb.re = scratch;
+    sb.bs = Nab;
+    AfPass p2 = base_pass(prec, (int)Nb, c.sh.r2, c.tw2, c.tw2_d, choose_B_cols(prec, Nb), Na, nrows * Nc, sign, plain);
+    set_in(p2, sb, Na, 1);
+    set_out(p2, sb, Na, 1);
+    p2.in_lines_fast = p2.out_lines_fast = 1;
+    p2.flags |= AFP_POSTTW;
+    p2.ftN = Nab;
+    p2.ftShift = c.ft2Shift;
+    p2.ftSign = sign;
+    p2.ft_coarse = c.ft2_coarse;
+    p2.ft_fine = c.ft2_fine;
+    p2.ft_coarse_d = c.ft2_coarse_d;
+    p2.ft_fine_d = c.ft2_fine_d;
+    ps.push_back(p2);
+    for (int64_t r = 0; r < nrows; ++r) {
+        AfPass p3 = base_pass(prec, (int)Na, c.sh.r1, c.tw1, c.tw1_d, choose_B_cols(prec, Na, out.im != nullptr), Nc, Nb,
+                              sign, plain);
+        Endpoint si;
+        si.re = (const char *)scratch + (size_t)r * N * eb;
+        si.bs = Na;
+        set_in(p3, si, 1, Nab);
+        Endpoint so = out;
+        const size_t rb = out.im ? eb / 2 : eb;
+        so.re = (const char *)out.re + (size_t)(r * out.bs) * rb;
+        if (out.im) so.im = (const char *)out.im + (size_t)(r * out.bs) * rb;
+        so.bs = Nc;
+        set_out(p3, so, Nc * Nb, 1);
+        p3.out_lines_fast = 1;
+        ps.push_back(p3);
+    }
+}
+
 // Plain transform of `nrows` rows (natural order in and out).  For the
 // four-step core, `row_order_out` leaves the spectrum in the row order
 // (k1 * N2 + k2 holds X[k1 + N1 k2]) used by the convolution passes.
@@ -262,6 +348,10 @@ void c2c_passes(int prec, const Core &c, int64_t nrows, int sign, bool plain, co
         set_out(d, o2, 1, out.bs);
         d.in_bs = d.out_bs = 0;
         ps.push_back(d);
+        return;
+    }
+    if (c.sh.kind == CORE_THREE) {
+        three_passes(prec, c, nrows, sign, plain, in, out, scratch, ps);
         return;
     }
     const int64_t N = c.sh.N, N1 = c.sh.N1, N2 = c.sh.N2;
@@ -394,6 +484,7 @@ struct af_plan_s {
             return e * n;
         }
         if (core.sh.kind == CORE_FOUR) n = (size_t)core.sh.N + (shape.algo == ALG_RADER ? 2 : 0);
+        if (core.sh.kind == CORE_THREE) n = (size_t)core.sh.N;
         return e * n;
     }
 };
@@ -438,7 +529,8 @@ af_status shape_for(const af_plan_desc &d, PlanShape &sh, std::string &why) {
         return AF_UNSUPPORTED;
     }
     if (d.kind == AF_C2C || d.kind == AF_CONV_CYCLIC) {
-        if (!plan_shape(prec, n, d.algorithm, sh, why)) return AF_UNSUPPORTED;
+        // three passes only for plain transforms (the convolution passes take one or two)
+        if (!plan_shape(prec, n, d.algorithm, sh, why, d.kind == AF_C2C)) return AF_UNSUPPORTED;
         return AF_SUCCESS;
     }
     // real kinds: the half-length transform
@@ -1309,7 +1401,7 @@ af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign, i
         const int a = B->plans[lengths[i]]->shape.algo;
         const int k = B->plans[lengths[i]]->core.sh.kind;
         std::string key = a == ALG_RADER ? "rader" : a == ALG_BLUESTEIN ? "bluestein" : a == ALG_COPY ? "copy" : "stockham";
-        key += k == CORE_FOUR ? "/fourstep" : "/single";
+        key += k == CORE_FOUR ? "/fourstep" : k == CORE_THREE ? "/threepass" : "/single";
         counts[key]++;
     }
     B->text = "batch" + std::to_string(count) + " launches=" + std::to_string(B->groups.size());

@@ -1,6 +1,7 @@
 // Host planner of the MI355X transform path; see planner.hpp.
 #include "planner.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 
@@ -112,7 +113,47 @@ int64_t lds_max_len(int prec) { return prec == AF_F64 ? 8192 : 16384; }
 // 3328 bytes in FP64).
 int lds_tile_bytes_cap() { return 156 * 1024; }
 
-CoreShape core_shape(int prec, int64_t N) {
+static bool three_pass(int64_t N, int64_t lmax, CoreShape &c) {
+    int64_t a = 0, b = 0, cc = 0;
+    if ((N & (N - 1)) == 0) {
+        // 2^k = 2^ka 2^kb 2^kc, balanced, the spare bits on the contiguous rows
+        int k = 0;
+        while (((int64_t)1 << k) < N) ++k;
+        const int kb = k / 3, kc = k / 3, ka = k - kb - kc;
+        a = (int64_t)1 << ka;
+        b = (int64_t)1 << kb;
+        cc = (int64_t)1 << kc;
+        if (a > lmax) return false;
+    } else {
+        // the most balanced smooth factorisation within the LDS limit
+        std::vector<int64_t> dv;
+        for (int64_t d = 2; d <= lmax && d * d <= N * lmax; ++d)
+            if (N % d == 0) dv.push_back(d);
+        double best = 1e300;
+        for (int64_t x : dv)
+            for (int64_t y : dv) {
+                if ((N / x) % y) continue;
+                const int64_t z = N / x / y;
+                if (z < 2 || z > lmax) continue;
+                std::vector<int> t;
+                if (!stockham_radices(x, t) || !stockham_radices(y, t) || !stockham_radices(z, t)) continue;
+                const double lx = std::log((double)x), ly = std::log((double)y), lz = std::log((double)z);
+                const double cost = std::max(lx, std::max(ly, lz)) - std::min(lx, std::min(ly, lz));
+                if (cost < best - 1e-12) { best = cost; a = z; b = y; cc = x; }
+            }
+        if (!a) return false;
+    }
+    c.kind = CORE_THREE;
+    c.N1 = a;
+    c.N2 = b;
+    c.N3 = cc;
+    stockham_radices(a, c.r1);
+    stockham_radices(b, c.r2);
+    stockham_radices(cc, c.r3);
+    return true;
+}
+
+CoreShape core_shape(int prec, int64_t N, bool allow_three) {
     CoreShape c;
     c.N = N;
     std::vector<int> r;
@@ -123,6 +164,7 @@ CoreShape core_shape(int prec, int64_t N) {
         c.r = r;
         return c;
     }
+    if (allow_three && (N & (N - 1)) == 0 && N >= ((int64_t)1 << 24) && three_pass(N, lmax, c)) return c;
     // four-step: the most balanced N1 * N2 with both factors within the LDS limit
     int64_t best1 = 0;
     double bestcost = 1e300;
@@ -135,7 +177,10 @@ CoreShape core_shape(int prec, int64_t N) {
         const double cost = std::fabs(std::log((double)d) - std::log((double)e));
         if (cost < bestcost - 1e-12) { bestcost = cost; best1 = d; }
     }
-    if (!best1) return c;
+    if (!best1) {
+        if (allow_three) three_pass(N, lmax, c);
+        return c;
+    }
     c.kind = CORE_FOUR;
     c.N1 = best1;
     c.N2 = N / best1;
@@ -144,14 +189,14 @@ CoreShape core_shape(int prec, int64_t N) {
     return c;
 }
 
-bool plan_shape(int prec, int64_t n, int algorithm, PlanShape &out, std::string &why) {
+bool plan_shape(int prec, int64_t n, int algorithm, PlanShape &out, std::string &why, bool allow_three) {
     out = PlanShape();
     out.n = n;
     if (n == 0) { out.algo = ALG_EMPTY; return true; }
     if (n == 1) { out.algo = ALG_COPY; return true; }
     const bool chirp_only = algorithm == AF_ALG_CHIRP2 || algorithm == AF_ALG_CHIRP235;
     if (!chirp_only) {
-        CoreShape c = core_shape(prec, n);
+        CoreShape c = core_shape(prec, n, allow_three);
         if (c.kind != CORE_NONE) {
             out.algo = ALG_STOCKHAM;
             out.core = c;
@@ -196,6 +241,9 @@ static std::string core_str(const CoreShape &c) {
     if (c.kind == CORE_FOUR)
         return "fourstep" + std::to_string(c.N) + "(" + std::to_string(c.N1) + radix_str(c.r1) + "x" +
                std::to_string(c.N2) + radix_str(c.r2) + ")";
+    if (c.kind == CORE_THREE)
+        return "threepass" + std::to_string(c.N) + "(" + std::to_string(c.N1) + radix_str(c.r1) + "x" +
+               std::to_string(c.N2) + radix_str(c.r2) + "x" + std::to_string(c.N3) + radix_str(c.r3) + ")";
     return "none";
 }
 

@@ -7,7 +7,8 @@
 // computes the same DFT with the structure that suits gfx950: any length whose
 // prime factors are radices of the register butterflies (2, 3, 5, 7, 11, 13)
 // runs as a mixed-radix Stockham sequence in LDS (one pass up to the LDS
-// limit, two passes of the four-step factorisation N = N1 N2 beyond it);
+// limit, two passes of the four-step factorisation N = N1 N2 beyond it, three
+// passes N = N1 N2 N3 for the largest plain transforms);
 // primes p with smooth p-1 run Rader's convolution of length p-1; everything
 // else runs Bluestein with a smooth padded length.
 #pragma once
@@ -37,17 +38,22 @@ int64_t lds_max_len(int prec);
 int lds_tile_bytes_cap();
 int elem_bytes(int prec);
 
-enum CoreKind { CORE_NONE = 0, CORE_SINGLE = 1, CORE_FOUR = 2 };
+enum CoreKind { CORE_NONE = 0, CORE_SINGLE = 1, CORE_FOUR = 2, CORE_THREE = 3 };
 struct CoreShape {
     int kind = CORE_NONE;
     int64_t N = 0;
     std::vector<int> r;        // single pass radices
-    int64_t N1 = 0, N2 = 0;    // four-step factors
-    std::vector<int> r1, r2;
+    int64_t N1 = 0, N2 = 0;    // four-step factors; three-pass: N = N1 N2 N3 with
+    std::vector<int> r1, r2;   // input index a + N1 b + N1 N2 c (a fastest)
+    int64_t N3 = 0;
+    std::vector<int> r3;
 };
-// Shape of an FFT of length N by passes (single or four-step); CORE_NONE if N
-// is not smooth or does not factor within the LDS limit.
-CoreShape core_shape(int prec, int64_t N);
+// Shape of an FFT of length N by passes (single, four-step or, with
+// allow_three, three passes); CORE_NONE if N is not smooth or does not factor
+// within the LDS limit.  Three passes are used for powers of two from 2^24 on
+// (every pass then has short lines and wide tiles) and for smooth lengths that
+// two passes cannot hold; only plain transforms take them.
+CoreShape core_shape(int prec, int64_t N, bool allow_three = false);
 
 enum PlanAlgo {
     ALG_EMPTY = 0,     // n = 0: nothing to do
@@ -63,7 +69,7 @@ struct PlanShape {
     CoreShape core;    // the FFT that runs on the device
 };
 // algorithm: af_algorithm (auto, decompose, chirp2, chirp235)
-bool plan_shape(int prec, int64_t n, int algorithm, PlanShape &out, std::string &why);
+bool plan_shape(int prec, int64_t n, int algorithm, PlanShape &out, std::string &why, bool allow_three = false);
 std::string describe_shape(const PlanShape &s);
 
 }  // namespace af

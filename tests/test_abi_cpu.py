@@ -119,7 +119,12 @@ def test_valid_plan_without_device(lib):
 def test_host_planner_descriptions():
     assert planned.describe(1024, precision=_abi.F64) == "c2c.f64:stockham1024[16,16,4]"
     assert planned.describe(65536) == "c2c.f32:fourstep65536(256[16,16]x256[16,16])"
-    assert planned.describe(1 << 24).startswith("c2c.f32:fourstep16777216(4096")
+    assert planned.describe(1 << 23).startswith("c2c.f32:fourstep8388608(2048")
+    assert planned.describe(1 << 24) == "c2c.f32:threepass16777216(256[16,16]x256[16,16]x256[16,16])"
+    # three passes for smooth lengths that two LDS passes cannot hold
+    assert planned.describe(3 ** 18).startswith("c2c.f32:threepass387420489(729")
+    # ... but never for the convolution cores
+    assert planned.describe(1 << 24, kind=_abi.CONV_CYCLIC).startswith("convcyclic.f32:fourstep")
     assert planned.describe(97).startswith("c2c.f32:rader97(stockham96")
     assert planned.describe(65521).startswith("c2c.f32:rader65521(fourstep65520")
     assert planned.describe(100003).startswith("c2c.f32:bluestein100003(")

@@ -122,7 +122,7 @@ def test_c2c_lengths(dry, prec):
 
 def test_large_power_of_two(dry):
     before = dry.dry_problem_count()
-    for n, batch in ((65536, 64), (1 << 24, 1), (1024, 256)):
+    for n, batch in ((65536, 64), (1 << 24, 1), (1 << 24, 2), (1 << 25, 1), (1024, 256)):
         for prec in (0, 1):
             for planar in (False, True):
                 assert run_c2c(dry, n, -1, prec, batch, planar) == 0

@@ -215,3 +215,28 @@ def test_real_convolution_packed(n):
     dx, dk = real.convolve_vjp(xd, kd, wd)
     assert rel_l2(host(dx), oracle.from_spectrum(Sw * np.conj(Sk))) <= tol_f64(n // 2)
     assert rel_l2(host(dk), oracle.from_spectrum(Sw * np.conj(Sx))) <= tol_f64(n // 2)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("layout", ["planar", "interleaved"])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_three_pass_2p24(prec, layout, sign):
+    """n = 2^24 runs as three passes of 256-point transforms (the last one per
+    transform of the batch); two rows against numpy's FP64 FFT, which the
+    oracle's restatement matches on the smaller lengths above."""
+    n, rows = 1 << 24, 2
+    assert planned.describe(n, sign, _abi.C2C, _abi.F64 if prec == "f64" else _abi.F32).split(":")[1].startswith(
+        "threepass16777216")
+    x = synthetic.complex_normal(11, (rows, n))
+    ref = np.fft.fft(x, axis=1) if sign < 0 else np.fft.ifft(x, axis=1) * n
+    ct = torch.complex128 if prec == "f64" else torch.complex64
+    rt = torch.float64 if prec == "f64" else torch.float32
+    t = planned.transform(Sign(sign), n)
+    if layout == "planar":
+        y = t((to_dev(x.real, rt), to_dev(x.imag, rt)))
+        got = host(y[0]) + 1j * host(y[1])
+    else:
+        got = host(t(to_dev(x, ct)))
+    tol = 1e-13 if prec == "f64" else 1e-6
+    for r in range(rows):
+        assert rel_l2(got[r], ref[r]) <= tol, (r, rel_l2(got[r], ref[r]))
