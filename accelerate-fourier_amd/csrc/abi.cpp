@@ -190,7 +190,7 @@ int lds_bytes_for(int prec, int L, int B, int ps) {
 }
 
 bool tile_fits(int prec, int L, int B) {
-    if ((int64_t)B * L > 16384) return false;  // 16 elements per thread, 1024 threads
+    if ((int64_t)B * L > 1024 * pass_ept(prec == PREC_F64)) return false;  // 1024 threads
     return lds_bytes_for(prec, L, B, 4) <= lds_tile_bytes_cap();
 }
 
@@ -236,7 +236,8 @@ AfPass base_pass(int prec, int L, const std::vector<int> &radices, const void *t
     d.nlines = nlines;
     d.nrows = nrows;
     d.ntiles = (nrows * nlines + B - 1) / B;
-    int nt = (int)((((int64_t)B * L + 15) / 16 + 63) / 64 * 64);
+    const int ept = pass_ept(prec == PREC_F64);
+    int nt = (int)((((int64_t)B * L + ept - 1) / ept + 63) / 64 * 64);
     if (nt < 64) nt = 64;
     if (nt > 1024) nt = 1024;
     d.nthreads = nt;

@@ -12,6 +12,11 @@
 
 #define AF_MAXST 16
 
+// Generic pass kernels: elements of a tile each thread holds in registers per
+// stage (16 complex floats or 8 complex doubles), so a tile of B lines of
+// length L takes ceil(B L / EPT) threads, at most 1024.
+__host__ __device__ constexpr int pass_ept(bool f64) { return f64 ? 8 : 16; }
+
 enum AfPassFlags : int32_t {
     AFP_CONV = 1 << 0,          // stages(sign) -> * spec -> stages(-sign)
     AFP_POSTTW = 1 << 1,        // multiply element e of line l by W_N^{ftSign * (l*e mod N)}

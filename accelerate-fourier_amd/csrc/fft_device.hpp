@@ -19,6 +19,28 @@ template <> struct Cx<double> { using C = double2; using W = double2; };
 
 __device__ __forceinline__ float2 mk(float a, float b) { return make_float2(a, b); }
 __device__ __forceinline__ double2 mk(double a, double b) { return make_double2(a, b); }
+// Loads and stores of element i through descriptor pointers as global (not
+// flat) memory instructions.  HIP's vector types move as clang vectors of the
+// same shape, which copy across address spaces.
+template <typename T> using gmem = __attribute__((address_space(1))) T;
+template <typename V> struct GVec { typedef V type; };
+template <> struct GVec<float2> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct GVec<float4> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct GVec<double2> { typedef double type __attribute__((ext_vector_type(2))); };
+template <typename V> __device__ __forceinline__ V gld(const void *p, int64_t i) {
+    typedef typename GVec<V>::type E;
+    const E e = ((const gmem<E> *)p)[i];
+    V v;
+    __builtin_memcpy(&v, &e, sizeof v);
+    return v;
+}
+template <typename V> __device__ __forceinline__ void gst(void *p, int64_t i, V v) {
+    typedef typename GVec<V>::type E;
+    E e;
+    __builtin_memcpy(&e, &v, sizeof e);
+    ((gmem<E> *)p)[i] = e;
+}
+
 template <class C> __device__ __forceinline__ C cadd(C a, C b) { return mk(a.x + b.x, a.y + b.y); }
 template <class C> __device__ __forceinline__ C csub(C a, C b) { return mk(a.x - b.x, a.y - b.y); }
 template <class C> __device__ __forceinline__ C cmul(C a, C b) {
@@ -215,10 +237,10 @@ __device__ __forceinline__ int pidx(int i, int ps) { return ps ? i + (i >> ps) :
 // (i - k) R + k + j p with k = i mod p, p = product of the previous radices.
 template <typename T, int R>
 __device__ __forceinline__ void stage(typename Cx<T>::C *lds, const int L, const int ld, const int B,
-                                      const int p, const typename Cx<T>::W *__restrict__ tw, const T s,
+                                      const int p, const void *__restrict__ tw, const T s,
                                       const int ps, const bool plain) {
     using C = typename Cx<T>::C;
-    constexpr int MB = (16 + R - 1) / R;
+    constexpr int MB = (pass_ept(sizeof(T) == 8) + R - 1) / R;
     const int nbl = L / R;
     const int nbf = B * nbl;
     const int tws = L / (p * R);
@@ -248,7 +270,8 @@ __device__ __forceinline__ void stage(typename Cx<T>::C *lds, const int L, const
             const int k = i % p;
             if (p > 1) {
 #pragma unroll
-                for (int j = 1; j < R; ++j) v[b][j] = twmul(v[b][j], tw[j * k * tws], s, plain);
+                for (int j = 1; j < R; ++j)
+                    v[b][j] = twmul(v[b][j], gld<typename Cx<T>::W>(tw, j * k * tws), s, plain);
             }
             dft<R>(v[b], s);
             const int o = base[b] + (i - k) * R + k;
@@ -262,7 +285,7 @@ __device__ __forceinline__ void stage(typename Cx<T>::C *lds, const int L, const
 template <typename T>
 __device__ __forceinline__ void run_stages(typename Cx<T>::C *lds, const AfPass &d, const T s) {
     using W = typename Cx<T>::W;
-    const W *tw = (const W *)d.tw;
+    const void *tw = d.tw;
     const bool plain = (d.flags & AFP_PLAIN_TW) != 0;
     for (int st = 0; st < d.nst; ++st) {
         const int R = d.radix[st], p = d.pst[st];
@@ -286,16 +309,16 @@ __device__ __forceinline__ void run_stages(typename Cx<T>::C *lds, const AfPass 
 
 template <typename T>
 __device__ __forceinline__ typename Cx<T>::C ld_c(const void *re, const void *im, int64_t a) {
-    if (im) return mk(((const T *)re)[a], ((const T *)im)[a]);
-    return ((const typename Cx<T>::C *)re)[a];
+    if (im) return mk(gld<T>(re, a), gld<T>(im, a));
+    return gld<typename Cx<T>::C>(re, a);
 }
 template <typename T>
 __device__ __forceinline__ void st_c(void *re, void *im, int64_t a, typename Cx<T>::C v) {
     if (im) {
-        ((T *)re)[a] = v.x;
-        ((T *)im)[a] = v.y;
+        gst<T>(re, a, v.x);
+        gst<T>(im, a, v.y);
     } else {
-        ((typename Cx<T>::C *)re)[a] = v;
+        gst<typename Cx<T>::C>(re, a, v);
     }
 }
 
@@ -304,23 +327,23 @@ __device__ __forceinline__ void st_c(void *re, void *im, int64_t a, typename Cx<
 // hi parts' product plus the first-order lo corrections, accurate to about one
 // FP32 ulp.
 __device__ __forceinline__ double2 ft_twiddle_t(const AfPass &d, uint64_t a, double) {
-    const double2 c = ((const double2 *)d.ft_coarse)[a >> d.ftShift];
-    const double2 f = ((const double2 *)d.ft_fine)[a & ((uint64_t(1) << d.ftShift) - 1)];
+    const double2 c = gld<double2>(d.ft_coarse, a >> d.ftShift);
+    const double2 f = gld<double2>(d.ft_fine, a & ((uint64_t(1) << d.ftShift) - 1));
     double2 w = cmul(c, f);
     if (d.ftSign > 0) w.y = -w.y;
     return w;
 }
 // FP64 four-step twiddle from the FP64 tables (both precisions).
 __device__ __forceinline__ double2 ft_twiddle_d(const AfPass &d, uint64_t a) {
-    const double2 c = ((const double2 *)d.ft_coarse_d)[a >> d.ftShift];
-    const double2 f = ((const double2 *)d.ft_fine_d)[a & ((uint64_t(1) << d.ftShift) - 1)];
+    const double2 c = gld<double2>(d.ft_coarse_d, a >> d.ftShift);
+    const double2 f = gld<double2>(d.ft_fine_d, a & ((uint64_t(1) << d.ftShift) - 1));
     double2 w = cmul(c, f);
     if (d.ftSign > 0) w.y = -w.y;
     return w;
 }
 __device__ __forceinline__ float2 ft_twiddle_t(const AfPass &d, uint64_t a, float) {
-    const float4 c = ((const float4 *)d.ft_coarse)[a >> d.ftShift];
-    const float4 f = ((const float4 *)d.ft_fine)[a & ((uint64_t(1) << d.ftShift) - 1)];
+    const float4 c = gld<float4>(d.ft_coarse, a >> d.ftShift);
+    const float4 f = gld<float4>(d.ft_fine, a & ((uint64_t(1) << d.ftShift) - 1));
     float re = fmaf(c.x, f.z, fmaf(-c.y, f.w, fmaf(c.z, f.x, -c.w * f.y)));
     float im = fmaf(c.x, f.w, fmaf(c.y, f.z, fmaf(c.z, f.y, c.w * f.x)));
     re = fmaf(c.x, f.x, fmaf(-c.y, f.y, re));
@@ -371,7 +394,7 @@ __device__ __forceinline__ typename Cx<T>::C entangle(typename Cx<T>::C p0, type
 // -> store.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C *lds, TileLines &tl,
+__device__ __forceinline__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C *lds, TileLines &tl,
                           typename Cx<T>::C *s_x0, typename Cx<T>::C *s_dc) {
     using C = typename Cx<T>::C;
     using W = typename Cx<T>::W;
@@ -388,7 +411,7 @@ __device__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C
         tl.line[tid] = v ? gl - (gl / d.nlines) * d.nlines : 0;
         if (v && (d.flags & AFP_RADER)) s_x0[tid] = ld_c<T>(d.in_re, d.in_im, tl.row[tid] * d.in_bs);
         if (v && (d.flags & AFP_X0_CAPTURE) && tl.line[tid] == 0)
-            ((C *)d.x0_out)[tl.row[tid]] = ld_c<T>(d.in_re, d.in_im, tl.row[tid] * d.in_bs);
+            gst<C>(d.x0_out, tl.row[tid], ld_c<T>(d.in_re, d.in_im, tl.row[tid] * d.in_bs));
     }
     __syncthreads();
 
@@ -406,7 +429,7 @@ __device__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C
                 // fromSpectrum (Real.hs:96-107): z = (S_e + S_{e+L}) + i W_{2L}^{+e} (S_e - S_{e+L})
                 const C p0 = ld_c<T>(d.in_re, d.in_im, rb + e);
                 const C p1 = ld_c<T>(d.in_re, d.in_im, rb + e + L);
-                const C t = twmul(csub(p0, p1), ((const W *)d.aux_tw)[e], T(1), plain);
+                const C t = twmul(csub(p0, p1), gld<W>(d.aux_tw, e), T(1), plain);
                 v = cadd(cadd(p0, p1), mk(-t.y, t.x));
             } else if (d.flags & AFP_UNTANGLE_ADJ) {
                 // Adjoint of toSpectrum: x -> F(x) maps reals to the full spectrum, so
@@ -428,14 +451,14 @@ __device__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C
                     q0 = cscale(cadd(g0, cconj(h0)), T(0.5));
                     q1 = cscale(cadd(g1, cconj(h1)), T(0.5));
                 }
-                const C t = twmul(csub(q0, q1), ((const W *)d.aux_tw)[e], T(1), plain);
+                const C t = twmul(csub(q0, q1), gld<W>(d.aux_tw, e), T(1), plain);
                 v = cadd(cadd(q0, q1), mk(-t.y, t.x));
             } else {
                 const int64_t pos = l * d.in_ls + (int64_t)e * d.in_es;
                 if (pos < d.in_len) {
-                    const int64_t a = d.in_map ? (int64_t)d.in_map[pos] : pos;
+                    const int64_t a = d.in_map ? (int64_t)gld<int32_t>(d.in_map, pos) : pos;
                     v = ld_c<T>(d.in_re, d.in_im, rb + a);
-                    if (d.in_mul) v = vmul(v, ((const W *)d.in_mul)[pos], plain);
+                    if (d.in_mul) v = vmul(v, gld<W>(d.in_mul, pos), plain);
                 }
             }
         }
@@ -457,11 +480,11 @@ __device__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C
             C v = lds[li];
             if (l == 0 && e == 0) {
                 if (d.flags & AFP_RADER) s_dc[b] = v;
-                if (d.flags & AFP_DC_CAPTURE) ((C *)d.dc_out)[r] = v;
+                if (d.flags & AFP_DC_CAPTURE) gst<C>(d.dc_out, r, v);
             }
             const int64_t si = r * d.spec_bs + l * d.spec_ls + e;
-            if (d.flags & AFP_SPEC_PLAIN) lds[li] = cmul(v, ((const C *)d.spec)[si]);
-            else lds[li] = vmul(v, ((const W *)d.spec)[si], plain);
+            if (d.flags & AFP_SPEC_PLAIN) lds[li] = cmul(v, gld<C>(d.spec, si));
+            else lds[li] = vmul(v, gld<W>(d.spec, si), plain);
         }
         __syncthreads();
         run_stages<T>(lds, d, -s1);
@@ -479,7 +502,7 @@ __device__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C
             const C xp = cscale(cadd(zk, zr), T(0.5)), xm = cscale(csub(zk, zr), T(0.5));
             const C ev = mk(xp.x, xm.y);
             // odds = i W_{2L}^e (xm.re + i xp.im) with the forward-sign W_{2L}^e
-            const C t = twmul(mk(xm.x, xp.y), ((const W *)d.aux_tw)[e], T(-1), plain);
+            const C t = twmul(mk(xm.x, xp.y), gld<W>(d.aux_tw, e), T(-1), plain);
             const C od = mk(-t.y, t.x);
             C lo = csub(ev, od), hi = cadd(ev, od);
             if (d.flags & AFP_SCALE) { lo = cscale(lo, T(d.scale)); hi = cscale(hi, T(d.scale)); }
@@ -500,7 +523,7 @@ __device__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C
             const int64_t ob = tl.row[b] * d.out_bs;
             const C u = lds[pidx(b * ld + e, ps)];
             // c = conj(i W_{2L}^{+e}) * u = -i W_{2L}^{-e} u
-            const C t = twmul(u, ((const W *)d.aux_tw)[e], T(-1), plain);
+            const C t = twmul(u, gld<W>(d.aux_tw, e), T(-1), plain);
             const C c = mk(t.y, -t.x);
             st_c<T>(d.out_re, d.out_im, ob + e, cadd(u, c));
             st_c<T>(d.out_re, d.out_im, ob + e + L, csub(u, c));
@@ -524,7 +547,7 @@ __device__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C
             const uint64_t a = ((N & (N - 1)) == 0) ? (le & (N - 1)) : (le % N);
             v = cmul(v, ft_twiddle_t(d, a, T(0)));
         }
-        if (d.out_mul) v = vmul(v, ((const W *)d.out_mul)[o], plain);
+        if (d.out_mul) v = vmul(v, gld<W>(d.out_mul, o), plain);
         if (d.flags & AFP_SCALE) v = cscale(v, T(d.scale));
         const int64_t ob = r * d.out_bs;
         if (rader_any) {
@@ -533,7 +556,7 @@ __device__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C
             else x0 = ld_c<T>(d.x0_re, d.x0_im, r * d.x0_bs);
             v = cadd(v, x0);
         }
-        const int64_t a = d.out_map ? (int64_t)d.out_map[o] : o;
+        const int64_t a = d.out_map ? (int64_t)gld<int32_t>(d.out_map, o) : o;
         st_c<T>(d.out_re, d.out_im, ob + a, v);
     }
     if (rader_any && tid < B && tl.valid[tid] && tl.line[tid] == 0) {
@@ -541,7 +564,7 @@ __device__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C
         const int64_t r = tl.row[tid];
         C x0, dc;
         if (d.flags & AFP_RADER) { x0 = s_x0[tid]; dc = s_dc[tid]; }
-        else { x0 = ld_c<T>(d.x0_re, d.x0_im, r * d.x0_bs); dc = ((const C *)d.dc_in)[r]; }
+        else { x0 = ld_c<T>(d.x0_re, d.x0_im, r * d.x0_bs); dc = gld<C>(d.dc_in, r); }
         st_c<T>(d.out_re, d.out_im, r * d.out_bs, cadd(x0, dc));
     }
     __syncthreads();

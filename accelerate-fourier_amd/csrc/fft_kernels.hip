@@ -7,6 +7,8 @@
 //  k_gen_*         FP64 generation of twiddles, chirps and Rader kernels on the GPU
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fft_device.hpp"
 #include "kernels.hpp"
 
@@ -16,29 +18,33 @@ namespace af {
 // Pass kernels
 // ---------------------------------------------------------------------------
 
+// All tiles of one workgroup: one call, so the function's register saves are
+// paid once per workgroup rather than once per tile.
+template <typename T>
+__device__ __noinline__ void pass_tiles(const AfPass &d, typename Cx<T>::C *lds, TileLines &tl,
+                                        typename Cx<T>::C *s_x0, typename Cx<T>::C *s_dc) {
+    const bool one = (int64_t)gridDim.x == d.ntiles;
+    for (int64_t t = blockIdx.x; t < d.ntiles; t += gridDim.x)
+        pass_tile<T>(d, one ? xcd_tile(t, d.ntiles) : t, lds, tl, s_x0, s_dc);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(1024) k_pass(const AfPass d) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ TileLines tl;
     __shared__ typename Cx<T>::C s_x0[64], s_dc[64];
-    auto *lds = (typename Cx<T>::C *)smem;
-    if ((int64_t)gridDim.x == d.ntiles) {
-        pass_tile<T>(d, xcd_tile(blockIdx.x, gridDim.x), lds, tl, s_x0, s_dc);
-        return;
-    }
-    for (int64_t tile = blockIdx.x; tile < d.ntiles; tile += gridDim.x)
-        pass_tile<T>(d, tile, lds, tl, s_x0, s_dc);
+    pass_tiles<T>(d, (typename Cx<T>::C *)smem, tl, s_x0, s_dc);
 }
 
 // tile_prefix[i] = first global tile of descriptor i; tile_prefix[ndesc] = total.
 template <typename T>
-__global__ void __launch_bounds__(1024)
-k_pass_grouped(const AfPass *__restrict__ descs, const int64_t *__restrict__ tile_prefix, const int ndesc) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ TileLines tl;
-    __shared__ typename Cx<T>::C s_x0[64], s_dc[64];
-    auto *lds = (typename Cx<T>::C *)smem;
+__device__ __noinline__ void pass_tiles_grouped(const AfPass *__restrict__ descs, const int64_t *__restrict__ tile_prefix,
+                                                const int ndesc, typename Cx<T>::C *lds, TileLines &tl,
+                                                typename Cx<T>::C *s_x0, typename Cx<T>::C *s_dc) {
+    // the tile's descriptor is copied into LDS, where the tile reads it
+    __shared__ AfPass sd;
     const int64_t total = tile_prefix[ndesc];
+    int cur = -1;
     for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
         int lo = 0, hi = ndesc - 1;
         while (lo < hi) {  // last descriptor whose first tile is <= g
@@ -46,8 +52,25 @@ k_pass_grouped(const AfPass *__restrict__ descs, const int64_t *__restrict__ til
             if (tile_prefix[mid] <= g) lo = mid;
             else hi = mid - 1;
         }
-        pass_tile<T>(descs[lo], g - tile_prefix[lo], lds, tl, s_x0, s_dc);
+        if (lo != cur) {
+            __syncthreads();  // the previous tile is done with sd
+            const uint32_t *src = (const uint32_t *)&descs[lo];
+            uint32_t *dst = (uint32_t *)&sd;
+            for (int i = threadIdx.x; i < (int)(sizeof(AfPass) / 4); i += blockDim.x) dst[i] = src[i];
+            __syncthreads();
+            cur = lo;
+        }
+        pass_tile<T>(sd, g - tile_prefix[lo], lds, tl, s_x0, s_dc);
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_pass_grouped(const AfPass *__restrict__ descs, const int64_t *__restrict__ tile_prefix, const int ndesc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ TileLines tl;
+    __shared__ typename Cx<T>::C s_x0[64], s_dc[64];
+    pass_tiles_grouped<T>(descs, tile_prefix, ndesc, (typename Cx<T>::C *)smem, tl, s_x0, s_dc);
 }
 
 // ---------------------------------------------------------------------------
@@ -311,7 +334,11 @@ template <typename T>
 static hipError_t launch_pass_t(const AfPass &d, hipStream_t s) {
     if (d.ntiles <= 0) return hipSuccess;
     if (!pass_lds_fits(sizeof(T) == 8 ? PREC_F64 : PREC_F32, d.lds_bytes) || d.B > 64) return hipErrorInvalidValue;
-    const int64_t g = d.ntiles < (int64_t)1 << 30 ? d.ntiles : (int64_t)1 << 30;
+    // a workgroup loops over tiles once there are more than four rounds of the
+    // 256 CUs at the tile's LDS occupancy
+    const int per_cu = std::max(1, (int)((160 * 1024) / std::max(d.lds_bytes + 4096, 1)));
+    const int64_t cap = (int64_t)256 * per_cu * 4;
+    const int64_t g = d.ntiles < cap ? d.ntiles : cap;
     hipLaunchKernelGGL(k_pass<T>, dim3((unsigned)g), dim3(d.nthreads), d.lds_bytes, s, d);
     return hipGetLastError();
 }
@@ -328,7 +355,9 @@ hipError_t launch_pass_grouped(int prec, const AfPass *descs, const int64_t *pre
                                int64_t total_tiles, int nthreads, int lds_bytes, hipStream_t s) {
     if (total_tiles <= 0) return hipSuccess;
     if (!pass_lds_fits(prec, lds_bytes)) return hipErrorInvalidValue;
-    const int64_t g = total_tiles < (int64_t)1 << 30 ? total_tiles : (int64_t)1 << 30;
+    const int per_cu = std::max(1, (int)((160 * 1024) / std::max(lds_bytes + 4096, 1)));
+    const int64_t cap = (int64_t)256 * per_cu * 4;
+    const int64_t g = total_tiles < cap ? total_tiles : cap;
     if (prec == PREC_F64)
         hipLaunchKernelGGL(k_pass_grouped<double>, dim3((unsigned)g), dim3(nthreads), lds_bytes, s, descs, prefix, ndesc);
     else

@@ -225,7 +225,8 @@ void check_generic_launch(const AfPass &d, int tsz, unsigned block, size_t shmem
     const std::string k = k0 + " [L=" + std::to_string(d.L) + " B=" + std::to_string(d.B) + " lds=" +
                           std::to_string(shmem) + " threads=" + std::to_string(block) + "]";
     if (block > 1024 || block < 1) problem(k + ": block size out of range");
-    if ((int64_t)block * 16 < (int64_t)d.B * d.L) problem(k + ": fewer than B*L/16 threads (butterflies dropped)");
+    if ((int64_t)block * pass_ept(tsz == 8) < (int64_t)d.B * d.L)
+        problem(k + ": fewer than B*L/EPT threads (butterflies dropped)");
     if (d.B > 64) problem(k + ": B above the 64 lines of TileLines");
     if (d.ld < d.L + (d.B > 1 ? 1 : 0)) problem(k + ": ld too small");
     const int64_t need = ((int64_t)pidx_h(d.B * d.ld - 1, d.padshift) + 1) * csz;
