@@ -1245,6 +1245,7 @@ struct af_batch_s {
     void *scratch = nullptr;
     struct Group {
         int phase = 0, nthreads = 0, lds = 0;
+        bool full = false;  // some descriptor sets a flag of the full pass kernel
         std::vector<int64_t> rows;  // transform indices
         int64_t desc_off = 0, prefix_off = 0, total_tiles = 0;
     };
@@ -1357,6 +1358,7 @@ af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign, i
                 if (cls != c) continue;
                 g.rows.push_back(i);
                 g.lds = std::max(g.lds, d.lds_bytes);
+                if (d.flags & kRarePassFlags) g.full = true;
             }
             if (g.rows.empty()) continue;
             g.desc_off = ndesc;
@@ -1384,7 +1386,7 @@ af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign, i
     for (const auto &g : B->groups) {
         if (st != AF_SUCCESS) break;
         st = HIPST(launch_pass_grouped(B->prec, B->d_descs + g.desc_off, B->d_prefix + g.prefix_off,
-                                       (int)g.rows.size(), g.total_tiles, g.nthreads, g.lds, s));
+                                       (int)g.rows.size(), g.total_tiles, g.nthreads, g.lds, g.full, s));
     }
     hipGraph_t graph = nullptr;
     { const af_status h__ = HIPST(hipStreamEndCapture(s, &graph)); if (st == AF_SUCCESS) st = h__; }

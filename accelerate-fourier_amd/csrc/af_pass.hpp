@@ -34,6 +34,12 @@ enum AfPassFlags : int32_t {
     AFP_X0_CAPTURE = 1 << 13,   // copy row element 0 of the input to x0_out (Rader, before any store)
 };
 
+// Flags only the real-signal packings and Rader passes use.  The generic pass
+// kernel has a lean instantiation without their code (small enough to inline
+// without spills) and a full one for passes that set any of them.
+constexpr int32_t kRarePassFlags = AFP_RADER | AFP_X0_CAPTURE | AFP_RADER_OUT | AFP_REAL_UNTANGLE | AFP_REAL_ENTANGLE |
+                                   AFP_UNTANGLE_ADJ | AFP_ENTANGLE_ADJ;
+
 struct AfPass {
     // geometry
     int32_t L;          // line (transform) length

@@ -23,6 +23,7 @@ __device__ __forceinline__ double2 mk(double a, double b) { return make_double2(
 // flat) memory instructions.  HIP's vector types move as clang vectors of the
 // same shape, which copy across address spaces.
 template <typename T> using gmem = __attribute__((address_space(1))) T;
+template <typename T> using kmem = __attribute__((address_space(4))) T;  // kernel arguments
 template <typename V> struct GVec { typedef V type; };
 template <> struct GVec<float2> { typedef float type __attribute__((ext_vector_type(2))); };
 template <> struct GVec<float4> { typedef float type __attribute__((ext_vector_type(4))); };
@@ -230,7 +231,23 @@ __device__ __forceinline__ void dft(C *v, T s) {
 
 // LDS index padding: one spare slot every 2^ps elements spreads strided
 // accesses over the banks.
-__device__ __forceinline__ int pidx(int i, int ps) { return ps ? i + (i >> ps) : i; }
+// The pass descriptors always pad with one slot every 16 elements (padshift 4),
+// so the shift is a constant here.
+__device__ __forceinline__ int pidx(int i, int) { return i + (i >> 4); }
+
+// q = n / d, r = n % d for 0 <= n < 2^22, d >= 1: the float product n * (1/d)
+// is within 0.5 of n / d there, so truncation is off by at most one.
+__device__ __forceinline__ void divmod_small(int n, int d, float rcp, int &q, int &r) {
+    q = (int)((float)n * rcp);
+    r = n - q * d;
+    if (r < 0) {
+        --q;
+        r += d;
+    } else if (r >= d) {
+        ++q;
+        r -= d;
+    }
+}
 
 // One Stockham stage of radix R over a tile of B lines of length L.
 // Butterfly i of a line reads elements i + j L/R and writes
@@ -245,6 +262,7 @@ __device__ __forceinline__ void stage(typename Cx<T>::C *lds, const int L, const
     const int nbf = B * nbl;
     const int tws = L / (p * R);
     const int tid = threadIdx.x, NT = blockDim.x;
+    const float rnbl = 1.0f / (float)nbl, rp = 1.0f / (float)p;
     C v[MB][R];
     int base[MB], ii[MB];
 #pragma unroll
@@ -253,8 +271,8 @@ __device__ __forceinline__ void stage(typename Cx<T>::C *lds, const int L, const
         base[b] = 0;
         ii[b] = 0;
         if (bf < nbf) {
-            const int line = bf / nbl;
-            const int i = bf - line * nbl;
+            int line, i;
+            divmod_small(bf, nbl, rnbl, line, i);
             base[b] = line * ld;
             ii[b] = i;
 #pragma unroll
@@ -267,7 +285,8 @@ __device__ __forceinline__ void stage(typename Cx<T>::C *lds, const int L, const
         const int bf = tid + b * NT;
         if (bf < nbf) {
             const int i = ii[b];
-            const int k = i % p;
+            int q, k;
+            divmod_small(i, p, rp, q, k);
             if (p > 1) {
 #pragma unroll
                 for (int j = 1; j < R; ++j)
@@ -282,8 +301,8 @@ __device__ __forceinline__ void stage(typename Cx<T>::C *lds, const int L, const
     __syncthreads();
 }
 
-template <typename T>
-__device__ __forceinline__ void run_stages(typename Cx<T>::C *lds, const AfPass &d, const T s) {
+template <typename T, typename D>
+__device__ __forceinline__ void run_stages(typename Cx<T>::C *lds, const D &d, const T s) {
     using W = typename Cx<T>::W;
     const void *tw = d.tw;
     const bool plain = (d.flags & AFP_PLAIN_TW) != 0;
@@ -326,7 +345,8 @@ __device__ __forceinline__ void st_c(void *re, void *im, int64_t a, typename Cx<
 // fine[a & mask] of two table entries.  FP64: plain double product.  FP32: the
 // hi parts' product plus the first-order lo corrections, accurate to about one
 // FP32 ulp.
-__device__ __forceinline__ double2 ft_twiddle_t(const AfPass &d, uint64_t a, double) {
+template <typename D>
+__device__ __forceinline__ double2 ft_twiddle_t(const D &d, uint64_t a, double) {
     const double2 c = gld<double2>(d.ft_coarse, a >> d.ftShift);
     const double2 f = gld<double2>(d.ft_fine, a & ((uint64_t(1) << d.ftShift) - 1));
     double2 w = cmul(c, f);
@@ -334,14 +354,16 @@ __device__ __forceinline__ double2 ft_twiddle_t(const AfPass &d, uint64_t a, dou
     return w;
 }
 // FP64 four-step twiddle from the FP64 tables (both precisions).
-__device__ __forceinline__ double2 ft_twiddle_d(const AfPass &d, uint64_t a) {
+template <typename D>
+__device__ __forceinline__ double2 ft_twiddle_d(const D &d, uint64_t a) {
     const double2 c = gld<double2>(d.ft_coarse_d, a >> d.ftShift);
     const double2 f = gld<double2>(d.ft_fine_d, a & ((uint64_t(1) << d.ftShift) - 1));
     double2 w = cmul(c, f);
     if (d.ftSign > 0) w.y = -w.y;
     return w;
 }
-__device__ __forceinline__ float2 ft_twiddle_t(const AfPass &d, uint64_t a, float) {
+template <typename D>
+__device__ __forceinline__ float2 ft_twiddle_t(const D &d, uint64_t a, float) {
     const float4 c = gld<float4>(d.ft_coarse, a >> d.ftShift);
     const float4 f = gld<float4>(d.ft_fine, a & ((uint64_t(1) << d.ftShift) - 1));
     float re = fmaf(c.x, f.z, fmaf(-c.y, f.w, fmaf(c.z, f.x, -c.w * f.y)));
@@ -393,14 +415,16 @@ __device__ __forceinline__ typename Cx<T>::C entangle(typename Cx<T>::C p0, type
 // One tile of a pass: load -> stages -> [spectrum product -> inverse stages]
 // -> store.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ void pass_tile(const AfPass &d, const int64_t tile, typename Cx<T>::C *lds, TileLines &tl,
+// D is AfPass, or AfPass in the kernel-argument (constant) address space.
+template <typename T, bool FULL, typename D>
+__device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typename Cx<T>::C *lds, TileLines &tl,
                           typename Cx<T>::C *s_x0, typename Cx<T>::C *s_dc) {
     using C = typename Cx<T>::C;
     using W = typename Cx<T>::W;
     const int tid = threadIdx.x, NT = blockDim.x;
     const int B = d.B, L = d.L, ld = d.ld, ps = d.padshift;
-    const bool plain = (d.flags & AFP_PLAIN_TW) != 0;
+    const int32_t flags = FULL ? d.flags : (d.flags & ~kRarePassFlags);
+    const bool plain = (flags & AFP_PLAIN_TW) != 0;
     const int64_t nlines_total = d.nrows * d.nlines;
 
     if (tid < B) {
@@ -409,34 +433,35 @@ __device__ __forceinline__ void pass_tile(const AfPass &d, const int64_t tile, t
         tl.valid[tid] = v;
         tl.row[tid] = v ? gl / d.nlines : 0;
         tl.line[tid] = v ? gl - (gl / d.nlines) * d.nlines : 0;
-        if (v && (d.flags & AFP_RADER)) s_x0[tid] = ld_c<T>(d.in_re, d.in_im, tl.row[tid] * d.in_bs);
-        if (v && (d.flags & AFP_X0_CAPTURE) && tl.line[tid] == 0)
+        if (v && (flags & AFP_RADER)) s_x0[tid] = ld_c<T>(d.in_re, d.in_im, tl.row[tid] * d.in_bs);
+        if (v && (flags & AFP_X0_CAPTURE) && tl.line[tid] == 0)
             gst<C>(d.x0_out, tl.row[tid], ld_c<T>(d.in_re, d.in_im, tl.row[tid] * d.in_bs));
     }
     __syncthreads();
 
     // ---- load phase ----
     const int total = B * L;
+    const float rB = 1.0f / (float)B, rL = 1.0f / (float)L;
     for (int idx = tid; idx < total; idx += NT) {
         int b, e;
-        if (d.in_lines_fast) { b = idx % B; e = idx / B; }
-        else { b = idx / L; e = idx - b * L; }
+        if (d.in_lines_fast) divmod_small(idx, B, rB, e, b);
+        else divmod_small(idx, L, rL, b, e);
         C v = mk(T(0), T(0));
         if (tl.valid[b]) {
             const int64_t r = tl.row[b], l = tl.line[b];
             const int64_t rb = r * d.in_bs;
-            if (d.flags & AFP_REAL_ENTANGLE) {
+            if (flags & AFP_REAL_ENTANGLE) {
                 // fromSpectrum (Real.hs:96-107): z = (S_e + S_{e+L}) + i W_{2L}^{+e} (S_e - S_{e+L})
                 const C p0 = ld_c<T>(d.in_re, d.in_im, rb + e);
                 const C p1 = ld_c<T>(d.in_re, d.in_im, rb + e + L);
                 const C t = twmul(csub(p0, p1), gld<W>(d.aux_tw, e), T(1), plain);
                 v = cadd(cadd(p0, p1), mk(-t.y, t.x));
-            } else if (d.flags & AFP_UNTANGLE_ADJ) {
+            } else if (flags & AFP_UNTANGLE_ADJ) {
                 // Adjoint of toSpectrum: x -> F(x) maps reals to the full spectrum, so
                 // its adjoint is G -> Re(F^H G) = fromSpectrum(H) with H the Hermitian
                 // part (G_k + conj G_{-k}) / 2, which makes F^H H real.  Load H and
                 // entangle it exactly as fromSpectrum does.
-                const int er = (L - e) % L;
+                const int er = e ? L - e : 0;
                 const C g0 = ld_c<T>(d.in_re, d.in_im, rb + e);
                 const C g1 = ld_c<T>(d.in_re, d.in_im, rb + e + L);
                 const C h0 = ld_c<T>(d.in_re, d.in_im, rb + er + L);
@@ -454,7 +479,8 @@ __device__ __forceinline__ void pass_tile(const AfPass &d, const int64_t tile, t
                 const C t = twmul(csub(q0, q1), gld<W>(d.aux_tw, e), T(1), plain);
                 v = cadd(cadd(q0, q1), mk(-t.y, t.x));
             } else {
-                const int64_t pos = l * d.in_ls + (int64_t)e * d.in_es;
+                // positions inside a row stay below 2^31 (lengths <= 2^30)
+                const int64_t pos = (int64_t)((uint32_t)l * (uint32_t)d.in_ls + (uint32_t)e * (uint32_t)d.in_es);
                 if (pos < d.in_len) {
                     const int64_t a = d.in_map ? (int64_t)gld<int32_t>(d.in_map, pos) : pos;
                     v = ld_c<T>(d.in_re, d.in_im, rb + a);
@@ -471,19 +497,20 @@ __device__ __forceinline__ void pass_tile(const AfPass &d, const int64_t tile, t
     run_stages<T>(lds, d, s1);
 
     // ---- convolution: spectrum product and inverse stages ----
-    if (d.flags & AFP_CONV) {
+    if (flags & AFP_CONV) {
         for (int idx = tid; idx < total; idx += NT) {
-            const int b = idx / L, e = idx - b * L;
+            int b, e;
+            divmod_small(idx, L, rL, b, e);
             if (!tl.valid[b]) continue;
             const int64_t r = tl.row[b], l = tl.line[b];
             const int li = pidx(b * ld + e, ps);
             C v = lds[li];
             if (l == 0 && e == 0) {
-                if (d.flags & AFP_RADER) s_dc[b] = v;
-                if (d.flags & AFP_DC_CAPTURE) gst<C>(d.dc_out, r, v);
+                if (flags & AFP_RADER) s_dc[b] = v;
+                if (flags & AFP_DC_CAPTURE) gst<C>(d.dc_out, r, v);
             }
-            const int64_t si = r * d.spec_bs + l * d.spec_ls + e;
-            if (d.flags & AFP_SPEC_PLAIN) lds[li] = cmul(v, gld<C>(d.spec, si));
+            const int64_t si = r * d.spec_bs + (int64_t)((uint32_t)l * (uint32_t)d.spec_ls + (uint32_t)e);
+            if (flags & AFP_SPEC_PLAIN) lds[li] = cmul(v, gld<C>(d.spec, si));
             else lds[li] = vmul(v, gld<W>(d.spec, si), plain);
         }
         __syncthreads();
@@ -491,34 +518,36 @@ __device__ __forceinline__ void pass_tile(const AfPass &d, const int64_t tile, t
     }
 
     // ---- store phase ----
-    if (d.flags & AFP_REAL_UNTANGLE) {
+    if (flags & AFP_REAL_UNTANGLE) {
         // toSpectrum (Real.hs:53-70) on the half-length spectrum Z of this line
         for (int idx = tid; idx < total; idx += NT) {
-            const int b = idx / L, e = idx - b * L;
+            int b, e;
+            divmod_small(idx, L, rL, b, e);
             if (!tl.valid[b]) continue;
             const int64_t ob = tl.row[b] * d.out_bs;
             const C zk = lds[pidx(b * ld + e, ps)];
-            const C zr = lds[pidx(b * ld + (L - e) % L, ps)];
+            const C zr = lds[pidx(b * ld + (e ? L - e : 0), ps)];
             const C xp = cscale(cadd(zk, zr), T(0.5)), xm = cscale(csub(zk, zr), T(0.5));
             const C ev = mk(xp.x, xm.y);
             // odds = i W_{2L}^e (xm.re + i xp.im) with the forward-sign W_{2L}^e
             const C t = twmul(mk(xm.x, xp.y), gld<W>(d.aux_tw, e), T(-1), plain);
             const C od = mk(-t.y, t.x);
             C lo = csub(ev, od), hi = cadd(ev, od);
-            if (d.flags & AFP_SCALE) { lo = cscale(lo, T(d.scale)); hi = cscale(hi, T(d.scale)); }
+            if (flags & AFP_SCALE) { lo = cscale(lo, T(d.scale)); hi = cscale(hi, T(d.scale)); }
             st_c<T>(d.out_re, d.out_im, ob + e, lo);
             st_c<T>(d.out_re, d.out_im, ob + e + L, hi);
         }
         __syncthreads();
         return;
     }
-    if (d.flags & AFP_ENTANGLE_ADJ) {
+    if (flags & AFP_ENTANGLE_ADJ) {
         // Adjoint of fromSpectrum's entangling z_e = S_e (1 + c_e) + S_{e+L} (1 - c_e),
         // c_e = i conj(W_{2L}^e): from the half-length cotangent U (after the
         // opposite-sign sub-transform) dS_e = U_e + conj(c_e) U_e and
         // dS_{e+L} = U_e - conj(c_e) U_e, with conj(c_e) = -i W_{2L}^e.
         for (int idx = tid; idx < total; idx += NT) {
-            const int b = idx / L, e = idx - b * L;
+            int b, e;
+            divmod_small(idx, L, rL, b, e);
             if (!tl.valid[b]) continue;
             const int64_t ob = tl.row[b] * d.out_bs;
             const C u = lds[pidx(b * ld + e, ps)];
@@ -531,28 +560,26 @@ __device__ __forceinline__ void pass_tile(const AfPass &d, const int64_t tile, t
         __syncthreads();
         return;
     }
-    const bool rader_any = (d.flags & (AFP_RADER | AFP_RADER_OUT)) != 0;
+    const bool rader_any = (flags & (AFP_RADER | AFP_RADER_OUT)) != 0;
     for (int idx = tid; idx < total; idx += NT) {
         int b, e;
-        if (d.out_lines_fast) { b = idx % B; e = idx / B; }
-        else { b = idx / L; e = idx - b * L; }
+        if (d.out_lines_fast) divmod_small(idx, B, rB, e, b);
+        else divmod_small(idx, L, rL, b, e);
         if (!tl.valid[b]) continue;
         const int64_t r = tl.row[b], l = tl.line[b];
-        const int64_t o = l * d.out_ls + (int64_t)e * d.out_es;
+        const int64_t o = (int64_t)((uint32_t)l * (uint32_t)d.out_ls + (uint32_t)e * (uint32_t)d.out_es);
         if (o >= d.out_len) continue;
         C v = lds[pidx(b * ld + e, ps)];
-        if (d.flags & AFP_POSTTW) {
-            // l * e < 2^62 for every supported N, so the product is exact in 64 bits
-            const uint64_t le = (uint64_t)l * (uint64_t)e, N = (uint64_t)d.ftN;
-            const uint64_t a = ((N & (N - 1)) == 0) ? (le & (N - 1)) : (le % N);
-            v = cmul(v, ft_twiddle_t(d, a, T(0)));
+        if (flags & AFP_POSTTW) {
+            // the lines of a four-step pass number N / L, so l * e < N: no reduction
+            v = cmul(v, ft_twiddle_t(d, (uint64_t)((uint32_t)l * (uint32_t)e), T(0)));
         }
         if (d.out_mul) v = vmul(v, gld<W>(d.out_mul, o), plain);
-        if (d.flags & AFP_SCALE) v = cscale(v, T(d.scale));
+        if (flags & AFP_SCALE) v = cscale(v, T(d.scale));
         const int64_t ob = r * d.out_bs;
         if (rader_any) {
             C x0;
-            if (d.flags & AFP_RADER) x0 = s_x0[b];
+            if (flags & AFP_RADER) x0 = s_x0[b];
             else x0 = ld_c<T>(d.x0_re, d.x0_im, r * d.x0_bs);
             v = cadd(v, x0);
         }
@@ -563,7 +590,7 @@ __device__ __forceinline__ void pass_tile(const AfPass &d, const int64_t tile, t
         // X[0] = x0 + sum of the other inputs = x0 + DC of the permuted sequence
         const int64_t r = tl.row[tid];
         C x0, dc;
-        if (d.flags & AFP_RADER) { x0 = s_x0[tid]; dc = s_dc[tid]; }
+        if (flags & AFP_RADER) { x0 = s_x0[tid]; dc = s_dc[tid]; }
         else { x0 = ld_c<T>(d.x0_re, d.x0_im, r * d.x0_bs); dc = gld<C>(d.dc_in, r); }
         st_c<T>(d.out_re, d.out_im, r * d.out_bs, cadd(x0, dc));
     }

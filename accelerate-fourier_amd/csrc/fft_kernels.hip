@@ -18,26 +18,39 @@ namespace af {
 // Pass kernels
 // ---------------------------------------------------------------------------
 
-// All tiles of one workgroup: one call, so the function's register saves are
-// paid once per workgroup rather than once per tile.
-template <typename T>
-__device__ __noinline__ void pass_tiles(const AfPass &d, typename Cx<T>::C *lds, TileLines &tl,
-                                        typename Cx<T>::C *s_x0, typename Cx<T>::C *s_dc) {
+// All tiles of one workgroup, inlined into the kernel.  The descriptor is read
+// in place from the kernel-argument segment through a pointer the compiler
+// cannot see through: its fields are then loaded where they are used instead
+// of all being held in registers (which spills), and no thread copies the
+// descriptor (as a reference to the parameter would make it do).
+template <typename T, bool FULL>
+__device__ __forceinline__ void pass_tiles_body(const AfPass &d, typename Cx<T>::C *lds, TileLines &tl,
+                                                typename Cx<T>::C *s_x0, typename Cx<T>::C *s_dc) {
     const bool one = (int64_t)gridDim.x == d.ntiles;
     for (int64_t t = blockIdx.x; t < d.ntiles; t += gridDim.x)
-        pass_tile<T>(d, one ? xcd_tile(t, d.ntiles) : t, lds, tl, s_x0, s_dc);
+        pass_tile<T, FULL>(d, one ? xcd_tile(t, d.ntiles) : t, lds, tl, s_x0, s_dc);
 }
 
 template <typename T>
+__device__ __forceinline__ const T *opaque(const T *p) {
+    asm volatile("" : "+v"(p));
+    return p;
+}
+
+template <typename T, bool FULL>
 __global__ void __launch_bounds__(1024) k_pass(const AfPass d) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ TileLines tl;
     __shared__ typename Cx<T>::C s_x0[64], s_dc[64];
-    pass_tiles<T>(d, (typename Cx<T>::C *)smem, tl, s_x0, s_dc);
+    (void)d;
+    const AfPass *dp = opaque((const AfPass *)(const void *)__builtin_amdgcn_kernarg_segment_ptr());
+    pass_tiles_body<T, FULL>(*dp, (typename Cx<T>::C *)smem, tl, s_x0, s_dc);
 }
 
 // tile_prefix[i] = first global tile of descriptor i; tile_prefix[ndesc] = total.
-template <typename T>
+// Not inlined: the register saves of the call are paid once per workgroup,
+// which loops over many tiles, and the descriptor in LDS is read directly.
+template <typename T, bool FULL>
 __device__ __noinline__ void pass_tiles_grouped(const AfPass *__restrict__ descs, const int64_t *__restrict__ tile_prefix,
                                                 const int ndesc, typename Cx<T>::C *lds, TileLines &tl,
                                                 typename Cx<T>::C *s_x0, typename Cx<T>::C *s_dc) {
@@ -60,17 +73,17 @@ __device__ __noinline__ void pass_tiles_grouped(const AfPass *__restrict__ descs
             __syncthreads();
             cur = lo;
         }
-        pass_tile<T>(sd, g - tile_prefix[lo], lds, tl, s_x0, s_dc);
+        pass_tile<T, FULL>(sd, g - tile_prefix[lo], lds, tl, s_x0, s_dc);
     }
 }
 
-template <typename T>
+template <typename T, bool FULL>
 __global__ void __launch_bounds__(1024)
 k_pass_grouped(const AfPass *__restrict__ descs, const int64_t *__restrict__ tile_prefix, const int ndesc) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ TileLines tl;
     __shared__ typename Cx<T>::C s_x0[64], s_dc[64];
-    pass_tiles_grouped<T>(descs, tile_prefix, ndesc, (typename Cx<T>::C *)smem, tl, s_x0, s_dc);
+    pass_tiles_grouped<T, FULL>(descs, tile_prefix, ndesc, (typename Cx<T>::C *)smem, tl, s_x0, s_dc);
 }
 
 // ---------------------------------------------------------------------------
@@ -307,10 +320,14 @@ __global__ void k_convert(WT *__restrict__ out, const double2 *__restrict__ in, 
 
 void init_kernel_attributes() {
     const int lds = 160 * 1024;
-    (void)hipFuncSetAttribute((const void *)k_pass<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    (void)hipFuncSetAttribute((const void *)k_pass<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    (void)hipFuncSetAttribute((const void *)k_pass_grouped<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    (void)hipFuncSetAttribute((const void *)k_pass_grouped<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_pass<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_pass<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_pass<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_pass<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_pass_grouped<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_pass_grouped<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_pass_grouped<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_pass_grouped<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     (void)hipFuncSetAttribute((const void *)k_real_conv<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     (void)hipFuncSetAttribute((const void *)k_real_conv<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     (void)hipFuncSetAttribute((const void *)k_real_conv_vjp<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -339,7 +356,8 @@ static hipError_t launch_pass_t(const AfPass &d, hipStream_t s) {
     const int per_cu = std::max(1, (int)((160 * 1024) / std::max(d.lds_bytes + 4096, 1)));
     const int64_t cap = (int64_t)256 * per_cu * 4;
     const int64_t g = d.ntiles < cap ? d.ntiles : cap;
-    hipLaunchKernelGGL(k_pass<T>, dim3((unsigned)g), dim3(d.nthreads), d.lds_bytes, s, d);
+    if (d.flags & kRarePassFlags) hipLaunchKernelGGL((k_pass<T, true>), dim3((unsigned)g), dim3(d.nthreads), d.lds_bytes, s, d);
+    else hipLaunchKernelGGL((k_pass<T, false>), dim3((unsigned)g), dim3(d.nthreads), d.lds_bytes, s, d);
     return hipGetLastError();
 }
 
@@ -352,16 +370,21 @@ hipError_t launch_pass(int prec, const AfPass &d, hipStream_t s) {
 }
 
 hipError_t launch_pass_grouped(int prec, const AfPass *descs, const int64_t *prefix, int ndesc,
-                               int64_t total_tiles, int nthreads, int lds_bytes, hipStream_t s) {
+                               int64_t total_tiles, int nthreads, int lds_bytes, bool full, hipStream_t s) {
     if (total_tiles <= 0) return hipSuccess;
     if (!pass_lds_fits(prec, lds_bytes)) return hipErrorInvalidValue;
     const int per_cu = std::max(1, (int)((160 * 1024) / std::max(lds_bytes + 4096, 1)));
     const int64_t cap = (int64_t)256 * per_cu * 4;
     const int64_t g = total_tiles < cap ? total_tiles : cap;
-    if (prec == PREC_F64)
-        hipLaunchKernelGGL(k_pass_grouped<double>, dim3((unsigned)g), dim3(nthreads), lds_bytes, s, descs, prefix, ndesc);
+    const dim3 gd((unsigned)g), bd(nthreads);
+    if (prec == PREC_F64 && full)
+        hipLaunchKernelGGL((k_pass_grouped<double, true>), gd, bd, lds_bytes, s, descs, prefix, ndesc);
+    else if (prec == PREC_F64)
+        hipLaunchKernelGGL((k_pass_grouped<double, false>), gd, bd, lds_bytes, s, descs, prefix, ndesc);
+    else if (full)
+        hipLaunchKernelGGL((k_pass_grouped<float, true>), gd, bd, lds_bytes, s, descs, prefix, ndesc);
     else
-        hipLaunchKernelGGL(k_pass_grouped<float>, dim3((unsigned)g), dim3(nthreads), lds_bytes, s, descs, prefix, ndesc);
+        hipLaunchKernelGGL((k_pass_grouped<float, false>), gd, bd, lds_bytes, s, descs, prefix, ndesc);
     return hipGetLastError();
 }
 

@@ -10,8 +10,8 @@ namespace af {
 enum { PREC_F64 = 0, PREC_F32 = 1 };
 
 hipError_t launch_pass(int prec, const AfPass &d, hipStream_t s);
-hipError_t launch_pass_grouped(int prec, const AfPass *descs, const int64_t *prefix, int ndesc,
-                               int64_t total_tiles, int nthreads, int lds_bytes, hipStream_t s);
+hipError_t launch_pass_grouped(int prec, const AfPass *descs, const int64_t *prefix, int ndesc, int64_t total_tiles,
+                               int nthreads, int lds_bytes, bool full, hipStream_t s);
 hipError_t launch_real_conv(int prec, const AfPass &d, const void *x, const void *k, void *y, hipStream_t s);
 hipError_t launch_real_conv_vjp(int prec, const AfPass &d, const void *x, const void *k, const void *w,
                                 void *dx, void *dk, hipStream_t s);

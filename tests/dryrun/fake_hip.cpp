@@ -147,6 +147,13 @@ void check_pass(const AfPass &d, int tsz, const std::string &k) {
     }
     if (d.nrows * d.nlines == 0) return;
     if (d.ntiles * d.B < d.nrows * d.nlines) problem(k + ": tiles do not cover the lines");
+    // positions inside a row are computed in 32 bits; four-step indices l * e without reduction
+    if ((d.nlines - 1) * d.in_ls + (int64_t)(d.L - 1) * d.in_es >= ((int64_t)1 << 31) ||
+        (d.nlines - 1) * d.out_ls + (int64_t)(d.L - 1) * d.out_es >= ((int64_t)1 << 31))
+        problem(k + ": row positions above 31 bits");
+    if ((d.flags & AFP_POSTTW) && (d.nlines - 1) * (int64_t)(d.L - 1) >= d.ftN)
+        problem(k + ": four-step index l * e reaches N");
+    if (d.B * (int64_t)d.L >= ((int64_t)1 << 22)) problem(k + ": tile index above the small division range");
     const int64_t R1 = d.nrows - 1;
     // stage tables: tw[j*k*tws] < L entries of 16 bytes
     if (d.nst > 0 || d.L > 1) in_alloc(d.tw, 0, (int64_t)d.L * 16, "tw", k);
