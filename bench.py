@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--profile", action="store_true",
                     help="only the headline transform (steps launches) and a 32 MiB device copy of "
                          "known size, for rocprofv3 kernel traces and counters")
+    ap.add_argument("--profile-uneven", action="store_true",
+                    help="only the 4096-row uneven batch (FP32 forward, warmup + steps runs), for "
+                         "rocprofv3 kernel traces")
     return ap.parse_args()
 
 
@@ -188,6 +191,21 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs the GPU"
     torch.cuda.init()
     steps, warmup = args.steps, args.warmup
+    if args.profile_uneven:
+        from accelerate_fourier import batch as ubatch, forward, synthetic as syn
+        lens, _ = syn.uneven_lengths(2)
+        tot = int(lens.sum())
+        b32 = ubatch.UnevenBatch(lens, forward, "f32")
+        v = syn.uniform(2, 2 * tot)
+        x32 = torch.complex(torch.from_numpy(v[:tot].astype(np.float32)),
+                            torch.from_numpy(v[tot:].astype(np.float32))).cuda()
+        del v
+        y = b32(x32)
+        for _ in range(warmup + steps):
+            b32(x32, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile": "done", "uneven_runs": 1 + warmup + steps, "plan": b32.describe()}))
+        return
     n, batch = 65536, 64
     flops = 5.0 * n * math.log2(n) * batch
     alg_bytes = 2 * 8 * n * batch  # one read and one write of the C32 array
