@@ -6,15 +6,20 @@
 // exact splitting gives the same bits, so here one workgroup handles one pair
 // and organises the same algebra around LDS:
 //
-//  * both operands are zero-padded to P = 2^d s (leaf size s <= 64) in LDS;
-//  * leaf t in {0,1,2}^d is the sum of the operand blocks selected by t
+//  * operands are zero-padded to P = 2^d s (leaf size s <= 64); the d levels
+//    are split into T = d - B top levels and B = min(d, 3) bottom levels;
+//  * the top levels are walked one subtree at a time: subtree t in {0,1,2}^T
+//    takes the sum of the operand blocks of size Q = 2^B s that t selects
 //    (t_i = 0: low half, 2: high half, 1: both -- the Karatsuba evaluation);
-//  * one wavefront forms one leaf product of two s-long blocks, each lane
-//    owning output coefficients lane and lane + s;
-//  * the leaf product is added into the LDS result with the signs and shifts of
-//    prod_i c_{t_i}(u_i), c_0 = 1 - u, c_1 = u, c_2 = u^2 - u (the Karatsuba
-//    interpolation, unrolled over the levels), using 64-bit LDS atomics.
+//  * inside a subtree, one wavefront forms one of its 3^B leaf products (two
+//    s-long blocks, schoolbook, each lane owning coefficients lane and
+//    lane + s), and the bottom B levels are recombined in LDS, level by level;
+//  * the subtree product is added to the LDS result with the shifts and signs
+//    of prod_i c_{t_i}(u_i), c_0 = 1 - u, c_1 = u, c_2 = u^2 - u, over the top
+//    levels -- one term at a time, so no atomics are needed.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kernels.hpp"
 
@@ -28,98 +33,161 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Base-3 digits of t over `levels` levels (level 1 = most significant digit):
+// base has bit (levels - lev) set where the digit is 2, freem where it is 1.
+__device__ __forceinline__ void digits3(int t, int levels, int &base, int &freem) {
+    base = 0;
+    freem = 0;
+    for (int lev = levels; lev >= 1; --lev) {
+        const int dig = t % 3;
+        t /= 3;
+        const int bit = 1 << (levels - lev);
+        if (dig == 2) base |= bit;
+        if (dig == 1) freem |= bit;
+    }
+}
+
+constexpr int kKarThreads = 1024;
+constexpr int kKarWaves = kKarThreads / 64;
+
+// LDS words (u64) of one pair: result 2P, subtree operands 2Q, leaf products
+// 3^B 2s, recombination buffer 3^(B-1) 4s, and 128 per wavefront.
+__host__ __device__ inline int karatsuba_lds_words(int P, int s, int B) {
+    int n3 = 1;
+    for (int i = 0; i < B; ++i) n3 *= 3;
+    const int Q = s << B;
+    return 2 * P + 2 * Q + n3 * 2 * s + (B ? n3 / 3 * 4 * s : 0) + kKarWaves * 128;
+}
+
 // meta[3p] = input offset, meta[3p+1] = length, meta[3p+2] = output offset
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kKarThreads)
 k_karatsuba(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__restrict__ z,
             const int64_t *__restrict__ meta, int64_t npairs, int cyclic) {
     extern __shared__ __align__(16) u64 sm[];
-    const int tid = threadIdx.x, NT = blockDim.x;
-    const int lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
+    const int tid = threadIdx.x, NT = kKarThreads;
+    const int lane = tid & 63, wid = tid >> 6;
     for (int64_t pi = blockIdx.x; pi < npairs; pi += gridDim.x) {
         const int64_t off = meta[3 * pi], Lp = meta[3 * pi + 1], zoff = meta[3 * pi + 2];
         if (Lp <= 0) continue;
         int d = 0;
-        int64_t s = Lp;
-        while (s > 64) {
+        int64_t s64 = Lp;
+        while (s64 > 64) {
             ++d;
-            s = (Lp + ((int64_t)1 << d) - 1) >> d;
+            s64 = (Lp + ((int64_t)1 << d) - 1) >> d;
         }
-        const int P = (int)(s << d);
-        u64 *xs = sm, *ys = sm + P, *acc = sm + 2 * P, *wb = sm + 4 * P;
-        u64 *wx = wb + wid * 128, *wy = wx + 64;
-        for (int i = tid; i < P; i += NT) {
-            xs[i] = i < Lp ? x[off + i] : 0ull;
-            ys[i] = i < Lp ? y[off + i] : 0ull;
-        }
+        const int s = (int)s64, P = s << d;
+        const int B = d < 3 ? d : 3, T = d - B, Q = s << B;
+        int nleaf = 1, nsub = 1;
+        for (int i = 0; i < B; ++i) nleaf *= 3;
+        for (int i = 0; i < T; ++i) nsub *= 3;
+        u64 *acc = sm, *qx = acc + 2 * P, *qy = qx + Q, *R2 = qy + Q, *R1 = R2 + nleaf * 2 * s;
+        u64 *wx = R1 + (B ? nleaf / 3 * 4 * s : 0) + wid * 128, *wy = wx + 64;
         for (int i = tid; i < 2 * P; i += NT) acc[i] = 0ull;
-        __syncthreads();
-
-        int nleaves = 1;
-        for (int i = 0; i < d; ++i) nleaves *= 3;
-        const int si = (int)s;
-        for (int leaf = wid; leaf < nleaves; leaf += nw) {
-            // digits: level lev (1 = top split) is the lev-th most significant base-3 digit
-            // base: levels with digit 2, freem: levels with digit 1 (bit d - lev)
-            int base = 0, freem = 0, rem = leaf;
-            for (int lev = d; lev >= 1; --lev) {
-                const int dig = rem % 3;
-                rem /= 3;
-                const int bit = 1 << (d - lev);
-                if (dig == 2) base |= bit;
-                if (dig == 1) freem |= bit;
-            }
-            // evaluation: sum of the selected blocks
-            u64 X = 0, Y = 0;
-            if (lane < si) {
-                int m = freem;
+        for (int sub = 0; sub < nsub; ++sub) {
+            int tbase, tfree;
+            digits3(sub, T, tbase, tfree);
+            // subtree operands: sum of the selected Q-blocks, straight from HBM
+            __syncthreads();
+            for (int j = tid; j < Q; j += NT) {
+                u64 X = 0, Y = 0;
+                int m = tfree;
                 for (;;) {
-                    const int b = base | m;
-                    X += xs[b * si + lane];
-                    Y += ys[b * si + lane];
+                    const int64_t idx = (int64_t)(tbase | m) * Q + j;
+                    if (idx < Lp) {
+                        X += x[off + idx];
+                        Y += y[off + idx];
+                    }
                     if (m == 0) break;
-                    m = (m - 1) & freem;
+                    m = (m - 1) & tfree;
                 }
+                qx[j] = X;
+                qy[j] = Y;
             }
-            wx[lane] = X;
-            wy[lane] = Y;
-            wave_sync();
-            // leaf product: lane j owns coefficients j and j + s
-            u64 lo = 0, hi = 0;
-            if (lane < si) {
-                for (int i = 0; i < si; ++i) {
-                    const u64 xi = wx[i];
-                    const int j = lane - i;
-                    if (j >= 0) lo += xi * wy[j];
-                    else hi += xi * wy[j + si];
+            __syncthreads();
+            // leaf products, one wavefront per leaf
+            for (int leaf = wid; leaf < nleaf; leaf += kKarWaves) {
+                int base, freem;
+                digits3(leaf, B, base, freem);
+                u64 X = 0, Y = 0;
+                if (lane < s) {
+                    int m = freem;
+                    for (;;) {
+                        const int b = base | m;
+                        X += qx[b * s + lane];
+                        Y += qy[b * s + lane];
+                        if (m == 0) break;
+                        m = (m - 1) & freem;
+                    }
                 }
+                wx[lane] = X;
+                wy[lane] = Y;
+                wave_sync();
+                u64 lo = 0, hi = 0;
+                if (lane < s) {
+                    for (int i = 0; i < s; ++i) {
+                        const u64 xi = wx[i];
+                        const int j = lane - i;
+                        if (j >= 0) lo += xi * wy[j];
+                        else hi += xi * wy[j + s];
+                    }
+                    u64 *pr = R2 + leaf * 2 * s;
+                    pr[lane] = lo;
+                    pr[s + lane] = lane < s - 1 ? hi : 0ull;  // the last slot is the zero pad
+                }
+                wave_sync();
             }
-            // interpolation terms
-            const int nvar = d - __popc(freem);
+            __syncthreads();
+            // bottom levels, deepest first: triples (digit 0, 1, 2) of products of
+            // 2h slots become one product of 4h slots, z0 + u (z1 - z0 - z2) + u^2 z2
+            u64 *src = R2, *dst = R1;
+            int h = s, cnt = nleaf;
+            for (int lev = B; lev >= 1; --lev) {
+                const int ng = cnt / 3, w4 = 4 * h;
+                for (int idx = tid; idx < ng * w4; idx += NT) {
+                    const int g = idx / w4, k = idx - g * w4;
+                    const u64 *z0 = src + 3 * g * 2 * h, *z1 = z0 + 2 * h, *z2 = z1 + 2 * h;
+                    u64 v = 0;
+                    if (k < 2 * h) v += z0[k];
+                    if (k >= h && k < 3 * h) v += z1[k - h] - z0[k - h] - z2[k - h];
+                    if (k >= 2 * h) v += z2[k - 2 * h];
+                    dst[idx] = v;
+                }
+                __syncthreads();
+                u64 *tmp = src;
+                src = dst;
+                dst = tmp;
+                h *= 2;
+                cnt = ng;
+            }
+            // src: the subtree product (2Q slots); add it for each top-level term
+            const int nvar = T - __popc(tfree);
             for (int c = 0; c < (1 << nvar); ++c) {
                 int64_t shift = 0;
                 bool neg = false;
                 int v = 0;
-                for (int lev = 1; lev <= d; ++lev) {
-                    const int64_t h = s << (d - lev);
-                    const int bit = 1 << (d - lev);
-                    const int t = (base & bit) ? 2 : ((freem & bit) ? 1 : 0);
+                for (int lev = 1; lev <= T; ++lev) {
+                    const int64_t hl = (int64_t)s << (d - lev);
+                    const int bit = 1 << (T - lev);
+                    const int t = (tbase & bit) ? 2 : ((tfree & bit) ? 1 : 0);
                     if (t == 1) {
-                        shift += h;
+                        shift += hl;
                     } else {
                         const int bc = (c >> v) & 1;
                         ++v;
                         if (t == 0) {
-                            if (bc) { shift += h; neg = !neg; }
+                            if (bc) { shift += hl; neg = !neg; }
                         } else {
-                            if (bc) shift += 2 * h;
-                            else { shift += h; neg = !neg; }
+                            if (bc) shift += 2 * hl;
+                            else { shift += hl; neg = !neg; }
                         }
                     }
                 }
-                if (lane < si) atomicAdd(&acc[shift + lane], neg ? (u64)0 - lo : lo);
-                if (lane < si - 1) atomicAdd(&acc[shift + lane + si], neg ? (u64)0 - hi : hi);
+                for (int k = tid; k < 2 * Q - 1; k += NT) {
+                    const u64 p = src[k];
+                    acc[shift + k] += neg ? (u64)0 - p : p;
+                }
+                __syncthreads();
             }
-            wave_sync();
         }
         __syncthreads();
         if (cyclic) {
@@ -139,10 +207,15 @@ void init_karatsuba_attributes() {
 hipError_t launch_karatsuba(const int64_t *x, const int64_t *y, int64_t *z, const int64_t *dev_meta, int64_t npairs,
                             int max_P, int cyclic, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
-    const int nthreads = 1024;
-    const size_t lds = (size_t)(4 * max_P + (nthreads / 64) * 128) * sizeof(u64);
+    // LDS of the largest pair: P = 2^d s with s <= 64
+    int words = 0;
+    for (int d = 0; (1 << d) <= max_P; ++d)
+        for (int sl = 1; sl <= 64; ++sl)
+            if ((sl << d) <= max_P) words = std::max(words, karatsuba_lds_words(sl << d, sl, d < 3 ? d : 3));
+    const size_t lds = (size_t)words * sizeof(u64);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
     const int64_t g = npairs < (int64_t)1 << 30 ? npairs : (int64_t)1 << 30;
-    hipLaunchKernelGGL(k_karatsuba, dim3((unsigned)g), dim3(nthreads), lds, s, (const u64 *)x, (const u64 *)y,
+    hipLaunchKernelGGL(k_karatsuba, dim3((unsigned)g), dim3(kKarThreads), lds, s, (const u64 *)x, (const u64 *)y,
                        (u64 *)z, dev_meta, npairs, cyclic);
     return hipGetLastError();
 }

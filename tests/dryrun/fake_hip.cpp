@@ -333,6 +333,39 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
             else in_alloc(d.tw_d, 0, (int64_t)L * 16, "tw_d", name);
         };
     }
+    if (name.find("af::k_karatsuba") == 0) {
+        const void *x = *(const void *const *)args[0];
+        const void *y = *(const void *const *)args[1];
+        void *z = *(void **)args[2];
+        const int64_t *meta = *(const int64_t *const *)args[3];
+        const int64_t np = *(const int64_t *)args[4];
+        const int cyc = *(const int *)args[5];
+        return [=] {
+            if (!in_alloc(meta, 0, np * 24, "meta", name)) return;
+            for (int64_t p = 0; p < np; ++p) {
+                const int64_t off = meta[3 * p], L = meta[3 * p + 1], zo = meta[3 * p + 2];
+                if (L <= 0) continue;
+                in_alloc(x, off * 8, (off + L) * 8, "x", name);
+                in_alloc(y, off * 8, (off + L) * 8, "y", name);
+                in_alloc(z, zo * 8, (zo + (cyc ? L : 2 * L - 1)) * 8, "z", name);
+                // LDS words of the pair: 2P + 2Q + 3^B 2s + 3^(B-1) 4s + 16 * 128
+                int d = 0;
+                int64_t sl = L;
+                while (sl > 64) {
+                    ++d;
+                    sl = (L + ((int64_t)1 << d) - 1) >> d;
+                }
+                const int64_t P = sl << d, B = d < 3 ? d : 3;
+                int64_t n3 = 1;
+                for (int i = 0; i < B; ++i) n3 *= 3;
+                const int64_t words = 2 * P + 2 * (sl << B) + n3 * 2 * sl + (B ? n3 / 3 * 4 * sl : 0) + 16 * 128;
+                if (words * 8 > (int64_t)shmem)
+                    problem(name + ": a pair needs more LDS than the launch has (L=" + std::to_string(L) + " need " +
+                            std::to_string(words * 8) + " have " + std::to_string(shmem) + ")");
+                if (block.x != 1024) problem(name + ": block differs from 1024");
+            }
+        };
+    }
     if (name.find("af::k_gen_twiddles<") == 0) {
         void *out = *(void **)args[0];
         const int64_t count = *(const int64_t *)args[2];

@@ -57,6 +57,9 @@ def dry():
                                     ctypes.POINTER(V)]
     lib.af_batch_execute.argtypes = [V, V, V, V, V, V]
     lib.af_batch_destroy.argtypes = [V]
+    lib.af_karatsuba_ragged_i64.argtypes = [ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), V, V, V, V]
+    lib.af_karatsuba_i64.argtypes = [ctypes.c_int64, ctypes.c_int64, V, V, V, V]
+    lib.af_karatsuba_cyclic_i64.argtypes = [ctypes.c_int64, ctypes.c_int64, V, V, V, V]
     return lib
 
 
@@ -194,5 +197,22 @@ def test_uneven_batch_4096_rows(dry, prec):
     tot = int(np.sum(lens))
     assert dry.af_batch_execute(h, b(tot * e), None, b(tot * e), None, None) == 0
     dry.af_batch_destroy(h)
+    b.free()
+    assert problems(dry, before) == []
+
+
+def test_karatsuba(dry):
+    before = dry.dry_problem_count()
+    b = Bufs(dry)
+    for L in (1, 2, 63, 64, 65, 127, 128, 129, 500, 512, 513, 1000, 2048, 2049, 4095, 4096, 4097, 4608):
+        for batch in (1, 3):
+            x, y = b(batch * L * 8), b(batch * L * 8)
+            assert dry.af_karatsuba_i64(batch, L, x, y, b(batch * (2 * L - 1) * 8), None) == 0
+            assert dry.af_karatsuba_cyclic_i64(batch, L, x, y, b(batch * L * 8), None) == 0
+    lens = np.array([256, 4096, 1, 77, 3000, 4096, 2049], dtype=np.int64)
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    arr = (ctypes.c_int64 * len(offs))(*[int(v) for v in offs])
+    tot = int(offs[-1])
+    assert dry.af_karatsuba_ragged_i64(len(lens), arr, b(tot * 8), b(tot * 8), b((2 * tot - len(lens)) * 8), None) == 0
     b.free()
     assert problems(dry, before) == []
