@@ -33,6 +33,19 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Whole-wavefront rotation by one lane (DPP wave_ror:1): lane l receives the
+// value of lane l - 1 (mod 64).
+__device__ __forceinline__ u64 rot1(u64 v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x13C, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), 0x13C, 0xF, 0xF, false);
+    return ((u64)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+__device__ __forceinline__ u64 readlane64(u64 v, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return ((u64)hi << 32) | lo;
+}
+
 // Base-3 digits of t over `levels` levels (level 1 = most significant digit):
 // base has bit (levels - lev) set where the digit is 2, freem where it is 1.
 __device__ __forceinline__ void digits3(int t, int levels, int &base, int &freem) {
@@ -81,7 +94,6 @@ k_karatsuba(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__restric
         for (int i = 0; i < B; ++i) nleaf *= 3;
         for (int i = 0; i < T; ++i) nsub *= 3;
         u64 *acc = sm, *qx = acc + 2 * P, *qy = qx + Q, *R2 = qy + Q, *R1 = R2 + nleaf * 2 * s;
-        u64 *wx = R1 + (B ? nleaf / 3 * 4 * s : 0) + wid * 128, *wy = wx + 64;
         for (int i = tid; i < 2 * P; i += NT) acc[i] = 0ull;
         for (int sub = 0; sub < nsub; ++sub) {
             int tbase, tfree;
@@ -119,22 +131,19 @@ k_karatsuba(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__restric
                         m = (m - 1) & freem;
                     }
                 }
-                wx[lane] = X;
-                wy[lane] = Y;
-                wave_sync();
-                u64 lo = 0, hi = 0;
-                if (lane < s) {
-                    for (int i = 0; i < s; ++i) {
-                        const u64 xi = wx[i];
-                        const int j = lane - i;
-                        if (j >= 0) lo += xi * wy[j];
-                        else hi += xi * wy[j + s];
-                    }
-                    u64 *pr = R2 + leaf * 2 * s;
-                    pr[lane] = lo;
-                    pr[s + lane] = lane < s - 1 ? hi : 0ull;  // the last slot is the zero pad
+                // schoolbook in registers: x_i is broadcast from lane i, y turns one lane per
+                // step, so lane l sees y_(l-i) mod 64; the blocks are zero beyond s, so lane l
+                // collects z_l (i <= l) and z_(l+64) (i > l)
+                u64 yr = Y, lo = 0, hi = 0;
+                for (int i = 0; i < s; ++i) {
+                    const u64 p = readlane64(X, i) * yr;
+                    if (i <= lane) lo += p;
+                    else hi += p;
+                    yr = rot1(yr);
                 }
-                wave_sync();
+                u64 *pr = R2 + leaf * 2 * s;  // 2s slots: z_0 .. z_(2s-2) and the zero pad
+                if (lane < 2 * s) pr[lane] = lo;
+                if (64 + lane < 2 * s) pr[64 + lane] = hi;
             }
             __syncthreads();
             // bottom levels, deepest first: triples (digit 0, 1, 2) of products of
