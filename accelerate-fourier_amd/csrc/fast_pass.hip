@@ -91,20 +91,27 @@ __device__ __forceinline__ Rsrc mk_rsrc(const void *p, uint32_t bytes) {
 }
 constexpr int kNT = 2;  // cache policy: nontemporal
 
-template <typename V, int AUX> __device__ __forceinline__ V bld(Rsrc r, uint32_t off) {
-    if constexpr (sizeof(V) == 4) return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, AUX));
-    else if constexpr (sizeof(V) == 8) return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX));
-    else return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, AUX));
+// The byte offset is split into a per-lane part (voff, a VGPR) and a part that
+// is uniform over the wavefront (soff, an SGPR): the per-element steps of a
+// butterfly go into soff and cost no vector instruction.
+template <typename V, int AUX> __device__ __forceinline__ V bld(Rsrc r, uint32_t voff, uint32_t soff) {
+    if constexpr (sizeof(V) == 4) return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
+    else if constexpr (sizeof(V) == 8) return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX));
+    else return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, AUX));
 }
 constexpr int kStoreAux = 0;  // stores: default policy (streaming stores measured no faster end to end)
-template <typename V> __device__ __forceinline__ void bst(Rsrc r, uint32_t off, V v) {
-    if constexpr (sizeof(V) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, kStoreAux);
+// 16-byte stores keep the whole offset in the VGPR: with an SGPR soffset the
+// compiler inserts no wait state between such a store and a vector write of its
+// data registers, and on gfx950 the store then wrote partly overwritten data
+// under load (wrong FP64 interleaved results, seen at 32+ rows of 65536).
+template <typename V> __device__ __forceinline__ void bst(Rsrc r, uint32_t voff, uint32_t soff, V v) {
+    if constexpr (sizeof(V) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, kStoreAux);
     else if constexpr (sizeof(V) == 8) {
         typedef uint32_t u2 __attribute__((ext_vector_type(2)));
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, off, 0, kStoreAux);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, kStoreAux);
     } else {
         typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, off, 0, kStoreAux);
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, voff + soff, 0, kStoreAux);
     }
 }
 
@@ -174,18 +181,18 @@ __device__ __forceinline__ void fast_load(const AfPass &d, const FastRow &r, typ
     if (r.in_planar && nt) {
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-            const uint32_t off = o0 + (uint32_t)j * step;
-            v[j] = mk(bld<T, kNT>(r.in_re, off), bld<T, kNT>(r.in_im, off));
+            const uint32_t so = (uint32_t)j * step;
+            v[j] = mk(bld<T, kNT>(r.in_re, o0, so), bld<T, kNT>(r.in_im, o0, so));
         }
     } else if (r.in_planar) {
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-            const uint32_t off = o0 + (uint32_t)j * step;
-            v[j] = mk(bld<T, 0>(r.in_re, off), bld<T, 0>(r.in_im, off));
+            const uint32_t so = (uint32_t)j * step;
+            v[j] = mk(bld<T, 0>(r.in_re, o0, so), bld<T, 0>(r.in_im, o0, so));
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < 16; ++j) v[j] = bld<C, 0>(r.in_re, o0 + (uint32_t)j * step);
+        for (int j = 0; j < 16; ++j) v[j] = bld<C, 0>(r.in_re, o0, (uint32_t)j * step);
     }
 }
 
@@ -197,12 +204,12 @@ __device__ __forceinline__ void fast_store(const FastRow &r, const int R, const 
                                            const double2 tw_step, const T sc, const uint32_t o0, const uint32_t step) {
     using C = typename Cx<T>::C;
     auto put = [&](int j, C x) {
-        const uint32_t off = o0 + (uint32_t)j * step;
+        const uint32_t so = (uint32_t)j * step;
         if (PLANAR) {
-            bst<T>(r.out_re, off, x.x);
-            bst<T>(r.out_im, off, x.y);
+            bst<T>(r.out_re, o0, so, x.x);
+            bst<T>(r.out_im, o0, so, x.y);
         } else {
-            bst<C>(r.out_re, off, x);
+            bst<C>(r.out_re, o0, so, x);
         }
     };
     if (mode == 2) {
@@ -444,7 +451,7 @@ __device__ __forceinline__ void fast_rows_load(const T *src, int64_t row0, typen
     const Rsrc r = mk_rsrc(src + row0 * 2 * L, (uint32_t)(B * L * sizeof(C)));
     const uint32_t o0 = (uint32_t)((b * L + i) * sizeof(C));
 #pragma unroll
-    for (int j = 0; j < 16; ++j) v[j] = bld<C, kNT>(r, o0 + (uint32_t)(j * NB * sizeof(C)));
+    for (int j = 0; j < 16; ++j) v[j] = bld<C, kNT>(r, o0, (uint32_t)(j * NB * sizeof(C)));
 }
 
 // Inverse transform of the B lines in buf (natural order), stored to HBM as
@@ -465,7 +472,7 @@ __device__ __forceinline__ void fast_inverse_store(const AfPass &d, typename Cx<
     const Rsrc r = mk_rsrc(dst + row0 * 2 * L, (uint32_t)(B * L * sizeof(C)));
     const uint32_t o0 = (uint32_t)((b * L + i) * sizeof(C));
 #pragma unroll
-    for (int j = 0; j < 16; ++j) bst<C>(r, o0 + (uint32_t)(j * NB * sizeof(C)), buf[b * LD + fpad(i + j * NB)]);
+    for (int j = 0; j < 16; ++j) bst<C>(r, o0, (uint32_t)(j * NB * sizeof(C)), buf[b * LD + fpad(i + j * NB)]);
     lds_barrier();
 }
 

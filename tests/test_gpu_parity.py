@@ -99,6 +99,31 @@ def test_c2c_big(n, prec):
     _check_c2c(n, -1, prec, "interleaved", rows=2)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("layout", ["interleaved", "planar"])
+def test_c2c_65536_batch64(prec, layout):
+    """64 x 65536 (the batched four-step shape) with every tile resident at
+    once, run twice: both runs within tolerance of numpy's FP64 FFT and
+    identical to each other."""
+    x = synthetic.complex_normal(1, (64, 65536))
+    ref = np.fft.fft(x, axis=1)
+    ct = torch.complex128 if prec == "f64" else torch.complex64
+    rt = torch.float64 if prec == "f64" else torch.float32
+    t = planned.transform(forward, 65536)
+    outs = []
+    for _ in range(2):
+        if layout == "planar":
+            y = t((to_dev(x.real, rt), to_dev(x.imag, rt)))
+            outs.append(host(y[0]) + 1j * host(y[1]))
+        else:
+            outs.append(host(t(to_dev(x, ct))))
+    tol = 1e-13 if prec == "f64" else 1e-6
+    for got in outs:
+        worst = max(rel_l2(got[r], ref[r]) for r in range(64))
+        assert worst <= tol, worst
+    assert np.array_equal(outs[0], outs[1])
+
+
 def test_config1_plain_f64():
     """256 x 1024 Complex Double, U[-1,1) from seed 0: forward spectrum and
     round trip within 1e-12 relative L2 of the FP64 oracle."""
