@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cstdio>
 #include <cstring>
@@ -409,7 +410,10 @@ void conv_passes(int prec, const Core &c, int64_t nrows, bool plain, const Endpo
     a.in_lines_fast = a.out_lines_fast = 1;
     set_ft(a, c, -1);
     ps.push_back(a);
-    AfPass b = base_pass(prec, (int)N2, c.sh.r2, c.tw2, c.tw2_d, choose_B_rows(prec, N2, nrows * N1), N1, nrows, -1, plain);
+    // power-of-two rows take the line count of their compile-time kernel
+    const int bconv = fast_conv_lines(prec, N2);
+    AfPass b = base_pass(prec, (int)N2, c.sh.r2, c.tw2, c.tw2_d, bconv ? bconv : choose_B_rows(prec, N2, nrows * N1), N1,
+                         nrows, -1, plain);
     set_in(b, sc, 1, N2);
     set_out(b, sc, 1, N2);
     b.flags |= AFP_CONV | spf;
@@ -1248,6 +1252,10 @@ struct af_batch_s {
     struct Group {
         int phase = 0, nthreads = 0, lds = 0;
         bool full = false;  // some descriptor sets a flag of the full pass kernel
+        // power-of-two passes of compile-time shape (L, B, conv, sign) run on the
+        // extended fast kernel instead of the generic one
+        bool ext = false, conv = false;
+        int L = 0, B = 0, sign = 0;
         std::vector<int64_t> rows;  // transform indices
         int64_t desc_off = 0, prefix_off = 0, total_tiles = 0;
     };
@@ -1347,22 +1355,40 @@ af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign, i
     }
     const int classes[] = {64, 128, 256, 512, 1024};
     int64_t ndesc = 0, nprefix = 0;
-    for (int ph = 0; ph < maxp; ++ph)
-        for (int c = 0; c < 5; ++c) {
-            af_batch_s::Group g;
-            g.phase = ph;
-            g.nthreads = classes[c];
-            for (int64_t i = 0; i < count; ++i) {
-                if (B->npasses[i] <= ph) continue;
-                const AfPass &d = per[i][ph];
+    for (int ph = 0; ph < maxp; ++ph) {
+        // groups of this phase: generic thread classes (key 0, class) and
+        // extended fast shapes (key 1, L, B, conv, sign)
+        std::map<std::array<int, 5>, af_batch_s::Group> gm;
+        for (int64_t i = 0; i < count; ++i) {
+            if (B->npasses[i] <= ph) continue;
+            const AfPass &d = per[i][ph];
+            std::array<int, 5> key;
+            if (fast_ext_eligible(B->prec, d)) {
+                key = {1, d.L, d.B, (d.flags & AFP_CONV) ? 1 : 0, d.sign};
+            } else {
                 const int cls = d.nthreads <= 64 ? 0 : d.nthreads <= 128 ? 1 : d.nthreads <= 256 ? 2
                                 : d.nthreads <= 512 ? 3 : 4;
-                if (cls != c) continue;
-                g.rows.push_back(i);
-                g.lds = std::max(g.lds, d.lds_bytes);
-                if (d.flags & kRarePassFlags) g.full = true;
+                key = {0, cls, 0, 0, 0};
             }
-            if (g.rows.empty()) continue;
+            af_batch_s::Group &g = gm[key];
+            if (g.rows.empty()) {
+                g.phase = ph;
+                if (key[0] == 1) {
+                    g.ext = true;
+                    g.L = d.L;
+                    g.B = d.B;
+                    g.conv = key[3] != 0;
+                    g.sign = d.sign;
+                } else {
+                    g.nthreads = classes[key[1]];
+                }
+            }
+            g.rows.push_back(i);
+            g.lds = std::max(g.lds, d.lds_bytes);
+            if (d.flags & kRarePassFlags) g.full = true;
+        }
+        for (auto &kv : gm) {
+            af_batch_s::Group g = kv.second;
             g.desc_off = ndesc;
             g.prefix_off = nprefix;
             int64_t t = 0;
@@ -1376,6 +1402,7 @@ af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign, i
             nprefix += (int64_t)g.rows.size() + 1;
             B->groups.push_back(g);
         }
+    }
     B->h_descs.resize(ndesc);
     AFCHK(B->own.alloc((void **)&B->d_descs, std::max<int64_t>(ndesc, 1) * sizeof(AfPass)));
     AFCHK(B->own.alloc((void **)&B->d_prefix, std::max<int64_t>(nprefix, 1) * 8));
@@ -1387,8 +1414,12 @@ af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign, i
     { const af_status h__ = HIPST(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)); if (st == AF_SUCCESS) st = h__; }
     for (const auto &g : B->groups) {
         if (st != AF_SUCCESS) break;
-        st = HIPST(launch_pass_grouped(B->prec, B->d_descs + g.desc_off, B->d_prefix + g.prefix_off,
-                                       (int)g.rows.size(), g.total_tiles, g.nthreads, g.lds, g.full, s));
+        if (g.ext)
+            st = HIPST(launch_fast_ext_grouped(B->prec, g.L, g.B, g.conv, g.sign, B->d_descs + g.desc_off,
+                                               B->d_prefix + g.prefix_off, (int)g.rows.size(), g.total_tiles, s));
+        else
+            st = HIPST(launch_pass_grouped(B->prec, B->d_descs + g.desc_off, B->d_prefix + g.prefix_off,
+                                           (int)g.rows.size(), g.total_tiles, g.nthreads, g.lds, g.full, s));
     }
     hipGraph_t graph = nullptr;
     { const af_status h__ = HIPST(hipStreamEndCapture(s, &graph)); if (st == AF_SUCCESS) st = h__; }

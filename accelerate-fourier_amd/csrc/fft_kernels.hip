@@ -334,6 +334,7 @@ void init_kernel_attributes() {
     (void)hipFuncSetAttribute((const void *)k_real_conv_vjp<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     init_karatsuba_attributes();
     init_fast_attributes();
+    init_fast_ext_attributes();
     (void)hipGetLastError();
 }
 
@@ -365,6 +366,8 @@ hipError_t launch_pass(int prec, const AfPass &d, hipStream_t s) {
     if (d.ntiles <= 0) return hipSuccess;
     bool done = false;
     hipError_t e = launch_fast(prec, d, s, done);
+    if (done || e != hipSuccess) return e;
+    e = launch_fast_ext(prec, d, s, done);
     if (done || e != hipSuccess) return e;
     return prec == PREC_F64 ? launch_pass_t<double>(d, s) : launch_pass_t<float>(d, s);
 }

@@ -38,6 +38,17 @@ hipError_t launch_fast(int prec, const AfPass &d, hipStream_t s, bool &done);
 // the largest line count <= B with a power-of-two kernel for lines of length L
 // (B itself when L has none)
 int fast_tile_lines(int prec, int64_t L, int B);
+// Power-of-two passes of convolution plans (fast_ext.hip): chirp / zero-pad
+// loads, spectrum product between forward and inverse stages, chirp / length
+// limited stores.  done = false when the descriptor has no such kernel.
+bool fast_ext_eligible(int prec, const AfPass &d);
+hipError_t launch_fast_ext(int prec, const AfPass &d, hipStream_t s, bool &done);
+// the same over the tiles of many descriptors with equal L, B, CONV and sign
+hipError_t launch_fast_ext_grouped(int prec, int L, int B, bool conv, int sign, const AfPass *descs,
+                                   const int64_t *prefix, int ndesc, int64_t total_tiles, hipStream_t s);
+// lines per tile of the middle convolution pass for lines of length L (0: none)
+int fast_conv_lines(int prec, int64_t L);
+void init_fast_ext_attributes();
 // fused real convolution (vjp = false: out0 = conv(in0, in1)) or its adjoint
 // (vjp = true: cotangent in0, out0 = dk from in1 = x, out1 = dx from in2 = k)
 // for power-of-two half lengths; done = false when the shape has no kernel

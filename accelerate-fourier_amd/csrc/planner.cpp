@@ -216,9 +216,19 @@ bool plan_shape(int prec, int64_t n, int algorithm, PlanShape &out, std::string 
         }
     }
     int64_t M;
-    if (algorithm == AF_ALG_CHIRP2) M = ceil_pow2(2 * n - 1);
-    else if (algorithm == AF_ALG_CHIRP235) M = ceil_5smooth(2 * n - 1);
-    else M = ceil_7smooth(2 * n - 1);
+    if (algorithm == AF_ALG_CHIRP2) {
+        M = ceil_pow2(2 * n - 1);
+    } else if (algorithm == AF_ALG_CHIRP235) {
+        M = ceil_5smooth(2 * n - 1);
+    } else {
+        // the smallest 7-smooth pad, except where a power of two gives a four-step
+        // core of power-of-two factors 256 ... 2048: its three convolution passes
+        // run on compile-time stage kernels, several times faster per element than
+        // the mixed-radix ones, which outweighs the up-to-2x longer pad
+        M = ceil_7smooth(2 * n - 1);
+        const int64_t M2 = ceil_pow2(2 * n - 1);
+        if (M2 >= ((int64_t)1 << 16) && M2 <= ((int64_t)1 << 22)) M = M2;
+    }
     CoreShape cb = core_shape(prec, M);
     if (cb.kind == CORE_NONE) {
         why = "padded length does not factor within the LDS limit";

@@ -251,7 +251,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom),
                      "algorithmic_bytes": alg_bytes,
-                     "kernel": "k_fast<float, 256, 32, -1> " + PASS_NAMES.get(dom, f"pass {dom}"),
+                     "kernel": "k_fast<float, 256, 16, -1> " + PASS_NAMES.get(dom, f"pass {dom}"),
                      "pass_ms": pass_ms},
     }
     y32 = f32.result()

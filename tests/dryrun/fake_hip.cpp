@@ -272,6 +272,35 @@ void check_fast_launch(const AfPass &d, int tsz, int L, int B, int sg, dim3 grid
     check_pass(d, tsz, k);
 }
 
+// k_fast_ext<T, L, B, SG, CONV> and the tiles of one descriptor of k_fast_ext_grouped
+void check_ext_desc(const AfPass &d, int tsz, int L, int B, int sg, bool conv, dim3 block, size_t shmem,
+                    const std::string &k) {
+    if (d.L != L || d.B != B) problem(k + ": descriptor shape differs from the instantiation");
+    if (d.sign != sg) problem(k + ": descriptor sign differs from the instantiation");
+    if (((d.flags & AFP_CONV) != 0) != conv) problem(k + ": CONV differs from the instantiation");
+    if (conv && d.in_lines_fast) problem(k + ": convolution lines must be rows");
+    if ((int64_t)block.x != (int64_t)B * L / 16) problem(k + ": block differs from B*L/16");
+    const int64_t need = (int64_t)fast_ld_h(L, B) * B * 2 * tsz + (L <= 1024 ? L : 0) * 16;
+    if ((int64_t)shmem != need) problem(k + ": LDS size differs from the kernel's layout");
+    if (shmem > 160 * 1024) problem(k + ": LDS above 160 KiB");
+    if (d.flags & ~(AFP_POSTTW | AFP_SCALE | AFP_CONV | AFP_SPEC_PLAIN)) problem(k + ": flags the kernel ignores");
+    if (d.in_map || d.out_map) problem(k + ": maps the kernel ignores");
+    if (d.nlines % B) problem(k + ": B does not divide the lines per row");
+    const int64_t eb = tsz;
+    const int64_t span_i = ((int64_t)(B - 1) * d.in_ls + (int64_t)(L - 1) * d.in_es) * (d.in_im ? eb : 2 * eb) + eb;
+    const int64_t span_o = ((int64_t)(B - 1) * d.out_ls + (int64_t)(L - 1) * d.out_es) * (d.out_im ? eb : 2 * eb) + eb;
+    if (span_i >= ((int64_t)1 << 32) || span_o >= ((int64_t)1 << 32)) problem(k + ": tile span above 32 bits");
+    in_alloc(d.tw_d, 0, (int64_t)L * 16, "tw_d", k);
+    if (L <= 1024) in_alloc(d.tw, 0, (int64_t)L * 16, "tw", k);
+    if (d.flags & AFP_POSTTW) {
+        const int64_t T = (int64_t)1 << d.ftShift;
+        in_alloc(d.ft_coarse_d, 0, (((d.ftN - 1) >> d.ftShift) + 1) * 16, "ft_coarse_d", k);
+        in_alloc(d.ft_fine_d, 0, std::min(T, d.ftN) * 16, "ft_fine_d", k);
+        if (d.ftN & (d.ftN - 1)) problem(k + ": four-step N not a power of two");
+    }
+    check_pass(d, tsz, k);
+}
+
 int tsize_of(const std::string &n) { return n.find("<double") != std::string::npos ? 8 : 4; }
 
 // template arguments of k_fast<T, L, B, SG>
@@ -296,6 +325,34 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
         if (!fast_args(name, L, B, sg)) return [=] { problem(name + ": cannot parse the instantiation"); };
         return [=] { check_fast_launch(d, tsz, L, B, sg, grid, block, shmem, name); };
     }
+    if (name.find("af::k_fast_ext<") == 0) {
+        const AfPass d = *(const AfPass *)args[0];
+        int L = 0, B = 0, sg = 0;
+        if (!fast_args(name, L, B, sg)) return [=] { problem(name + ": cannot parse the instantiation"); };
+        const bool conv = name.find("true>") != std::string::npos;
+        return [=] {
+            if ((int64_t)grid.x != d.ntiles) problem(name + ": grid differs from ntiles");
+            check_ext_desc(d, tsz, L, B, sg, conv, block, shmem, name);
+        };
+    }
+    if (name.find("af::k_fast_ext_grouped<") == 0) {
+        const AfPass *descs = *(const AfPass *const *)args[0];
+        const int64_t *prefix = *(const int64_t *const *)args[1];
+        const int nd = *(const int *)args[2];
+        int L = 0, B = 0, sg = 0;
+        if (!fast_args(name, L, B, sg)) return [=] { problem(name + ": cannot parse the instantiation"); };
+        const bool conv = name.find("true>") != std::string::npos;
+        return [=] {
+            if (!in_alloc(descs, 0, (int64_t)nd * sizeof(AfPass), "descs", name)) return;
+            if (!in_alloc(prefix, 0, (int64_t)(nd + 1) * 8, "tile_prefix", name)) return;
+            if (prefix[0] != 0) problem(name + ": tile prefix does not start at 0");
+            if ((int64_t)grid.x != prefix[nd]) problem(name + ": grid differs from the total tiles");
+            for (int i = 0; i < nd; ++i) {
+                if (prefix[i + 1] - prefix[i] != descs[i].ntiles) problem(name + ": tile prefix differs from ntiles");
+                check_ext_desc(descs[i], tsz, L, B, sg, conv, block, shmem, name + " desc " + std::to_string(i));
+            }
+        };
+    }
     if (name.find("af::k_pass_grouped<") == 0) {
         const AfPass *descs = *(const AfPass *const *)args[0];
         const int64_t *prefix = *(const int64_t *const *)args[1];
@@ -306,6 +363,18 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
             for (int i = 0; i < nd; ++i) {
                 if (prefix[i + 1] - prefix[i] != descs[i].ntiles) problem(name + ": tile prefix differs from ntiles");
                 check_generic_launch(descs[i], tsz, block.x, shmem, name + " desc " + std::to_string(i));
+            }
+            // optional record of the descriptors (AF_DRYRUN_DUMP=file): one line per
+            // descriptor "threads lds L B nlines nrows ntiles flags", "--" after each launch
+            if (const char *f = std::getenv("AF_DRYRUN_DUMP")) {
+                if (FILE *o = std::fopen(f, "a")) {
+                    for (int i = 0; i < nd; ++i)
+                        std::fprintf(o, "%u %zu %d %d %lld %lld %lld %d\n", block.x, shmem, descs[i].L, descs[i].B,
+                                     (long long)descs[i].nlines, (long long)descs[i].nrows, (long long)descs[i].ntiles,
+                                     descs[i].flags);
+                    std::fprintf(o, "--\n");
+                    std::fclose(o);
+                }
             }
         };
     }

@@ -205,6 +205,58 @@ def test_uneven_batch():
         assert rel_l2(y[off[i]:off[i + 1]], ref) <= tol_f64(n), f"row {i} length {n}"
 
 
+# primes whose Bluestein pad is a power of two from 2^17 to 2^22 (four-step
+# cores of power-of-two factors: the extended fast kernels)
+BLUESTEIN_P2 = [33013, 70003, 140009, 300007, 600011, 1100009]
+
+
+@pytest.mark.parametrize("n", BLUESTEIN_P2)
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_bluestein_power_of_two_pad(n, prec):
+    """Two rows, both layouts and signs, against numpy's FP64 FFT (which the
+    oracle's Bluestein restatement matches on the smaller primes above)."""
+    desc = planned.describe(n, -1, _abi.C2C, _abi.F64 if prec == "f64" else _abi.F32)
+    m = 1 << (2 * n - 2).bit_length()
+    assert f"bluestein{n}(fourstep{m}(" in desc, desc
+    x = synthetic.complex_normal(17, (2, n))
+    ct = torch.complex128 if prec == "f64" else torch.complex64
+    rt = torch.float64 if prec == "f64" else torch.float32
+    tol = 1e-11 if prec == "f64" else 1e-6
+    for sign, layout in ((-1, "interleaved"), (1, "planar")):
+        ref = np.fft.fft(x, axis=1) if sign < 0 else np.fft.ifft(x, axis=1) * n
+        t = planned.transform(Sign(sign), n)
+        if layout == "planar":
+            y = t((to_dev(x.real, rt), to_dev(x.imag, rt)))
+            got = host(y[0]) + 1j * host(y[1])
+        else:
+            got = host(t(to_dev(x, ct)))
+        for r in range(2):
+            assert rel_l2(got[r], ref[r]) <= tol, (sign, layout, r, rel_l2(got[r], ref[r]))
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_uneven_batch_power_of_two_groups(prec):
+    """Rows whose passes run as grouped extended fast launches (Bluestein with
+    power-of-two pads, a 65536 four-step) between generic ones, each row
+    against numpy's FP64 FFT."""
+    lengths = [100003, 600011, 70003, 65536, 0, 300007, 1024, 97, 100003, 1100009]
+    off = np.concatenate([[0], np.cumsum(lengths)])
+    x = synthetic.complex_normal(19, (int(off[-1]),))
+    from accelerate_fourier.batch import UnevenBatch
+    ct = torch.complex128 if prec == "f64" else torch.complex64
+    b = UnevenBatch(lengths, forward, prec)
+    xd = to_dev(x, ct)
+    y1 = host(b(xd))
+    y2 = host(b(xd))
+    assert np.array_equal(y1, y2)
+    tol = 1e-11 if prec == "f64" else 1e-6
+    for i, n in enumerate(lengths):
+        if n == 0:
+            continue
+        ref = np.fft.fft(x[off[i]:off[i + 1]])
+        assert rel_l2(y1[off[i]:off[i + 1]], ref) <= tol, f"row {i} length {n}"
+
+
 @pytest.mark.parametrize("n", [86, 58, 40000, 2 * 65521])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_real_spectrum_packed(n, prec):

@@ -22,7 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 CSRC = os.path.join(REPO, "accelerate-fourier_amd", "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["fast_pass.hip", "fft_kernels.hip", "karatsuba.hip"]
+SOURCES = ["fast_pass.hip", "fast_ext.hip", "fft_kernels.hip", "karatsuba.hip"]
 
 pytestmark = pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc is needed to compile the kernels")
 
