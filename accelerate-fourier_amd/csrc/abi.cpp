@@ -472,7 +472,7 @@ struct af_plan_s {
     Owner own;
     int32_t *in_map = nullptr, *out_map = nullptr;  // Rader gathers / scatters
     void *chirp = nullptr;                          // Bluestein chirp, table format, length n
-    void *spec = nullptr;                           // convolution spectrum, table format
+    void *spec = nullptr;                           // convolution spectrum, plain complex of the precision
     void *aux_tw = nullptr;                         // real packing twiddles W_n^k, k < n/2
     af_plan dual = nullptr;                         // the opposite sign (adjoint of Rader / Bluestein)
     af_plan sub = nullptr;                          // convolveCyclic of non-smooth lengths, or the half-length
@@ -604,10 +604,10 @@ af_status build_rader(af_plan P, hipStream_t s) {
     af_status st = AF_SUCCESS;
     { const af_status h__ = HIPST(gen_rader_kernel_d(kb, P->out_map, p, L, P->desc.sign, s)); if (st == AF_SUCCESS) st = h__; }
     if (st == AF_SUCCESS) st = fft_f64_forward(L, kb, kf, s);
-    if (st == AF_SUCCESS) st = P->own.alloc(&P->spec, L * 16);
+    if (st == AF_SUCCESS) st = P->own.alloc(&P->spec, L * elem_bytes(P->desc.precision));
     if (st == AF_SUCCESS) {
         const bool four = P->core.sh.kind == CORE_FOUR;
-        st = HIPST(convert_to_w(P->prec, P->spec, kf, L, 1.0 / (double)L, four ? P->core.sh.N1 : 0,
+        st = HIPST(convert_to_plain(P->prec, P->spec, kf, L, 1.0 / (double)L, four ? P->core.sh.N1 : 0,
                                 four ? P->core.sh.N2 : 0, s));
     }
     { const af_status h__ = HIPST(hipStreamSynchronize(s)); if (st == AF_SUCCESS) st = h__; }
@@ -629,10 +629,10 @@ af_status build_bluestein(af_plan P, hipStream_t s) {
     if (st == AF_SUCCESS) st = HIPST(convert_to_w(P->prec, P->chirp, cd, n, 1.0, 0, 0, s));
     if (st == AF_SUCCESS) st = HIPST(gen_chirp_d(hb, n, M, P->desc.sign, M, 1, s));
     if (st == AF_SUCCESS) st = fft_f64_forward(M, hb, hf, s);
-    if (st == AF_SUCCESS) st = P->own.alloc(&P->spec, M * 16);
+    if (st == AF_SUCCESS) st = P->own.alloc(&P->spec, M * elem_bytes(P->desc.precision));
     if (st == AF_SUCCESS) {
         const bool four = P->core.sh.kind == CORE_FOUR;
-        st = HIPST(convert_to_w(P->prec, P->spec, hf, M, 1.0 / (double)M, four ? P->core.sh.N1 : 0,
+        st = HIPST(convert_to_plain(P->prec, P->spec, hf, M, 1.0 / (double)M, four ? P->core.sh.N1 : 0,
                                 four ? P->core.sh.N2 : 0, s));
     }
     { const af_status h__ = HIPST(hipStreamSynchronize(s)); if (st == AF_SUCCESS) st = h__; }
@@ -809,7 +809,7 @@ af_status build_c2c(af_plan P, int sign, int64_t batch, const Endpoint &in, cons
         const int64_t L = n - 1;
         const bool four = P->core.sh.kind == CORE_FOUR;
         const size_t first = ps.size();
-        conv_passes(prec, P->core, batch, P->plain, in, out, P->spec, 0, false, scratch, ps);
+        conv_passes(prec, P->core, batch, P->plain, in, out, P->spec, 0, true, scratch, ps);
         AfPass &a = ps[first];
         a.in_map = P->in_map;
         a.in_len = L;
@@ -836,7 +836,7 @@ af_status build_c2c(af_plan P, int sign, int64_t batch, const Endpoint &in, cons
     }
     case ALG_BLUESTEIN: {
         const size_t first = ps.size();
-        conv_passes(prec, P->core, batch, P->plain, in, out, P->spec, 0, false, scratch, ps);
+        conv_passes(prec, P->core, batch, P->plain, in, out, P->spec, 0, true, scratch, ps);
         ps[first].in_len = n;
         ps[first].in_mul = P->chirp;
         ps.back().out_len = n;

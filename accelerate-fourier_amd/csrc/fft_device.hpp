@@ -84,6 +84,8 @@ __device__ __forceinline__ void to_w(double2 v, float4 &w) {
     w = make_float4(hx, hy, (float)(v.x - (double)hx), (float)(v.y - (double)hy));
 }
 __device__ __forceinline__ void to_w(double2 v, double2 &w) { w = v; }
+// plain complex value of the precision (one rounding)
+__device__ __forceinline__ void to_w(double2 v, float2 &w) { w = make_float2((float)v.x, (float)v.y); }
 __device__ __forceinline__ double2 from_w(float4 w) {
     return make_double2((double)w.x + (double)w.z, (double)w.y + (double)w.w);
 }

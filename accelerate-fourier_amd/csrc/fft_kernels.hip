@@ -461,6 +461,18 @@ hipError_t convert_to_w(int prec, void *out, const void *in, int64_t count, doub
     return hipGetLastError();
 }
 
+hipError_t convert_to_plain(int prec, void *out, const void *in, int64_t count, double scale, int64_t N1, int64_t N2,
+                            hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    if (prec == PREC_F64)
+        hipLaunchKernelGGL(k_convert<double2>, dim3(grid_for(count, 256)), dim3(256), 0, s, (double2 *)out,
+                           (const double2 *)in, count, scale, N1, N2);
+    else
+        hipLaunchKernelGGL(k_convert<float2>, dim3(grid_for(count, 256)), dim3(256), 0, s, (float2 *)out,
+                           (const double2 *)in, count, scale, N1, N2);
+    return hipGetLastError();
+}
+
 template <typename T>
 static hipError_t launch_pack_t(int op, const void *a, const void *a_im, const void *b, void *o_re, void *o_im,
                                 const void *aux, int64_t L, int64_t rows, int plain, int arg, hipStream_t s) {

@@ -23,6 +23,9 @@ hipError_t gen_chirp_d(void *out, int64_t n, int64_t count, int sign, int64_t M,
 hipError_t gen_rader_kernel_d(void *out, const int32_t *powg, int64_t p, int64_t count, int sign, hipStream_t s);
 hipError_t convert_to_w(int prec, void *out, const void *in, int64_t count, double scale, int64_t N1, int64_t N2,
                         hipStream_t s);
+// the same as plain complex values of the precision (convolution spectra)
+hipError_t convert_to_plain(int prec, void *out, const void *in, int64_t count, double scale, int64_t N1, int64_t N2,
+                            hipStream_t s);
 
 // Real packing in global memory (fft_kernels.hip)
 enum { PACK_UNTANGLE = 0, PACK_ENTANGLE = 1, PACK_ENTANGLE_ADJ = 2, PACK_PRODUCT = 3 };

@@ -301,6 +301,21 @@ void check_ext_desc(const AfPass &d, int tsz, int L, int B, int sg, bool conv, d
     check_pass(d, tsz, k);
 }
 
+// Optional record of the descriptors of grouped launches (AF_DRYRUN_DUMP=file):
+// one line per descriptor "threads lds L B nlines nrows ntiles flags", "--"
+// after each launch.
+void dump_group(const AfPass *descs, int nd, dim3 block, size_t shmem) {
+    const char *f = std::getenv("AF_DRYRUN_DUMP");
+    if (!f) return;
+    FILE *o = std::fopen(f, "a");
+    if (!o) return;
+    for (int i = 0; i < nd; ++i)
+        std::fprintf(o, "%u %zu %d %d %lld %lld %lld %d\n", block.x, shmem, descs[i].L, descs[i].B,
+                     (long long)descs[i].nlines, (long long)descs[i].nrows, (long long)descs[i].ntiles, descs[i].flags);
+    std::fprintf(o, "--\n");
+    std::fclose(o);
+}
+
 int tsize_of(const std::string &n) { return n.find("<double") != std::string::npos ? 8 : 4; }
 
 // template arguments of k_fast<T, L, B, SG>
@@ -351,6 +366,7 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
                 if (prefix[i + 1] - prefix[i] != descs[i].ntiles) problem(name + ": tile prefix differs from ntiles");
                 check_ext_desc(descs[i], tsz, L, B, sg, conv, block, shmem, name + " desc " + std::to_string(i));
             }
+            dump_group(descs, nd, block, shmem);
         };
     }
     if (name.find("af::k_pass_grouped<") == 0) {
@@ -364,18 +380,7 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
                 if (prefix[i + 1] - prefix[i] != descs[i].ntiles) problem(name + ": tile prefix differs from ntiles");
                 check_generic_launch(descs[i], tsz, block.x, shmem, name + " desc " + std::to_string(i));
             }
-            // optional record of the descriptors (AF_DRYRUN_DUMP=file): one line per
-            // descriptor "threads lds L B nlines nrows ntiles flags", "--" after each launch
-            if (const char *f = std::getenv("AF_DRYRUN_DUMP")) {
-                if (FILE *o = std::fopen(f, "a")) {
-                    for (int i = 0; i < nd; ++i)
-                        std::fprintf(o, "%u %zu %d %d %lld %lld %lld %d\n", block.x, shmem, descs[i].L, descs[i].B,
-                                     (long long)descs[i].nlines, (long long)descs[i].nrows, (long long)descs[i].ntiles,
-                                     descs[i].flags);
-                    std::fprintf(o, "--\n");
-                    std::fclose(o);
-                }
-            }
+            dump_group(descs, nd, block, shmem);
         };
     }
     if (name.find("af::k_fast_rconv<") == 0) {
@@ -459,8 +464,10 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
         const void *in = *(const void *const *)args[1];
         const int64_t count = *(const int64_t *)args[2];
         const int64_t N1 = *(const int64_t *)args[4], N2 = *(const int64_t *)args[5];
+        // output element: float2 (plain FP32 spectra) 8 bytes, float4 / double2 16
+        const int64_t osz = name.find("HIP_vector_type<float, 2u>") != std::string::npos ? 8 : 16;
         return [=] {
-            in_alloc(out, 0, count * 16, "out", name);
+            in_alloc(out, 0, count * osz, "out", name);
             in_alloc(in, 0, count * 16, "in", name);
             if (N1 > 0 && N1 * N2 != count) problem(name + ": N1*N2 differs from count");
         };
