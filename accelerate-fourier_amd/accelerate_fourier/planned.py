@@ -95,6 +95,19 @@ class Transform:
                    "af_execute")
         return out
 
+    def tune(self, batch, precision="f32", layout="planar"):
+        """Autotune this transform's launches for `batch` rows in the given
+        layout ("planar" or "interleaved"); later calls with that batch and
+        layout use the fastest tile shapes.  Returns the tuner's report."""
+        prec = _abi.F64 if precision in ("f64", _abi.F64) else _abi.F32
+        if layout not in ("planar", "interleaved"):
+            raise ValueError("layout must be 'planar' or 'interleaved'")
+        p = self.plan(prec)
+        buf = ctypes.create_string_buffer(8192)
+        _abi.check(_abi.lib().af_plan_tune(p.handle, int(batch), 1 if layout == "planar" else 0,
+                                           _arrays.stream_handle(), buf, len(buf)), "af_plan_tune")
+        return buf.value.decode()
+
     def vjp(self, g, out=None):
         """Adjoint (conjugate transpose) applied to the cotangent g."""
         g = _arrays.materialize(g)
@@ -109,6 +122,20 @@ class Transform:
         _abi.check(_abi.lib().af_vjp_execute(p.handle, _arrays.batch_of(shape), re, im, ore, oim, None,
                                              _arrays.stream_handle()), "af_vjp_execute")
         return out
+
+
+def wisdom_export() -> str:
+    """The autotuner's choices in this process, as text."""
+    need = ctypes.c_size_t(0)
+    _abi.check(_abi.lib().af_wisdom_export(None, 0, ctypes.byref(need)), "af_wisdom_export")
+    buf = ctypes.create_string_buffer(max(need.value, 1))
+    _abi.check(_abi.lib().af_wisdom_export(buf, len(buf), ctypes.byref(need)), "af_wisdom_export")
+    return buf.value.decode()
+
+
+def wisdom_import(text: str) -> None:
+    """Adds exported autotuner choices to this process's wisdom."""
+    _abi.check(_abi.lib().af_wisdom_import(text.encode()), "af_wisdom_import")
 
 
 def transform(sign: Sign, n: int) -> Transform:

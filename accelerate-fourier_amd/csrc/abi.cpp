@@ -221,6 +221,21 @@ struct Endpoint {
     int64_t bs = 0;            // row stride in elements
 };
 
+// The tile geometry of a pass with B lines per tile: LDS pitch, tiles,
+// threads of the generic kernel (ceil(B L / EPT), whole wavefronts, at most
+// 1024) and its dynamic LDS.
+void set_tile(AfPass &d, int prec, int B) {
+    d.B = B;
+    d.ld = d.L + (B > 1 ? 1 : 0);
+    d.ntiles = (d.nrows * d.nlines + B - 1) / B;
+    const int ept = pass_ept(prec == PREC_F64);
+    int nt = (int)((((int64_t)B * d.L + ept - 1) / ept + 63) / 64 * 64);
+    if (nt < 64) nt = 64;
+    if (nt > 1024) nt = 1024;
+    d.nthreads = nt;
+    d.lds_bytes = lds_bytes_for(prec, d.L, B, d.padshift);
+}
+
 AfPass base_pass(int prec, int L, const std::vector<int> &radices, const void *tw, const void *tw_d, int B,
                  int64_t nlines, int64_t nrows, int sign, bool plain) {
     AfPass d;
@@ -233,18 +248,10 @@ AfPass base_pass(int prec, int L, const std::vector<int> &radices, const void *t
         d.pst[i] = p;
         p *= radices[i];
     }
-    d.B = B;
-    d.ld = L + (B > 1 ? 1 : 0);
     d.padshift = 4;
     d.nlines = nlines;
     d.nrows = nrows;
-    d.ntiles = (nrows * nlines + B - 1) / B;
-    const int ept = pass_ept(prec == PREC_F64);
-    int nt = (int)((((int64_t)B * L + ept - 1) / ept + 63) / 64 * 64);
-    if (nt < 64) nt = 64;
-    if (nt > 1024) nt = 1024;
-    d.nthreads = nt;
-    d.lds_bytes = lds_bytes_for(prec, L, B, d.padshift);
+    set_tile(d, prec, B);
     d.tw = tw;
     d.tw_d = tw_d;
     d.sign = sign;
@@ -460,6 +467,49 @@ int prec_of(const af_plan_desc &d) { return d.precision == AF_F64 ? PREC_F64 : P
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Autotuning records
+// ---------------------------------------------------------------------------
+
+// One pass's tuned choice: B lines per tile (0 = the planner's) and the order of
+// its radices (0 = as planned, 1 = reversed, 2 = odd radices first).
+struct PassTune {
+    int B = 0;
+    int order = 0;
+};
+
+// Radix order of a generic-kernel pass (the Stockham stages accept any order;
+// the stage twiddle table W_L^m does not depend on it).
+void reorder_radices(AfPass &d, int order) {
+    if (order == 0 || d.nst < 2) return;
+    std::vector<int> r(d.radix, d.radix + d.nst);
+    if (order == 1) std::reverse(r.begin(), r.end());
+    else std::stable_partition(r.begin(), r.end(), [](int x) { return x % 2 == 1; });
+    int p = 1;
+    for (int i = 0; i < d.nst; ++i) {
+        d.radix[i] = r[i];
+        d.pst[i] = p;
+        p *= r[i];
+    }
+}
+
+// A pass some kernel can run with its current tile geometry.
+bool tile_launchable(int prec, const AfPass &d) {
+    if (d.B < 1 || d.B > 64) return false;
+    if (fast_ext_eligible(prec, d)) return true;
+    if (fast_pass_eligible(prec, d) && fast_tile_lines(prec, d.L, d.B) == d.B) return true;
+    return d.nthreads <= 1024 && (int64_t)d.B * d.L <= 1024 * (int64_t)pass_ept(prec == PREC_F64) &&
+           d.lds_bytes <= lds_tile_bytes_cap() && (int64_t)d.B * d.L < ((int64_t)1 << 22);
+}
+
+std::mutex g_wisdom_mu;
+std::map<std::string, std::vector<PassTune>> g_wisdom;
+
+std::string wisdom_key(const af_plan_desc &d, int64_t batch, int planar) {
+    return std::string(d.precision == AF_F64 ? "f64" : "f32") + " " + std::to_string(d.algorithm) + " " +
+           std::to_string(d.length) + " " + std::to_string(batch) + " " + std::to_string(planar);
+}
+
+// ---------------------------------------------------------------------------
 // Plans
 // ---------------------------------------------------------------------------
 
@@ -478,6 +528,9 @@ struct af_plan_s {
     af_plan sub = nullptr;                          // convolveCyclic of non-smooth lengths, or the half-length
                                                     // transform of unfused real kinds: C2C
     bool real_fused = false;
+    // autotuned tile lines and radix orders of the C2C passes, per (batch, planar)
+    std::map<std::pair<int64_t, int>, std::vector<PassTune>> tuned;
+    std::mutex tune_mu;
     ~af_plan_s() {
         if (dual) af_plan_destroy(dual);
         if (sub) af_plan_destroy(sub);
@@ -784,8 +837,50 @@ Endpoint ep(const void *re, const void *im, int64_t bs) {
 // Passes of a complex transform with an explicit sign (the plan's own, or the
 // opposite for the adjoint of Stockham plans).  `scratch` holds
 // scratch_per_row() bytes per row.
+// The tuned choices recorded for (batch, planar) -- the plan's own, else the
+// process's wisdom -- applied to the passes where a kernel can run them.
+void apply_tuning(af_plan P, int64_t batch, int planar, std::vector<AfPass> &ps) {
+    std::vector<PassTune> t;
+    {
+        std::lock_guard<std::mutex> lk(P->tune_mu);
+        auto it = P->tuned.find({batch, planar});
+        if (it != P->tuned.end()) t = it->second;
+    }
+    if (t.empty()) {
+        std::lock_guard<std::mutex> lk(g_wisdom_mu);
+        auto it = g_wisdom.find(wisdom_key(P->desc, batch, planar));
+        if (it == g_wisdom.end()) return;
+        t = it->second;
+    }
+    if (t.size() != ps.size()) return;
+    for (size_t i = 0; i < ps.size(); ++i) {
+        AfPass d = ps[i];
+        reorder_radices(d, t[i].order);
+        if (t[i].B) set_tile(d, P->prec, t[i].B);
+        if (tile_launchable(P->prec, d)) ps[i] = d;
+    }
+}
+
+af_status build_c2c_passes(af_plan P, int sign, int64_t batch, const Endpoint &in, const Endpoint &out,
+                           void *scratch, std::vector<AfPass> &ps);
+
+// Passes of a complex transform, with the autotuned tile shapes when `tuned`
+// (the uneven batch builds untuned passes: its launch groups are fixed when it
+// is created).
 af_status build_c2c(af_plan P, int sign, int64_t batch, const Endpoint &in, const Endpoint &out, void *scratch,
-                    std::vector<AfPass> &ps) {
+                    std::vector<AfPass> &ps, bool tuned = true) {
+    const size_t first = ps.size();
+    AFCHK(build_c2c_passes(P, sign, batch, in, out, scratch, ps));
+    if (tuned && ps.size() > first) {
+        std::vector<AfPass> mine(ps.begin() + first, ps.end());
+        apply_tuning(P, batch, in.im != nullptr, mine);
+        std::copy(mine.begin(), mine.end(), ps.begin() + first);
+    }
+    return AF_SUCCESS;
+}
+
+af_status build_c2c_passes(af_plan P, int sign, int64_t batch, const Endpoint &in, const Endpoint &out,
+                           void *scratch, std::vector<AfPass> &ps) {
     if (sign != P->desc.sign && (P->shape.algo == ALG_RADER || P->shape.algo == ALG_BLUESTEIN)) {
         // Rader kernels and chirps depend on the sign: the opposite sign is the dual plan's
         if (!P->dual) return AF_INTERNAL;
@@ -945,6 +1040,153 @@ af_status af_execute_pass(af_plan plan, int32_t index, int64_t batch, const void
     AFCHK(build_c2c(plan, plan->desc.sign, batch, ep(in_re, in_im, n), ep(out_re, out_im, n), workspace, ps));
     if (index >= (int32_t)ps.size()) return AF_INVALID_ARG;
     HIPCHK(launch_pass(plan->prec, ps[index], (hipStream_t)stream));
+    return AF_SUCCESS;
+}
+
+// ---- autotuner ----
+
+af_status af_plan_tune(af_plan plan, int64_t batch, int32_t planar, void *stream, char *report, size_t report_len) {
+    if (!plan || plan->desc.kind != AF_C2C || batch <= 0 || (planar != 0 && planar != 1)) return AF_INVALID_ARG;
+    if (report && report_len) report[0] = 0;
+    const int64_t n = plan->desc.length;
+    if (n <= 1) return AF_SUCCESS;
+    hipStream_t s = (hipStream_t)stream;
+    const int prec = plan->prec;
+    const size_t eb = (size_t)elem_bytes(plan->desc.precision);
+    const size_t bytes = (size_t)batch * (size_t)n * eb;
+    // zero-filled buffers: the transform's timing does not depend on the values
+    void *in = nullptr, *out = nullptr, *ws = nullptr;
+    const size_t wsb = plan->scratch_per_row() * (size_t)batch;
+    HIPCHK(hipMalloc(&in, bytes));
+    af_status st = HIPST(hipMalloc(&out, bytes));
+    if (st == AF_SUCCESS && wsb) st = HIPST(hipMalloc(&ws, wsb));
+    if (st == AF_SUCCESS) st = HIPST(hipMemsetAsync(in, 0, bytes, s));
+    if (st == AF_SUCCESS) st = HIPST(hipMemsetAsync(out, 0, bytes, s));
+    if (st == AF_SUCCESS && wsb) st = HIPST(hipMemsetAsync(ws, 0, wsb, s));
+    Endpoint ei, eo;
+    if (planar) {
+        ei = ep(in, (char *)in + bytes / 2, n);
+        eo = ep(out, (char *)out + bytes / 2, n);
+    } else {
+        ei = ep(in, nullptr, n);
+        eo = ep(out, nullptr, n);
+    }
+    std::vector<AfPass> ps;
+    if (st == AF_SUCCESS) st = build_c2c(plan, plan->desc.sign, batch, ei, eo, ws, ps, false);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (st == AF_SUCCESS) st = HIPST(hipEventCreate(&e0));
+    if (st == AF_SUCCESS) st = HIPST(hipEventCreate(&e1));
+    // average microseconds per launch of d: two warm-ups, then the best of three runs of five
+    auto time_pass = [&](const AfPass &d, double &us) -> af_status {
+        for (int w = 0; w < 2; ++w) HIPCHK(launch_pass(prec, d, s));
+        us = 1e30;
+        for (int rep = 0; rep < 3; ++rep) {
+            HIPCHK(hipEventRecord(e0, s));
+            for (int k = 0; k < 5; ++k) HIPCHK(launch_pass(prec, d, s));
+            HIPCHK(hipEventRecord(e1, s));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            us = std::min(us, (double)ms * 1e3 / 5.0);
+        }
+        return AF_SUCCESS;
+    };
+    std::vector<PassTune> best(ps.size());
+    std::string text;
+    for (size_t i = 0; i < ps.size() && st == AF_SUCCESS; ++i) {
+        const AfPass base = ps[i];
+        double t0 = 0;
+        st = time_pass(base, t0);
+        double tb = t0;
+        // generic-kernel passes also try other radix orders
+        const bool generic = !fast_ext_eligible(prec, base) &&
+                             !(fast_pass_eligible(prec, base) && fast_tile_lines(prec, base.L, base.B) == base.B);
+        const int norders = generic && base.nst > 1 ? 3 : 1;
+        for (int B = 1; B <= 64 && st == AF_SUCCESS; B *= 2)
+            for (int o = 0; o < norders && st == AF_SUCCESS; ++o) {
+                if (B == base.B && o == 0) continue;
+                AfPass d = base;
+                reorder_radices(d, o);
+                set_tile(d, prec, B);
+                if (!tile_launchable(prec, d)) continue;
+                double t = 0;
+                st = time_pass(d, t);
+                if (st == AF_SUCCESS && t < tb * 0.98) {  // a clear win only
+                    tb = t;
+                    best[i].B = B;
+                    best[i].order = o;
+                }
+            }
+        char line[160];
+        std::snprintf(line, sizeof line, "pass %zu L=%d: B=%d order=%d %.2f us (planned B=%d %.2f us)\n", i, base.L,
+                      best[i].B ? best[i].B : base.B, best[i].order, tb, base.B, t0);
+        text += line;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    { const af_status h__ = HIPST(hipStreamSynchronize(s)); if (st == AF_SUCCESS) st = h__; }
+    if (ws) (void)hipFree(ws);
+    (void)hipFree(out);
+    (void)hipFree(in);
+    if (st != AF_SUCCESS) return st;
+    {
+        std::lock_guard<std::mutex> lk(plan->tune_mu);
+        plan->tuned[{batch, planar}] = best;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_wisdom_mu);
+        g_wisdom[wisdom_key(plan->desc, batch, planar)] = best;
+    }
+    if (report && report_len) std::snprintf(report, report_len, "%s", text.c_str());
+    return AF_SUCCESS;
+}
+
+af_status af_wisdom_export(char *buf, size_t buflen, size_t *needed) {
+    std::string text;
+    {
+        std::lock_guard<std::mutex> lk(g_wisdom_mu);
+        for (const auto &kv : g_wisdom) {
+            text += kv.first + " :";
+            for (const PassTune &t : kv.second) text += " " + std::to_string(t.B) + "/" + std::to_string(t.order);
+            text += "\n";
+        }
+    }
+    if (needed) *needed = text.size() + 1;
+    if (!buf) return buflen ? AF_INVALID_ARG : AF_SUCCESS;
+    if (buflen < text.size() + 1) return AF_INVALID_ARG;
+    std::memcpy(buf, text.c_str(), text.size() + 1);
+    return AF_SUCCESS;
+}
+
+af_status af_wisdom_import(const char *text) {
+    if (!text) return AF_INVALID_ARG;
+    std::map<std::string, std::vector<PassTune>> add;
+    const char *p = text;
+    while (*p) {
+        const char *e = std::strchr(p, '\n');
+        const std::string line(p, e ? (size_t)(e - p) : std::strlen(p));
+        p = e ? e + 1 : p + line.size();
+        if (line.empty()) continue;
+        const size_t c = line.find(" :");
+        if (c == std::string::npos) return AF_INVALID_ARG;
+        char pr[8];
+        long long alg, len, batch, planar;
+        if (std::sscanf(line.c_str(), "%7s %lld %lld %lld %lld", pr, &alg, &len, &batch, &planar) != 5)
+            return AF_INVALID_ARG;
+        if ((std::strcmp(pr, "f32") && std::strcmp(pr, "f64")) || len < 0 || batch <= 0 || planar < 0 || planar > 1)
+            return AF_INVALID_ARG;
+        std::vector<PassTune> v;
+        const char *q = line.c_str() + c + 2;
+        int B, o, used;
+        while (std::sscanf(q, " %d/%d%n", &B, &o, &used) == 2) {
+            if (B < 0 || B > 64 || (B & (B - 1)) || o < 0 || o > 2) return AF_INVALID_ARG;
+            v.push_back({B, o});
+            q += used;
+        }
+        add[line.substr(0, c)] = v;
+    }
+    std::lock_guard<std::mutex> lk(g_wisdom_mu);
+    for (auto &kv : add) g_wisdom[kv.first] = kv.second;
     return AF_SUCCESS;
 }
 
@@ -1299,7 +1541,8 @@ af_status batch_passes(af_batch B, int64_t i, const void *in_re, const void *in_
     af_plan P = B->plans.at(n);
     const size_t rb = B->precision == AF_F64 ? 8 : 4;
     return build_c2c(P, B->sign, 1, offset_ep(in_re, in_im, B->offsets[i], n, rb),
-                     offset_ep(out_re, out_im, B->offsets[i], n, rb), (char *)B->scratch + B->scratch_off[i], ps);
+                     offset_ep(out_re, out_im, B->offsets[i], n, rb), (char *)B->scratch + B->scratch_off[i], ps,
+                     false);
 }
 
 }  // namespace

@@ -154,17 +154,26 @@ k_fast_rconv(const AfPass d, const T *__restrict__ in0, const T *__restrict__ in
         for (int i = threadIdx.x; i < L; i += NT) stab[i] = ((const W *)d.tw)[i];
     const W *aux = (const W *)d.aux_tw;
     const int64_t row0 = (int64_t)blockIdx.x * B;
-    C v[16];
+    // FP32: every input row is requested before the first transform -- occupancy
+    // is bound by LDS (two line buffers), so the registers are free, and the HBM
+    // latency is paid once instead of once per signal.  FP64 rows would take the
+    // whole register file; they are loaded one before each transform.
+    constexpr bool PRE = sizeof(T) == 4;
+    C v[16], u[16], t[16];
     fast_rows_load<T, L, B>(in0, row0, v);
+    if (PRE) {
+        fast_rows_load<T, L, B>(in1, row0, u);
+        if (VJP) fast_rows_load<T, L, B>(in2, row0, t);
+    }
     fast_fft_lds<T, L, B, -1>(d, v, A, stab);
-    fast_rows_load<T, L, B>(in1, row0, v);
-    fast_fft_lds<T, L, B, -1>(d, v, Bb, stab);
+    if (!PRE) fast_rows_load<T, L, B>(in1, row0, u);
+    fast_fft_lds<T, L, B, -1>(d, u, Bb, stab);
     fast_spec_product<T, L, B>(A, Bb, Bb, aux, VJP);
     lds_barrier();
     fast_inverse_store<T, L, B>(d, Bb, stab, out0, row0);
     if (VJP) {
-        fast_rows_load<T, L, B>(in2, row0, v);
-        fast_fft_lds<T, L, B, -1>(d, v, Bb, stab);
+        if (!PRE) fast_rows_load<T, L, B>(in2, row0, t);
+        fast_fft_lds<T, L, B, -1>(d, t, Bb, stab);
         fast_spec_product<T, L, B>(A, Bb, Bb, aux, true);
         lds_barrier();
         fast_inverse_store<T, L, B>(d, Bb, stab, out1, row0);

@@ -38,6 +38,9 @@ void init_karatsuba_attributes();
 void init_fast_attributes();
 // Power-of-two passes with compile-time stages (fast_pass.hip); done = false when not eligible.
 hipError_t launch_fast(int prec, const AfPass &d, hipStream_t s, bool &done);
+// the descriptor meets the power-of-two kernel's conditions (its (L, B) may
+// still lack an instantiation: see fast_tile_lines)
+bool fast_pass_eligible(int prec, const AfPass &d);
 // the largest line count <= B with a power-of-two kernel for lines of length L
 // (B itself when L has none)
 int fast_tile_lines(int prec, int64_t L, int B);

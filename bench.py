@@ -18,6 +18,7 @@ import argparse
 import ctypes
 import json
 import math
+import re
 import os
 import sys
 import time
@@ -41,6 +42,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--skip-extras", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="run the headline transform with the planner's tile shapes instead of autotuning "
+                         "them first (af_plan_tune)")
     ap.add_argument("--profile", action="store_true",
                     help="only the headline transform (steps launches) and a 32 MiB device copy of "
                          "known size, for rocprofv3 kernel traces and counters")
@@ -86,6 +90,14 @@ class C2C:
         st = self.abi.lib().af_execute(self.plan.handle, self.batch, a[0], a[1], a[2], a[3], self.ws.data_ptr(), s)
         self.abi.check(st, "af_execute")
 
+    def tune(self):
+        """af_plan_tune for this batch and layout; returns the tuner's report."""
+        buf = ctypes.create_string_buffer(8192)
+        s = self.torch.cuda.current_stream().cuda_stream
+        self.abi.check(self.abi.lib().af_plan_tune(self.plan.handle, self.batch, 1 if self.planar else 0, s, buf,
+                                                   len(buf)), "af_plan_tune")
+        return buf.value.decode()
+
     def run_pass(self, i):
         s = self.torch.cuda.current_stream().cuda_stream
         a = self.ptrs
@@ -98,6 +110,12 @@ class C2C:
         if self.planar:
             return self.xout[0].cpu().numpy() + 1j * self.xout[1].cpu().numpy()
         return self.xout.cpu().numpy()
+
+
+def tile_lines(report, i, default):
+    """Lines per tile the autotuner chose for launch i (its report line), else default."""
+    m = re.search(rf"^pass {i} L=\d+: B=(\d+)", report, re.M)
+    return int(m.group(1)) if m else default
 
 
 def time_it(fn, steps, warmup):
@@ -213,6 +231,7 @@ def main():
     x = synthetic.complex_normal(1, (batch, n))
     f32 = C2C(n, batch, _abi.F32)
     f32.set_io(x)
+    tune_report = "" if args.no_tune else f32.tune()
     if args.profile:
         for _ in range(warmup + steps):
             f32.run()
@@ -251,9 +270,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom),
                      "algorithmic_bytes": alg_bytes,
-                     "kernel": "k_fast<float, 256, 16, -1> " + PASS_NAMES.get(dom, f"pass {dom}"),
+                     "kernel": f"k_fast<float, 256, {tile_lines(tune_report, dom, 16)}, -1> "
+                               + PASS_NAMES.get(dom, f"pass {dom}"),
                      "pass_ms": pass_ms},
     }
+    if tune_report:
+        out["config"]["autotune"] = [l for l in tune_report.strip().split("\n")]
     y32 = f32.result()
 
     if not args.skip_extras:

@@ -126,6 +126,30 @@ af_status af_execute_pass(af_plan plan, int32_t index, int64_t batch,
                           void *out_re, void *out_im, void *workspace, void *stream);
 
 /*
+ * Autotuner.  No reference counterpart: the reference's plans have fixed
+ * shapes.  Times, for `batch` rows in the given layout (planar = 1: re/im
+ * arrays, 0: interleaved) on `stream`, every tile line count 1..64 that some
+ * kernel can run for each launch of the transform, and for launches on the
+ * mixed-radix kernel also three orders of their radices (as planned, reversed,
+ * odd radices first); keeps what is fastest by more than 2% in the plan, where
+ * af_execute, af_vjp_execute and af_execute_pass with the same batch and
+ * layout use it, and in the process's wisdom.  Allocates and frees its own
+ * zero-filled buffers and synchronizes the stream.  `report` (may be NULL)
+ * receives one line per launch: chosen and planned shape with their times.
+ */
+af_status af_plan_tune(af_plan plan, int64_t batch, int32_t planar, void *stream, char *report, size_t report_len);
+
+/*
+ * Wisdom: the autotuner's choices as text, one line per (precision,
+ * algorithm, length, batch, layout).  Export writes it to buf and the size it
+ * needs, terminator included, to *needed (buf NULL with buflen 0 only
+ * queries).  Import adds the lines to the process's wisdom, which plans
+ * without a tuning of their own for a (batch, layout) use.
+ */
+af_status af_wisdom_export(char *buf, size_t buflen, size_t *needed);
+af_status af_wisdom_import(const char *text);
+
+/*
  * Real signal to full spectrum (AF_R2C_FULL plans).
  * Replaces: Real.toSpectrum (Planned.transform sign (n/2)) (Real.hs:49-70).
  * x: batch x n reals; out: batch x n complex (the full, not halved, spectrum).

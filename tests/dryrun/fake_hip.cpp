@@ -553,6 +553,29 @@ hipError_t hipFree(void *p) {
     return hipSuccess;
 }
 hipError_t hipFreeAsync(void *p, hipStream_t) { return hipFree(p); }
+hipError_t hipMemsetAsync(void *dst, int v, size_t n, hipStream_t) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!in_alloc(dst, 0, n, "memset dst", "hipMemsetAsync")) return hipErrorInvalidValue;
+    std::memset(dst, v, n);
+    return hipSuccess;
+}
+hipError_t hipMemset(void *dst, int v, size_t n) { return hipMemsetAsync(dst, v, n, nullptr); }
+
+// ---- events: every interval reads as 1 ms (the autotuner then keeps the planned shapes) ----
+hipError_t hipEventCreate(hipEvent_t *e) {
+    static char token[64];
+    static int next = 0;
+    *e = (hipEvent_t)&token[next++ % 64];
+    return hipSuccess;
+}
+hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
+hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) {
+    *ms = 1.0f;
+    return hipSuccess;
+}
+hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 hipError_t hipMemcpy(void *dst, const void *src, size_t n, hipMemcpyKind kind) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (kind == hipMemcpyHostToDevice || kind == hipMemcpyDeviceToDevice) in_alloc(dst, 0, n, "memcpy dst", "hipMemcpy");

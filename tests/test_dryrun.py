@@ -216,3 +216,45 @@ def test_karatsuba(dry):
     assert dry.af_karatsuba_ragged_i64(len(lens), arr, b(tot * 8), b(tot * 8), b((2 * tot - len(lens)) * 8), None) == 0
     b.free()
     assert problems(dry, before) == []
+
+
+def test_autotune_candidates(dry):
+    """af_plan_tune launches every candidate tile shape and radix order of each
+    pass (all checked here); the stand-in runtime's events read 1 ms for every
+    interval, so the planned shapes stay.  Wisdom imported by hand then makes
+    af_execute launch other shapes, which are checked too, and shapes no kernel
+    can run are ignored."""
+    dry.af_plan_tune.argtypes = [V, ctypes.c_int64, ctypes.c_int32, V, ctypes.c_char_p, ctypes.c_size_t]
+    dry.af_wisdom_export.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    dry.af_wisdom_import.argtypes = [ctypes.c_char_p]
+    dry.af_plan_num_passes.argtypes = [V, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)]
+    before = dry.dry_problem_count()
+    launches0 = dry.dry_launches()
+    cases = [(65536, 1, 64, 1), (20000, 1, 3, 0), (1000, 0, 5, 1), (100003, 1, 2, 0), (1 << 24, 1, 1, 1),
+             (4099, 0, 2, 0), (360, 1, 7, 1)]
+    for n, prec, batch, planar in cases:
+        d = PlanDesc(n, -1, C2C, prec, 0, 0)
+        p = V()
+        assert dry.af_plan_create(ctypes.byref(d), ctypes.byref(p)) == 0
+        rep = ctypes.create_string_buffer(4096)
+        assert dry.af_plan_tune(p, batch, planar, None, rep, len(rep)) == 0, (n, prec)
+        cnt = ctypes.c_int32()
+        assert dry.af_plan_num_passes(p, batch, ctypes.byref(cnt)) == 0
+        assert rep.value.decode().count("pass ") == cnt.value
+        dry.af_plan_destroy(p)
+    assert dry.dry_launches() - launches0 > 200
+    need = ctypes.c_size_t()
+    assert dry.af_wisdom_export(None, 0, ctypes.byref(need)) == 0
+    buf = ctypes.create_string_buffer(need.value)
+    assert dry.af_wisdom_export(buf, len(buf), ctypes.byref(need)) == 0
+    text = buf.value.decode()
+    assert len(text.strip().split("\n")) == len(cases)
+    assert "f32 0 65536 64 1 : 0/0 0/0" in text
+    # other shapes by hand: 65536 planar batch 64 with 8- and 32-line tiles, and
+    # 20000 with reversed radix orders and a 64-line tile that no kernel can run
+    assert dry.af_wisdom_import(b"f32 0 65536 64 1 : 8/0 32/0\nf32 0 20000 3 0 : 2/1 64/2\n") == 0
+    assert dry.af_wisdom_import(b"f32 0 65536 64 1 : 3/0\n") != 0
+    assert dry.af_wisdom_import(b"garbage\n") != 0
+    for n, batch, planar in ((65536, 64, True), (20000, 3, False)):
+        assert run_c2c(dry, n, -1, 1, batch, planar) == 0
+    assert problems(dry, before) == []

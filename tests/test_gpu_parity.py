@@ -317,3 +317,32 @@ def test_three_pass_2p24(prec, layout, sign):
     tol = 1e-13 if prec == "f64" else 1e-6
     for r in range(rows):
         assert rel_l2(got[r], ref[r]) <= tol, (r, rel_l2(got[r], ref[r]))
+
+
+@pytest.mark.parametrize("n,batch,layout,prec", [(65536, 8, "planar", "f32"), (20000, 3, "interleaved", "f32"),
+                                                (100003, 2, "planar", "f32"), (1000, 16, "interleaved", "f64")])
+def test_autotuned_transform(n, batch, layout, prec):
+    """After af_plan_tune the transform (and its adjoint) with the tuned shapes
+    still matches numpy's FP64 FFT; the wisdom text round-trips."""
+    t = planned.transform(forward, n)
+    report = t.tune(batch, prec, layout)
+    assert report.count("pass ") >= 1, report
+    x = synthetic.complex_normal(23, (batch, n))
+    ct = torch.complex128 if prec == "f64" else torch.complex64
+    rt = torch.float64 if prec == "f64" else torch.float32
+    tol = 1e-11 if prec == "f64" else 1e-6
+    if layout == "planar":
+        y = t((to_dev(x.real, rt), to_dev(x.imag, rt)))
+        got = host(y[0]) + 1j * host(y[1])
+        g = t.vjp((to_dev(x.real, rt), to_dev(x.imag, rt)))
+        gv = host(g[0]) + 1j * host(g[1])
+    else:
+        got = host(t(to_dev(x, ct)))
+        gv = host(t.vjp(to_dev(x, ct)))
+    assert rel_l2(got, np.fft.fft(x, axis=1)) <= tol, report
+    assert rel_l2(gv, np.fft.ifft(x, axis=1) * n) <= tol, report
+    text = planned.wisdom_export()
+    key = f"{prec} 0 {n} {batch} {1 if layout == 'planar' else 0} :"
+    assert key in text
+    planned.wisdom_import(text)
+    assert planned.wisdom_export() == text
