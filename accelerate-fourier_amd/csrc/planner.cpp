@@ -229,7 +229,20 @@ bool plan_shape(int prec, int64_t n, int algorithm, PlanShape &out, std::string 
         const int64_t M2 = ceil_pow2(2 * n - 1);
         if (M2 >= ((int64_t)1 << 16) && M2 <= ((int64_t)1 << 22)) M = M2;
     }
-    CoreShape cb = core_shape(prec, M);
+    CoreShape cb;
+    if (algorithm == AF_ALG_AUTO && (M & (M - 1)) == 0 && M >= ((int64_t)1 << 16) && M <= ((int64_t)1 << 22)) {
+        // power-of-two convolution core: short columns (N1 = 256 or 512 points,
+        // several workgroups per CU) and rows of up to 4096 points, which the
+        // convolution's middle pass reads contiguously
+        cb.N = M;
+        cb.kind = CORE_FOUR;
+        cb.N1 = std::max<int64_t>(256, M / 4096);
+        cb.N2 = M / cb.N1;
+        stockham_radices(cb.N1, cb.r1);
+        stockham_radices(cb.N2, cb.r2);
+    } else {
+        cb = core_shape(prec, M);
+    }
     if (cb.kind == CORE_NONE) {
         why = "padded length does not factor within the LDS limit";
         return false;
