@@ -36,8 +36,15 @@ __device__ __forceinline__ void wave_sync() {
 // Whole-wavefront rotation by one lane (DPP wave_ror:1): lane l receives the
 // value of lane l - 1 (mod 64).
 __device__ __forceinline__ u64 rot1(u64 v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x13C, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), 0x13C, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)v, 0x13C, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(v >> 32), 0x13C, 0xF, 0xF, true);
+    return ((u64)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+// Whole-wavefront shift by one lane (DPP wave_shr:1): lane l receives the
+// value of lane l - 1, lane 0 receives 0.
+__device__ __forceinline__ u64 shr1(u64 v) {
+    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)v, 0x138, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(v >> 32), 0x138, 0xF, 0xF, true);
     return ((u64)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 __device__ __forceinline__ u64 readlane64(u64 v, int l) {
@@ -62,6 +69,9 @@ __device__ __forceinline__ void digits3(int t, int levels, int &base, int &freem
 
 constexpr int kKarThreads = 1024;
 constexpr int kKarWaves = kKarThreads / 64;
+// operand blocks per pair: P = 2^d s <= 4608 with s <= 64 gives d <= 7, so at
+// most 2^(7-3) = 16 blocks of Q = 8 s <= 512 elements
+constexpr int kKarMaxBlocks = 16;
 
 // LDS words (u64) of one pair: result 2P, subtree operands 2Q, leaf products
 // 3^B 2s, recombination buffer 3^(B-1) 4s, and 128 per wavefront.
@@ -95,25 +105,57 @@ k_karatsuba(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__restric
         for (int i = 0; i < T; ++i) nsub *= 3;
         u64 *acc = sm, *qx = acc + 2 * P, *qy = qx + Q, *R2 = qy + Q, *R1 = R2 + nleaf * 2 * s;
         for (int i = tid; i < 2 * P; i += NT) acc[i] = 0ull;
+        // element tid of each of the 2^T operand blocks of size Q, held in
+        // registers for the whole pair (Q <= 512 <= NT; 2^T <= kKarMaxBlocks)
+        u64 xb[kKarMaxBlocks], yb[kKarMaxBlocks];
+        u64 mx = 0, my = 0;  // largest magnitudes of the pair's coefficients
+#pragma unroll
+        for (int b = 0; b < kKarMaxBlocks; ++b) {
+            const int64_t idx = (int64_t)b * Q + tid;
+            const bool in = b < (1 << T) && tid < Q && idx < Lp;
+            xb[b] = in ? x[off + idx] : 0ull;
+            yb[b] = in ? y[off + idx] : 0ull;
+            mx = max(mx, (int64_t)xb[b] < 0 ? 0ull - xb[b] : xb[b]);
+            my = max(my, (int64_t)yb[b] < 0 ? 0ull - yb[b] : yb[b]);
+        }
+        // Leaf products in FP64 are exact when every leaf operand and every sum of
+        // s products is an integer below 2^53: leaf operands are sums of up to 2^d
+        // coefficients, so |X| < 2^(bits(mx) + d), |Y| < 2^(bits(my) + d), and the
+        // s-term sums stay below 2^(bits(mx) + bits(my) + 2d + bits(s)).
+        u64 *red = R1 + (B ? nleaf / 3 * 4 * s : 0);  // kKarWaves * 128 spare words
+        for (int o = 32; o >= 1; o >>= 1) {
+            mx = max(mx, (u64)__shfl_xor((unsigned long long)mx, o));
+            my = max(my, (u64)__shfl_xor((unsigned long long)my, o));
+        }
+        if (lane == 0) {
+            red[wid] = mx;
+            red[kKarWaves + wid] = my;
+        }
+        __syncthreads();
+        // every thread reduces the wavefront maxima (LDS broadcast reads), so the
+        // choice below is uniform over the workgroup
+        mx = my = 0;
+        for (int w = 0; w < kKarWaves; ++w) {
+            mx = max(mx, red[w]);
+            my = max(my, red[kKarWaves + w]);
+        }
+        const int bits = (64 - __clzll((long long)mx)) + (64 - __clzll((long long)my)) + 2 * d + (32 - __clz(s));
+        const bool exact_f64 = mx != 0 && my != 0 && bits <= 53;
         for (int sub = 0; sub < nsub; ++sub) {
             int tbase, tfree;
             digits3(sub, T, tbase, tfree);
-            // subtree operands: sum of the selected Q-blocks, straight from HBM
+            // subtree operands: the sum of the Q-blocks b with b & ~tfree == tbase
             __syncthreads();
-            for (int j = tid; j < Q; j += NT) {
+            if (tid < Q) {
                 u64 X = 0, Y = 0;
-                int m = tfree;
-                for (;;) {
-                    const int64_t idx = (int64_t)(tbase | m) * Q + j;
-                    if (idx < Lp) {
-                        X += x[off + idx];
-                        Y += y[off + idx];
-                    }
-                    if (m == 0) break;
-                    m = (m - 1) & tfree;
+#pragma unroll
+                for (int b = 0; b < kKarMaxBlocks; ++b) {
+                    const bool sel = (b & ~tfree) == tbase;
+                    X += sel ? xb[b] : 0ull;
+                    Y += sel ? yb[b] : 0ull;
                 }
-                qx[j] = X;
-                qy[j] = Y;
+                qx[tid] = X;
+                qy[tid] = Y;
             }
             __syncthreads();
             // leaf products, one wavefront per leaf
@@ -131,15 +173,37 @@ k_karatsuba(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__restric
                         m = (m - 1) & freem;
                     }
                 }
-                // schoolbook in registers: x_i is broadcast from lane i, y turns one lane per
-                // step, so lane l sees y_(l-i) mod 64; the blocks are zero beyond s, so lane l
-                // collects z_l (i <= l) and z_(l+64) (i > l)
-                u64 yr = Y, lo = 0, hi = 0;
-                for (int i = 0; i < s; ++i) {
-                    const u64 p = readlane64(X, i) * yr;
-                    if (i <= lane) lo += p;
-                    else hi += p;
-                    yr = rot1(yr);
+                // schoolbook in registers: x_i is broadcast from lane i and y moves one lane
+                // per step.  Rotated, lane l sees y_((l-i) mod 64); shifted (zeros entering at
+                // lane 0), it sees y_(l-i) for i <= l and 0 otherwise.  The blocks are zero
+                // beyond s, so lane l's z_l is the shifted sum and z_(l+64) is the rotated sum
+                // minus the shifted one.
+                u64 lo = 0, hi = 0;
+                if (exact_f64) {
+                    // FP64 fused multiply-adds: exact, see above
+                    const double Xd = (double)(int64_t)X;
+                    double yr = (double)(int64_t)Y, ys = yr, dt = 0.0, dl = 0.0;
+#pragma unroll 4
+                    for (int i = 0; i < s; ++i) {
+                        const double xi = __longlong_as_double((long long)readlane64((u64)__double_as_longlong(Xd), i));
+                        dt = fma(xi, yr, dt);
+                        dl = fma(xi, ys, dl);
+                        yr = __longlong_as_double((long long)rot1((u64)__double_as_longlong(yr)));
+                        ys = __longlong_as_double((long long)shr1((u64)__double_as_longlong(ys)));
+                    }
+                    lo = (u64)(int64_t)dl;
+                    hi = (u64)(int64_t)(dt - dl);
+                } else {
+                    u64 yr = Y, ys = Y, t = 0;
+#pragma unroll 4
+                    for (int i = 0; i < s; ++i) {
+                        const u64 xi = readlane64(X, i);
+                        t += xi * yr;
+                        lo += xi * ys;
+                        yr = rot1(yr);
+                        ys = shr1(ys);
+                    }
+                    hi = t - lo;
                 }
                 u64 *pr = R2 + leaf * 2 * s;  // 2s slots: z_0 .. z_(2s-2) and the zero pad
                 if (lane < 2 * s) pr[lane] = lo;

@@ -190,6 +190,28 @@ def test_karatsuba_wraparound():
     assert np.array_equal(host(z), oracle.karatsuba(x, y))
 
 
+def test_karatsuba_ragged_mixed_magnitudes():
+    """Pairs take the exact-FP64 leaf products only when their coefficients are
+    small enough (bits(x) + bits(y) + 2d + bits(s) <= 53); ragged pairs of both
+    kinds, and ones right at the bound, match the oracle's ring arithmetic bit
+    for bit."""
+    import oracle
+    lens = [4096, 777, 4608, 256, 3000, 2049, 1]
+    mags = [2 ** 15, 2 ** 62, 2 ** 14, 2 ** 20, 2 ** 15 + 1, 2 ** 40, 2 ** 63 - 1]
+    xs, ys = [], []
+    for i, (L, m) in enumerate(zip(lens, mags)):
+        xs.append(synthetic.integers(60 + i, L, -m, m))
+        ys.append(synthetic.integers(70 + i, L, -m, m))
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    z = convolution.karatsuba_ragged(off, to_dev(np.concatenate(xs), torch.int64),
+                                     to_dev(np.concatenate(ys), torch.int64))
+    z = host(z)
+    for p, L in enumerate(lens):
+        ref = oracle.karatsuba(xs[p][None, :], ys[p][None, :])[0]
+        got = z[2 * off[p] - p: 2 * off[p] - p + 2 * L - 1]
+        assert np.array_equal(got, ref), (p, L, mags[p])
+
+
 def test_uneven_batch():
     import oracle
     lengths = [64, 1, 0, 60, 97, 257, 1000, 4099, 20000, 65536, 17, 65521, 100003, 5]
