@@ -95,13 +95,74 @@ __device__ __forceinline__ void fast_inverse_store(const AfPass &d, typename Cx<
     lds_barrier();
 }
 
-// Per line: untangle both half-length spectra to full spectra, multiply (by
-// the conjugate of the second for the adjoint's correlations) and entangle the
-// product back to the half-length spectrum of the inverse.  One thread takes a
-// pair (e, L - e), so dst may be A or Bb.
+// The product pairs (e, L - e) of the B lines, H = L/2 + 1 per line: pair
+// idx = tid + q NT (q < kRconvPairs) belongs to one thread throughout.
+template <int L, int B> struct RconvPairs {
+    static constexpr int NT = B * L / 16;
+    static constexpr int H = L / 2 + 1;
+    static constexpr int NP = (B * H + NT - 1) / NT;  // 9 for every shape
+};
+
+// The half-length spectrum values a thread's pairs need, from LDS into registers.
 template <typename T, int L, int B>
-__device__ __forceinline__ void fast_spec_product(const typename Cx<T>::C *A, const typename Cx<T>::C *Bb,
-                                                  typename Cx<T>::C *dst, const typename Cx<T>::W *aux, bool conj_b) {
+__device__ __forceinline__ void fast_pairs_read(const typename Cx<T>::C *A, typename Cx<T>::C (&p0)[RconvPairs<L, B>::NP],
+                                                typename Cx<T>::C (&p1)[RconvPairs<L, B>::NP]) {
+    using RP = RconvPairs<L, B>;
+    constexpr int LD = fast_ld(L, B);
+#pragma unroll
+    for (int q = 0; q < RP::NP; ++q) {
+        const int idx = threadIdx.x + q * RP::NT;
+        if (idx < B * RP::H) {
+            const int b = idx / RP::H, e = idx - b * RP::H;
+            const int er = (L - e) & (L - 1);
+            p0[q] = A[b * LD + fpad(e)];
+            p1[q] = A[b * LD + fpad(er)];
+        }
+    }
+}
+
+// Per pair: untangle both half-length spectra (the first from registers, the
+// second from Bb) to full spectra, multiply (by the conjugate of the second
+// for the adjoint's correlations) and entangle the product back into Bb as the
+// half-length spectrum of the inverse.  Each pair has one owner thread.
+template <typename T, int L, int B>
+__device__ __forceinline__ void fast_spec_product(const typename Cx<T>::C (&p0)[RconvPairs<L, B>::NP],
+                                                  const typename Cx<T>::C (&p1)[RconvPairs<L, B>::NP],
+                                                  typename Cx<T>::C *Bb, const typename Cx<T>::W *aux, bool conj_b) {
+    using C = typename Cx<T>::C;
+    using RP = RconvPairs<L, B>;
+    constexpr int LD = fast_ld(L, B);
+#pragma unroll
+    for (int q = 0; q < RP::NP; ++q) {
+        const int idx = threadIdx.x + q * RP::NT;
+        if (idx >= B * RP::H) continue;
+        const int b = idx / RP::H, e = idx - b * RP::H;
+        const int er = (L - e) & (L - 1);
+        const int ie = b * LD + fpad(e), ir = b * LD + fpad(er);
+        const C a0 = p0[q], a1 = p1[q], b0 = Bb[ie], b1 = Bb[ir];
+        C sa_lo, sa_hi, sb_lo, sb_hi;
+        untangle<T>(a0, a1, aux[e], false, sa_lo, sa_hi);
+        untangle<T>(b0, b1, aux[e], false, sb_lo, sb_hi);
+        if (conj_b) { sb_lo = cconj(sb_lo); sb_hi = cconj(sb_hi); }
+        const C z0 = entangle<T>(cmul(sa_lo, sb_lo), cmul(sa_hi, sb_hi), aux[e], false);
+        C z1 = z0;
+        if (er != e) {
+            C ta_lo, ta_hi, tb_lo, tb_hi;
+            untangle<T>(a1, a0, aux[er], false, ta_lo, ta_hi);
+            untangle<T>(b1, b0, aux[er], false, tb_lo, tb_hi);
+            if (conj_b) { tb_lo = cconj(tb_lo); tb_hi = cconj(tb_hi); }
+            z1 = entangle<T>(cmul(ta_lo, tb_lo), cmul(ta_hi, tb_hi), aux[er], false);
+        }
+        Bb[ie] = z0;
+        if (er != e) Bb[ir] = z1;
+    }
+}
+
+// The same with the first spectrum in a second LDS buffer (FP64, whose pair
+// registers would take the whole register file); dst may be A or Bb.
+template <typename T, int L, int B>
+__device__ __forceinline__ void fast_spec_product2(const typename Cx<T>::C *A, const typename Cx<T>::C *Bb,
+                                                   typename Cx<T>::C *dst, const typename Cx<T>::W *aux, bool conj_b) {
     using C = typename Cx<T>::C;
     constexpr int NT = B * L / 16;
     constexpr int LD = fast_ld(L, B);
@@ -129,9 +190,13 @@ __device__ __forceinline__ void fast_spec_product(const typename Cx<T>::C *A, co
     }
 }
 
+// FP32: one line buffer; the first signal's spectrum is kept in the registers of the
+// threads that own its pairs, so the buffer serves every transform in turn and
+// twice as many workgroups fit per CU as with two buffers.
 template <typename T, int L, int B>
 __host__ __device__ constexpr size_t fast_rconv_lds_bytes() {
-    return 2 * (size_t)fast_ld(L, B) * B * sizeof(typename Cx<T>::C) + (size_t)FastTables<L>::STAGE_ENTRIES * 16;
+    return (sizeof(T) == 4 ? 1 : 2) * (size_t)fast_ld(L, B) * B * sizeof(typename Cx<T>::C) +
+           (size_t)FastTables<L>::STAGE_ENTRIES * 16;
 }
 
 // Forward (VJP = false): out0 = fromSpectrum(toSpectrum in0 * toSpectrum in1), in0 = x, in1 = k.
@@ -144,39 +209,52 @@ k_fast_rconv(const AfPass d, const T *__restrict__ in0, const T *__restrict__ in
     using C = typename Cx<T>::C;
     using W = typename Cx<T>::W;
     using FT = FastTables<L>;
+    using RP = RconvPairs<L, B>;
     constexpr int NT = B * L / 16;
     constexpr int LD = fast_ld(L, B);
+    constexpr bool ONE = sizeof(T) == 4;  // one line buffer (FP32) or two (FP64)
     extern __shared__ __align__(16) unsigned char smem_rc[];
     C *A = (C *)smem_rc;
-    C *Bb = A + LD * B;
+    C *Bb = ONE ? A : A + LD * B;
     W *stab = (W *)(Bb + LD * B);
     if (FT::STAGE_LDS)
         for (int i = threadIdx.x; i < L; i += NT) stab[i] = ((const W *)d.tw)[i];
     const W *aux = (const W *)d.aux_tw;
     const int64_t row0 = (int64_t)blockIdx.x * B;
-    // FP32: every input row is requested before the first transform -- occupancy
-    // is bound by LDS (two line buffers), so the registers are free, and the HBM
-    // latency is paid once instead of once per signal.  FP64 rows would take the
-    // whole register file; they are loaded one before each transform.
-    constexpr bool PRE = sizeof(T) == 4;
-    C v[16], u[16], t[16];
-    fast_rows_load<T, L, B>(in0, row0, v);
-    if (PRE) {
-        fast_rows_load<T, L, B>(in1, row0, u);
-        if (VJP) fast_rows_load<T, L, B>(in2, row0, t);
-    }
-    fast_fft_lds<T, L, B, -1>(d, v, A, stab);
-    if (!PRE) fast_rows_load<T, L, B>(in1, row0, u);
-    fast_fft_lds<T, L, B, -1>(d, u, Bb, stab);
-    fast_spec_product<T, L, B>(A, Bb, Bb, aux, VJP);
-    lds_barrier();
-    fast_inverse_store<T, L, B>(d, Bb, stab, out0, row0);
-    if (VJP) {
-        if (!PRE) fast_rows_load<T, L, B>(in2, row0, t);
-        fast_fft_lds<T, L, B, -1>(d, t, Bb, stab);
-        fast_spec_product<T, L, B>(A, Bb, Bb, aux, true);
+    C v[16];
+    if constexpr (ONE) {
+        C p0[RP::NP], p1[RP::NP];
+        fast_rows_load<T, L, B>(in0, row0, v);
+        fast_fft_lds<T, L, B, -1>(d, v, A, stab);
+        fast_pairs_read<T, L, B>(A, p0, p1);
+        fast_rows_load<T, L, B>(in1, row0, v);
+        lds_barrier();  // every pair is read before the second transform writes A
+        fast_fft_lds<T, L, B, -1>(d, v, A, stab);
+        fast_spec_product<T, L, B>(p0, p1, A, aux, VJP);
         lds_barrier();
-        fast_inverse_store<T, L, B>(d, Bb, stab, out1, row0);
+        fast_inverse_store<T, L, B>(d, A, stab, out0, row0);
+        if (VJP) {
+            fast_rows_load<T, L, B>(in2, row0, v);
+            fast_fft_lds<T, L, B, -1>(d, v, A, stab);
+            fast_spec_product<T, L, B>(p0, p1, A, aux, true);
+            lds_barrier();
+            fast_inverse_store<T, L, B>(d, A, stab, out1, row0);
+        }
+    } else {
+        fast_rows_load<T, L, B>(in0, row0, v);
+        fast_fft_lds<T, L, B, -1>(d, v, A, stab);
+        fast_rows_load<T, L, B>(in1, row0, v);
+        fast_fft_lds<T, L, B, -1>(d, v, Bb, stab);
+        fast_spec_product2<T, L, B>(A, Bb, Bb, aux, VJP);
+        lds_barrier();
+        fast_inverse_store<T, L, B>(d, Bb, stab, out0, row0);
+        if (VJP) {
+            fast_rows_load<T, L, B>(in2, row0, v);
+            fast_fft_lds<T, L, B, -1>(d, v, Bb, stab);
+            fast_spec_product2<T, L, B>(A, Bb, Bb, aux, true);
+            lds_barrier();
+            fast_inverse_store<T, L, B>(d, Bb, stab, out1, row0);
+        }
     }
 }
 

@@ -91,11 +91,18 @@ __device__ __forceinline__ double2 from_w(float4 w) {
 }
 __device__ __forceinline__ double2 from_w(double2 w) { return w; }
 
-// x * w for an FP64 twiddle w, in precision T (compensated for FP32).
+// x * w for an FP64 twiddle w, in precision T: for FP32 the product is formed
+// in FP64 and rounded once (two conversions, a complex FP64 product and two
+// conversions back -- fewer instructions than splitting w into a hi/lo pair for
+// a compensated FP32 product, and no less accurate).
 __device__ __forceinline__ float2 dmul(float2 x, double2 w, bool plain) {
-    float4 ww;
-    to_w(w, ww);
-    return twmul(x, ww, -1.0f, plain);
+    if (plain) {
+        float4 ww;
+        to_w(w, ww);
+        return twmul(x, ww, -1.0f, true);
+    }
+    const double xr = (double)x.x, xi = (double)x.y;
+    return make_float2((float)(xr * w.x - xi * w.y), (float)(xr * w.y + xi * w.x));
 }
 __device__ __forceinline__ double2 dmul(double2 x, double2 w, bool) { return cmul(x, w); }
 

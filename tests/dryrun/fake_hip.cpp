@@ -398,7 +398,8 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
             if (d.L != L) problem(name + ": descriptor L differs from the instantiation");
             if ((int64_t)grid.x * B != d.nrows) problem(name + ": grid * B differs from the rows");
             if ((int64_t)block.x != (int64_t)B * L / 16) problem(name + ": block differs from B*L/16");
-            const int64_t need = 2 * (int64_t)fast_ld_h(L, B) * B * 2 * tsz + (L <= 1024 ? L : 0) * 16;
+            // FP32: one line buffer (first spectrum in registers); FP64: two
+            const int64_t need = (tsz == 4 ? 1 : 2) * (int64_t)fast_ld_h(L, B) * B * 2 * tsz + (L <= 1024 ? L : 0) * 16;
             if ((int64_t)shmem != need) problem(name + ": LDS size differs from the kernel's layout");
             for (int i = 0; i < (vjp ? 3 : 2); ++i) in_alloc(in[i], 0, d.nrows * row, "input", name);
             for (int i = 0; i < (vjp ? 2 : 1); ++i) in_alloc(out[i], 0, d.nrows * row, "output", name);
