@@ -57,6 +57,10 @@ _SIGNATURES = {
     "af_execute": (ctypes.c_int, [_v, _i64, _v, _v, _v, _v, _v, _v]),
     "af_plan_num_passes": (ctypes.c_int, [_v, _i64, ctypes.POINTER(_i32)]),
     "af_execute_pass": (ctypes.c_int, [_v, _i32, _i64, _v, _v, _v, _v, _v, _v]),
+    "af_two_to_spectrum": (ctypes.c_int, [_v, _i64, _v, _v, _v, _v, _v, _v, _v]),
+    "af_two_from_spectrum": (ctypes.c_int, [_v, _i64, _v, _v, _v, _v, _v, _v, _v]),
+    "af_untangle_spectra": (ctypes.c_int, [_i32, _i64, _i64, _v, _v, _v, _v, _v, _v, _v]),
+    "af_entangle_spectra": (ctypes.c_int, [_i32, _i64, _i64, _v, _v, _v, _v, _v, _v, _v]),
     "af_plan_tune": (ctypes.c_int, [_v, _i64, _i32, _v, ctypes.c_char_p, ctypes.c_size_t]),
     "af_wisdom_export": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "af_wisdom_import": (ctypes.c_int, [ctypes.c_char_p]),

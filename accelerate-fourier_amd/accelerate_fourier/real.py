@@ -6,6 +6,12 @@ with ``sub`` the half-length complex transform; the result is the full (not
 halved) spectrum.  ``from_spectrum(sub, s)`` is ``fromSpectrum`` (Real.hs:92-107),
 unnormalised.  Both run as one fused gfx950 pass: the half-length FFT with the
 (de)interleaving and the (un)tangling folded into its loads and stores.
+
+``two_to_spectrum(sub, x, y)`` / ``two_from_spectrum(sub, a, b)`` are
+``twoToSpectrum`` / ``twoFromSpectrum`` (Real.hs:129-196): two real signals
+through one complex transform ``sub`` of their full length, with
+``untangle_spectra`` / ``entangle_spectra`` (Real.hs:168-236) as the packing
+steps.  The reference's arrays of pairs are passed as their two components.
 """
 from __future__ import annotations
 
@@ -128,3 +134,88 @@ def convolve_vjp(x, k, w, dx=None, dk=None):
                                                k.data_ptr(), w.data_ptr(), dx.data_ptr(), dk.data_ptr(),
                                                _arrays.stream_handle()), "af_vjp_real_convolve")
     return dx, dk
+
+
+def _two_shape(a, b, what):
+    if tuple(a.shape) != tuple(b.shape):
+        raise ValueError(f"the two {what} must have equal shapes")
+
+
+def two_to_spectrum(sub: Transform, x, y, out=None):
+    """Real.twoToSpectrum sub (Real.hs:129-135): the spectra (a, b) of the real
+    rows x and y from one complex transform of x + i y (any length)."""
+    x, y = _arrays.materialize(x), _arrays.materialize(y)
+    _check_real(x, "x")
+    _check_real(y, "y")
+    _two_shape(x, y, "signals")
+    if x.dtype != y.dtype:
+        raise TypeError("the two signals must have the same dtype")
+    n = x.shape[-1]
+    if sub.n != n:
+        raise ValueError("the sub-transform must have the signals' length")
+    prec = _real_prec(x)
+    if out is None:
+        ct = _arrays.complex_dtype(prec)
+        out = (torch.empty(x.shape, dtype=ct, device=x.device), torch.empty(x.shape, dtype=ct, device=x.device))
+    are, aim = _arrays.ptrs(out[0])
+    bre, bim = _arrays.ptrs(out[1])
+    p = sub.plan(prec)
+    _abi.check(_abi.lib().af_two_to_spectrum(p.handle, _arrays.batch_of(tuple(x.shape)), x.data_ptr(), y.data_ptr(),
+                                             are, aim, bre, bim, _arrays.stream_handle()), "af_two_to_spectrum")
+    return out
+
+
+def two_from_spectrum(sub: Transform, a, b, out=None):
+    """Real.twoFromSpectrum sub (Real.hs:190-196): the real and imaginary
+    parts (x, y) of sub applied to the entangled spectrum a + i b."""
+    a, b = _arrays.materialize(a), _arrays.materialize(b)
+    are, aim, shape = _arrays.complex_view(a, "a")
+    bre, bim, shape_b = _arrays.complex_view(b, "b")
+    if tuple(shape) != tuple(shape_b):
+        raise ValueError("the two spectra must have equal shapes")
+    n = shape[-1]
+    if sub.n != n:
+        raise ValueError("the sub-transform must have the spectra's length")
+    prec = _arrays.precision_of_complex(a)
+    if prec != _arrays.precision_of_complex(b):
+        raise TypeError("the two spectra must have the same precision")
+    if out is None:
+        dev = a[0].device if isinstance(a, tuple) else a.device
+        rt = _arrays.real_dtype(prec)
+        out = (torch.empty(shape, dtype=rt, device=dev), torch.empty(shape, dtype=rt, device=dev))
+    p = sub.plan(prec)
+    _abi.check(_abi.lib().af_two_from_spectrum(p.handle, _arrays.batch_of(shape), are, aim, bre, bim,
+                                               out[0].data_ptr(), out[1].data_ptr(), _arrays.stream_handle()),
+               "af_two_from_spectrum")
+    return out
+
+
+def untangle_spectra(z, out=None):
+    """Real.untangleSpectra (Real.hs:168-187): (a_k, b_k) = ((z_k + conj z_-k) / 2,
+    (z_k - conj z_-k) (-i / 2)) along the last axis."""
+    z = _arrays.materialize(z)
+    zre, zim, shape = _arrays.complex_view(z, "z")
+    prec = _arrays.precision_of_complex(z)
+    if out is None:
+        out = (_arrays.empty_like_complex(z), _arrays.empty_like_complex(z))
+    are, aim = _arrays.ptrs(out[0])
+    bre, bim = _arrays.ptrs(out[1])
+    _abi.check(_abi.lib().af_untangle_spectra(prec, _arrays.batch_of(shape), shape[-1], zre, zim, are, aim, bre, bim,
+                                              _arrays.stream_handle()), "af_untangle_spectra")
+    return out
+
+
+def entangle_spectra(c, d, out=None):
+    """Real.entangleSpectra (Real.hs:208-236): c + d i."""
+    c, d = _arrays.materialize(c), _arrays.materialize(d)
+    cre, cim, shape = _arrays.complex_view(c, "c")
+    dre, dim, shape_d = _arrays.complex_view(d, "d")
+    if tuple(shape) != tuple(shape_d):
+        raise ValueError("the two spectra must have equal shapes")
+    prec = _arrays.precision_of_complex(c)
+    if out is None:
+        out = _arrays.empty_like_complex(c)
+    zre, zim = _arrays.ptrs(out)
+    _abi.check(_abi.lib().af_entangle_spectra(prec, _arrays.batch_of(shape), shape[-1], cre, cim, dre, dim, zre, zim,
+                                              _arrays.stream_handle()), "af_entangle_spectra")
+    return out

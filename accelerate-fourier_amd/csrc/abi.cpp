@@ -1307,6 +1307,64 @@ af_status af_vjp_real_convolve(af_plan plan, int64_t batch, const void *x, const
     return AF_SUCCESS;
 }
 
+// ---- two-for-one real packing (Real.hs:129-236) ----
+
+af_status af_two_to_spectrum(af_plan plan, int64_t batch, const void *x, const void *y, void *a_re, void *a_im,
+                             void *b_re, void *b_im, void *stream) {
+    if (!plan || plan->desc.kind != AF_C2C || batch < 0) return AF_INVALID_ARG;
+    const int64_t n = plan->desc.length;
+    if (n == 0 || batch == 0) return AF_SUCCESS;
+    if (!x || !y || !a_re || !b_re) return AF_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    // z = x + i y is the planar complex input (re = x, im = y); its spectrum goes
+    // to stream-ordered scratch, then untangles into a and b
+    Scratch sz;
+    AFCHK(sz.get((size_t)batch * (size_t)n * elem_bytes(plan->desc.precision), nullptr, s));
+    AFCHK(run_c2c(plan, plan->desc.sign, batch, x, y, sz.p, nullptr, nullptr, s));
+    HIPCHK(launch_two_pack(plan->prec, true, sz.p, nullptr, nullptr, nullptr, a_re, a_im, b_re, b_im, n, batch, s));
+    return AF_SUCCESS;
+}
+
+af_status af_two_from_spectrum(af_plan plan, int64_t batch, const void *a_re, const void *a_im, const void *b_re,
+                               const void *b_im, void *x, void *y, void *stream) {
+    if (!plan || plan->desc.kind != AF_C2C || batch < 0) return AF_INVALID_ARG;
+    const int64_t n = plan->desc.length;
+    if (n == 0 || batch == 0) return AF_SUCCESS;
+    if (!x || !y || !a_re || !b_re) return AF_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    // entangled spectrum a + i b to scratch; its transform's real and imaginary
+    // parts are the two signals (planar output: re -> x, im -> y)
+    Scratch sz;
+    AFCHK(sz.get((size_t)batch * (size_t)n * elem_bytes(plan->desc.precision), nullptr, s));
+    HIPCHK(launch_two_pack(plan->prec, false, a_re, a_im, b_re, b_im, sz.p, nullptr, nullptr, nullptr, n, batch, s));
+    return run_c2c(plan, plan->desc.sign, batch, sz.p, nullptr, x, y, nullptr, s);
+}
+
+af_status af_untangle_spectra(int32_t precision, int64_t batch, int64_t n, const void *z_re, const void *z_im,
+                              void *a_re, void *a_im, void *b_re, void *b_im, void *stream) {
+    if ((precision != AF_F64 && precision != AF_F32) || batch < 0 || n < 0) return AF_INVALID_ARG;
+    if (n == 0 || batch == 0) return AF_SUCCESS;
+    if (!z_re || !a_re || !b_re) return AF_INVALID_ARG;
+    // each output reads two input positions: no aliasing of input and outputs
+    for (const void *o : {(const void *)a_re, (const void *)a_im, (const void *)b_re, (const void *)b_im})
+        if (o && (o == z_re || o == z_im)) return AF_INVALID_ARG;
+    AFCHK(device_init());
+    HIPCHK(launch_two_pack(precision == AF_F64 ? PREC_F64 : PREC_F32, true, z_re, z_im, nullptr, nullptr, a_re, a_im,
+                           b_re, b_im, n, batch, (hipStream_t)stream));
+    return AF_SUCCESS;
+}
+
+af_status af_entangle_spectra(int32_t precision, int64_t batch, int64_t n, const void *c_re, const void *c_im,
+                              const void *d_re, const void *d_im, void *z_re, void *z_im, void *stream) {
+    if ((precision != AF_F64 && precision != AF_F32) || batch < 0 || n < 0) return AF_INVALID_ARG;
+    if (n == 0 || batch == 0) return AF_SUCCESS;
+    if (!c_re || !d_re || !z_re) return AF_INVALID_ARG;
+    AFCHK(device_init());
+    HIPCHK(launch_two_pack(precision == AF_F64 ? PREC_F64 : PREC_F32, false, c_re, c_im, d_re, d_im, z_re, z_im,
+                           nullptr, nullptr, n, batch, (hipStream_t)stream));
+    return AF_SUCCESS;
+}
+
 af_status af_convolve_cyclic(af_plan plan, int64_t batch, const void *x_re, const void *x_im, const void *y_re,
                              const void *y_im, void *out_re, void *out_im, void *workspace, void *stream) {
     if (!plan || plan->desc.kind != AF_CONV_CYCLIC || batch < 0) return AF_INVALID_ARG;

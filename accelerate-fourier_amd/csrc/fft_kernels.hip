@@ -315,6 +315,42 @@ __global__ void k_convert(WT *__restrict__ out, const double2 *__restrict__ in, 
 }
 
 // ---------------------------------------------------------------------------
+// Two-for-one real packing (Real.hs:129-236): elementwise over rows of n,
+// x over the elements of a row, y over the rows (grid-strided).
+// ---------------------------------------------------------------------------
+
+// untangleSpectra (Real.hs:168-187): for the spectrum z of x + i y,
+// a_k = (z_k + conj z_-k) / 2 and b_k = (z_k - conj z_-k) (-i / 2).
+template <typename T>
+__global__ void k_two_untangle(const void *z_re, const void *z_im, void *a_re, void *a_im, void *b_re, void *b_im,
+                               int64_t n, int64_t rows) {
+    using C = typename Cx<T>::C;
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t kr = k ? n - k : 0;
+    for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {
+        const C z = ld_c<T>(z_re, z_im, r * n + k), w = ld_c<T>(z_re, z_im, r * n + kr);
+        const T h = T(0.5);
+        // (z - conj w) (-i / 2) = ((z.im + w.im) / 2, -(z.re - w.re) / 2)
+        st_c<T>(a_re, a_im, r * n + k, mk((z.x + w.x) * h, (z.y - w.y) * h));
+        st_c<T>(b_re, b_im, r * n + k, mk((z.y + w.y) * h, (w.x - z.x) * h));
+    }
+}
+
+// entangleSpectra (Real.hs:208-236): z = c + d i.
+template <typename T>
+__global__ void k_two_entangle(const void *c_re, const void *c_im, const void *d_re, const void *d_im, void *z_re,
+                               void *z_im, int64_t n, int64_t rows) {
+    using C = typename Cx<T>::C;
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {
+        const C c = ld_c<T>(c_re, c_im, r * n + k), d = ld_c<T>(d_re, d_im, r * n + k);
+        st_c<T>(z_re, z_im, r * n + k, mk(c.x - d.y, c.y + d.x));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 
@@ -417,6 +453,25 @@ hipError_t launch_real_conv_vjp(int prec, const AfPass &d, const void *x, const 
     else
         hipLaunchKernelGGL(k_real_conv_vjp<float>, dim3((unsigned)d.nrows), dim3(d.nthreads), d.lds_bytes, s, d,
                            (const float *)x, (const float *)k, (const float *)w, (float *)dx, (float *)dk);
+    return hipGetLastError();
+}
+
+hipError_t launch_two_pack(int prec, bool untangle, const void *p_re, const void *p_im, const void *q_re,
+                           const void *q_im, void *o1_re, void *o1_im, void *o2_re, void *o2_im, int64_t n,
+                           int64_t rows, hipStream_t s) {
+    if (n <= 0 || rows <= 0) return hipSuccess;
+    const dim3 g(grid_for(n, 256), (unsigned)std::min<int64_t>(rows, 65535)), b(256);
+    if (untangle) {
+        if (prec == PREC_F64)
+            hipLaunchKernelGGL(k_two_untangle<double>, g, b, 0, s, p_re, p_im, o1_re, o1_im, o2_re, o2_im, n, rows);
+        else
+            hipLaunchKernelGGL(k_two_untangle<float>, g, b, 0, s, p_re, p_im, o1_re, o1_im, o2_re, o2_im, n, rows);
+    } else {
+        if (prec == PREC_F64)
+            hipLaunchKernelGGL(k_two_entangle<double>, g, b, 0, s, p_re, p_im, q_re, q_im, o1_re, o1_im, n, rows);
+        else
+            hipLaunchKernelGGL(k_two_entangle<float>, g, b, 0, s, p_re, p_im, q_re, q_im, o1_re, o1_im, n, rows);
+    }
     return hipGetLastError();
 }
 

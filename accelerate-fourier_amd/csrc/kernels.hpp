@@ -32,6 +32,13 @@ enum { PACK_UNTANGLE = 0, PACK_ENTANGLE = 1, PACK_ENTANGLE_ADJ = 2, PACK_PRODUCT
 hipError_t launch_pack(int prec, int op, const void *a, const void *a_im, const void *b, void *o_re, void *o_im,
                        const void *aux, int64_t L, int64_t rows, int plain, int arg, hipStream_t s);
 
+// Two-for-one real packing, elementwise over rows of n (fft_kernels.hip):
+// untangle = true: (o1, o2) = untangleSpectra p; false: o1 = entangleSpectra (p, q).
+// Each complex array is planar (re, im) or interleaved (im == nullptr).
+hipError_t launch_two_pack(int prec, bool untangle, const void *p_re, const void *p_im, const void *q_re,
+                           const void *q_im, void *o1_re, void *o1_im, void *o2_re, void *o2_im, int64_t n,
+                           int64_t rows, hipStream_t s);
+
 // Allow every kernel the full 160 KiB of LDS (called once).
 void init_kernel_attributes();
 void init_karatsuba_attributes();

@@ -150,6 +150,31 @@ af_status af_wisdom_export(char *buf, size_t buflen, size_t *needed);
 af_status af_wisdom_import(const char *text);
 
 /*
+ * Two-for-one real packing (AF_C2C plans of length n, any n).
+ * af_two_to_spectrum replaces Real.twoToSpectrum (Planned.transform sign n)
+ * (Real.hs:129-135): the spectra a and b of the real rows x and y from one
+ * complex transform of x + i y, untangled as untangleSpectra (168-187).
+ * af_two_from_spectrum replaces Real.twoFromSpectrum (190-196): x and y are the
+ * real and imaginary parts of the transform of the entangled a + i b
+ * (entangleSpectra, 208-236).  x, y: batch rows of n reals each (the
+ * reference's array of pairs as its two component arrays); a, b: planar
+ * (re, im) or interleaved (im NULL).  Scratch is stream-ordered.
+ */
+af_status af_two_to_spectrum(af_plan plan, int64_t batch, const void *x, const void *y,
+                             void *a_re, void *a_im, void *b_re, void *b_im, void *stream);
+af_status af_two_from_spectrum(af_plan plan, int64_t batch, const void *a_re, const void *a_im,
+                               const void *b_re, const void *b_im, void *x, void *y, void *stream);
+/*
+ * The packing steps alone, elementwise over batch rows of n: Real.untangleSpectra
+ * (Real.hs:168-187; z must not share memory with a or b) and
+ * Real.entangleSpectra (Real.hs:208-236).
+ */
+af_status af_untangle_spectra(int32_t precision, int64_t batch, int64_t n, const void *z_re, const void *z_im,
+                              void *a_re, void *a_im, void *b_re, void *b_im, void *stream);
+af_status af_entangle_spectra(int32_t precision, int64_t batch, int64_t n, const void *c_re, const void *c_im,
+                              const void *d_re, const void *d_im, void *z_re, void *z_im, void *stream);
+
+/*
  * Real signal to full spectrum (AF_R2C_FULL plans).
  * Replaces: Real.toSpectrum (Planned.transform sign (n/2)) (Real.hs:49-70).
  * x: batch x n reals; out: batch x n complex (the full, not halved, spectrum).

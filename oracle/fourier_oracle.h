@@ -66,6 +66,16 @@ int orc_to_spectrum_f32(int sub_sign, int64_t n, int64_t rows, const float *x,
 int orc_from_spectrum_f32(int sub_sign, int64_t n, int64_t rows, const float *re,
                           const float *im, float *y);
 
+/* ---- two-for-one real packing (src/.../Fourier/Real.hs:124-236) ---- */
+int orc_untangle_spectra_f64(int64_t n, int64_t rows, const double *zre, const double *zim, double *are,
+                             double *aim, double *bre, double *bim);
+int orc_entangle_spectra_f64(int64_t n, int64_t rows, const double *cre, const double *cim, const double *dre,
+                             const double *dim, double *zre, double *zim);
+int orc_untangle_spectra_f32(int64_t n, int64_t rows, const float *zre, const float *zim, float *are,
+                             float *aim, float *bre, float *bim);
+int orc_entangle_spectra_f32(int64_t n, int64_t rows, const float *cre, const float *cim, const float *dre,
+                             const float *dim, float *zre, float *zim);
+
 /* ---- FFT cyclic convolution (src/.../Fourier/Planned.hs:779-821) ---- */
 int orc_convolve_cyclic_f64(int64_t n, int64_t rows, const double *xre, const double *xim,
                             const double *yre, const double *yim, double *ore, double *oim);

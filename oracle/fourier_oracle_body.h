@@ -365,6 +365,39 @@ int SFX(orc_from_spectrum)(int sub_sign, int64_t N, int64_t rows, const REAL *re
     return 0;
 }
 
+/* untangleSpectra (Real.hs:168-173) with untangleCoefficient (182-187) and
+ * Cyclic.reverse (Cyclic.hs:20-28): for the spectrum z of x + i y,
+ * a_k = (z_k + conj z_-k) / 2 and b_k = (z_k - conj z_-k) (-i / 2). */
+int SFX(orc_untangle_spectra)(int64_t n, int64_t rows, const REAL *zre, const REAL *zim, REAL *are,
+                              REAL *aim, REAL *bre, REAL *bim) {
+    if (n < 0 || rows < 0) return -1;
+    for (int64_t r = 0; r < rows; ++r)
+        for (int64_t k = 0; k < n; ++k) {
+            const int64_t kr = (n - k) % n;
+            CX a = SFX(mk)(zre[r * n + k], zim[r * n + k]);
+            CX bc = SFX(mk)(zre[r * n + kr], -zim[r * n + kr]);
+            CX s = SFX(add)(a, bc), d = SFX(sub)(a, bc);
+            CX half_s = SFX(mk)(s.re / 2, s.im / 2);
+            CX mi2 = SFX(mul)(d, SFX(mk)(0, (REAL)-1 / 2));
+            are[r * n + k] = half_s.re; aim[r * n + k] = half_s.im;
+            bre[r * n + k] = mi2.re; bim[r * n + k] = mi2.im;
+        }
+    return 0;
+}
+
+/* entangleSpectra (Real.hs:208-224): the first component of
+ * entangleCoefficient c d (231-236), c + d i. */
+int SFX(orc_entangle_spectra)(int64_t n, int64_t rows, const REAL *cre, const REAL *cim, const REAL *dre,
+                              const REAL *dim, REAL *zre, REAL *zim) {
+    if (n < 0 || rows < 0) return -1;
+    for (int64_t i = 0; i < n * rows; ++i) {
+        CX di = SFX(mul)(SFX(mk)(dre[i], dim[i]), SFX(mk)(0, 1));
+        CX z = SFX(add)(SFX(mk)(cre[i], cim[i]), di);
+        zre[i] = z.re; zim[i] = z.im;
+    }
+    return 0;
+}
+
 /* convolveCyclic (Planned.hs:779-821): transInv (scaleDown (trans x) * trans y)
  * with (trans, transInv) = (inverse sign, forward sign). */
 int SFX(orc_convolve_cyclic)(int64_t n, int64_t rows, const REAL *xre, const REAL *xim,

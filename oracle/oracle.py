@@ -51,6 +51,10 @@ def lib():
             "orc_transform_f64": (_int, [_int, _int, _i64, _i64, _p, _p, _p, _p]),
             "orc_transform_f32": (_int, [_int, _int, _i64, _i64, _p, _p, _p, _p]),
             "orc_dft_bruteforce": (None, [_int, _i64, _i64, _p, _p, _p, _p]),
+            "orc_untangle_spectra_f64": (_int, [_i64, _i64, _p, _p, _p, _p, _p, _p]),
+            "orc_entangle_spectra_f64": (_int, [_i64, _i64, _p, _p, _p, _p, _p, _p]),
+            "orc_untangle_spectra_f32": (_int, [_i64, _i64, _p, _p, _p, _p, _p, _p]),
+            "orc_entangle_spectra_f32": (_int, [_i64, _i64, _p, _p, _p, _p, _p, _p]),
             "orc_to_spectrum_f64": (_int, [_int, _i64, _i64, _p, _p, _p]),
             "orc_from_spectrum_f64": (_int, [_int, _i64, _i64, _p, _p, _p]),
             "orc_to_spectrum_f32": (_int, [_int, _i64, _i64, _p, _p, _p]),
@@ -190,6 +194,45 @@ def to_spectrum(x: np.ndarray, sub_sign: int = -1, precision: str = "f64") -> np
     if fn(sub_sign, n, rows, _ptr(x), _ptr(ore), _ptr(oim)) != 0:
         raise ValueError("invalid arguments")
     return ore + 1j * oim
+
+
+def untangle_spectra(z: np.ndarray, precision: str = "f64"):
+    """Real.untangleSpectra (Real.hs:168-187) over the last axis: (a, b)."""
+    rt = np.float64 if precision == "f64" else np.float32
+    zre, zim = _c(np.real(z), rt), _c(np.imag(z), rt)
+    n = zre.shape[-1]
+    rows = int(np.prod(zre.shape[:-1])) if zre.ndim > 1 else 1
+    out = [np.empty_like(zre) for _ in range(4)]
+    fn = lib().orc_untangle_spectra_f64 if precision == "f64" else lib().orc_untangle_spectra_f32
+    if fn(n, rows, _ptr(zre), _ptr(zim), *[_ptr(o) for o in out]) != 0:
+        raise ValueError("invalid arguments")
+    return out[0] + 1j * out[1], out[2] + 1j * out[3]
+
+
+def entangle_spectra(c: np.ndarray, d: np.ndarray, precision: str = "f64") -> np.ndarray:
+    """Real.entangleSpectra (Real.hs:208-236): c + d i."""
+    rt = np.float64 if precision == "f64" else np.float32
+    parts = [_c(np.real(c), rt), _c(np.imag(c), rt), _c(np.real(d), rt), _c(np.imag(d), rt)]
+    n = parts[0].shape[-1]
+    rows = int(np.prod(parts[0].shape[:-1])) if parts[0].ndim > 1 else 1
+    zre, zim = np.empty_like(parts[0]), np.empty_like(parts[0])
+    fn = lib().orc_entangle_spectra_f64 if precision == "f64" else lib().orc_entangle_spectra_f32
+    if fn(n, rows, *[_ptr(q) for q in parts], _ptr(zre), _ptr(zim)) != 0:
+        raise ValueError("invalid arguments")
+    return zre + 1j * zim
+
+
+def two_to_spectrum(sign: int, x: np.ndarray, y: np.ndarray, precision: str = "f64"):
+    """Real.twoToSpectrum (Planned.transform sign n) (Real.hs:129-135): the
+    spectra of two real signals from one complex transform of x + i y."""
+    return untangle_spectra(transform(sign, np.asarray(x) + 1j * np.asarray(y), precision=precision), precision)
+
+
+def two_from_spectrum(sign: int, a: np.ndarray, b: np.ndarray, precision: str = "f64"):
+    """Real.twoFromSpectrum (Planned.transform sign n) (Real.hs:190-196): the
+    real and imaginary parts of the transform of the entangled spectra."""
+    z = transform(sign, entangle_spectra(a, b, precision), precision=precision)
+    return np.real(z), np.imag(z)
 
 
 def from_spectrum(s: np.ndarray, sub_sign: int = 1, precision: str = "f64") -> np.ndarray:

@@ -460,6 +460,27 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
             in_alloc(powg, 0, count * 4, "powg", name);
         };
     }
+    if (name.find("af::k_two_untangle<") == 0 || name.find("af::k_two_entangle<") == 0) {
+        // (p_re, p_im, [q_re, q_im,] o1_re, o1_im, [o2_re, o2_im,] n, rows): every
+        // non-null array holds rows * n elements (components or interleaved pairs)
+        const bool unt = name.find("untangle") != std::string::npos;
+        const void *ptr[6];
+        for (int i = 0; i < 6; ++i) ptr[i] = *(const void *const *)args[i];
+        const int64_t n = *(const int64_t *)args[6], rows = *(const int64_t *)args[7];
+        return [=] {
+            // pairs (re, im): interleaved when im is null
+            for (int pr = 0; pr < 3; ++pr) {
+                const void *re = ptr[2 * pr], *im = ptr[2 * pr + 1];
+                if (!re) {
+                    if (im) problem(name + ": imaginary part without a real part");
+                    continue;
+                }
+                const std::string what = unt ? (pr == 0 ? "z" : pr == 1 ? "a" : "b") : (pr == 0 ? "c" : pr == 1 ? "d" : "z");
+                check_c(re, im, 0, rows * n, tsz, what.c_str(), name);
+            }
+            if ((int64_t)grid.x * block.x < n) problem(name + ": grid does not cover the row");
+        };
+    }
     if (name.find("af::k_convert<") == 0) {
         void *out = *(void **)args[0];
         const void *in = *(const void *const *)args[1];

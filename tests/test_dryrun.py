@@ -258,3 +258,39 @@ def test_autotune_candidates(dry):
     for n, batch, planar in ((65536, 64, True), (20000, 3, False)):
         assert run_c2c(dry, n, -1, 1, batch, planar) == 0
     assert problems(dry, before) == []
+
+
+def test_two_for_one_packing(dry):
+    """twoToSpectrum / twoFromSpectrum launches (the transform, the packing
+    kernels) for several lengths, layouts and precisions."""
+    dry.af_two_to_spectrum.argtypes = [V, ctypes.c_int64, V, V, V, V, V, V, V]
+    dry.af_two_from_spectrum.argtypes = [V, ctypes.c_int64, V, V, V, V, V, V, V]
+    dry.af_untangle_spectra.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, V, V, V, V, V, V, V]
+    dry.af_entangle_spectra.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, V, V, V, V, V, V, V]
+    before = dry.dry_problem_count()
+    for n in (1, 2, 7, 97, 1000, 4099, 65536, 100003):
+        for prec in (0, 1):
+            e = 16 if prec == 0 else 8
+            h = e // 2
+            batch = 3
+            d = PlanDesc(n, -1, C2C, prec, 0, 0)
+            p = V()
+            assert dry.af_plan_create(ctypes.byref(d), ctypes.byref(p)) == 0
+            b = Bufs(dry)
+            x, y = b(batch * n * h), b(batch * n * h)
+            for planar in (False, True):
+                if planar:
+                    a = (b(batch * n * h), b(batch * n * h))
+                    bb = (b(batch * n * h), b(batch * n * h))
+                else:
+                    a, bb = (b(batch * n * e), None), (b(batch * n * e), None)
+                assert dry.af_two_to_spectrum(p, batch, x, y, a[0], a[1], bb[0], bb[1], None) == 0
+                assert dry.af_two_from_spectrum(p, batch, a[0], a[1], bb[0], bb[1], x, y, None) == 0
+                z = b(batch * n * e)
+                assert dry.af_entangle_spectra(d.precision, batch, n, a[0], a[1], bb[0], bb[1], z, None, None) == 0
+                assert dry.af_untangle_spectra(d.precision, batch, n, z, None, a[0], a[1], bb[0], bb[1], None) == 0
+                # the untangling reads two positions per output: no aliasing
+                assert dry.af_untangle_spectra(d.precision, batch, n, z, None, z, None, bb[0], bb[1], None) != 0
+            b.free()
+            dry.af_plan_destroy(p)
+    assert problems(dry, before) == []

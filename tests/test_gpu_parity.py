@@ -368,3 +368,47 @@ def test_autotuned_transform(n, batch, layout, prec):
     assert key in text
     planned.wisdom_import(text)
     assert planned.wisdom_export() == text
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 97, 1000, 4099, 65536, 100003])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_two_for_one_real_packing(n, prec):
+    """twoToSpectrum / twoFromSpectrum (Real.hs:129-196) against the oracle,
+    planar and interleaved spectra, and the reference's properties
+    (Fourier.hs:503-536): the two spectra are those of x and y, and the inverse
+    transform of them gives back n x and n y."""
+    import oracle
+    rt = torch.float64 if prec == "f64" else torch.float32
+    ct = torch.complex128 if prec == "f64" else torch.complex64
+    tol = tol_f64(n) if prec == "f64" else 1e-6
+    x = synthetic.normal(31 + n, 3 * n).reshape(3, n)
+    y = synthetic.normal(41 + n, 3 * n).reshape(3, n)
+    ra, rb = oracle.two_to_spectrum(-1, x, y)
+    a, b = real.two_to_spectrum(planned.transform(forward, n), to_dev(x, rt), to_dev(y, rt))
+    assert rel_l2(host(a), ra) <= tol and rel_l2(host(b), rb) <= tol
+    ap = (torch.empty(3, n, dtype=rt, device=DEV), torch.empty(3, n, dtype=rt, device=DEV))
+    bp = (torch.empty(3, n, dtype=rt, device=DEV), torch.empty(3, n, dtype=rt, device=DEV))
+    real.two_to_spectrum(planned.transform(forward, n), to_dev(x, rt), to_dev(y, rt), out=(ap, bp))
+    assert rel_l2(host(ap[0]) + 1j * host(ap[1]), ra) <= tol
+    assert rel_l2(host(bp[0]) + 1j * host(bp[1]), rb) <= tol
+    xs, ys = real.two_from_spectrum(planned.transform(inverse, n), to_dev(ra, ct), to_dev(rb, ct))
+    ox, oy = oracle.two_from_spectrum(1, ra, rb)
+    assert rel_l2(host(xs), ox) <= tol and rel_l2(host(ys), oy) <= tol
+    assert rel_l2(host(xs), n * x) <= 10 * tol and rel_l2(host(ys), n * y) <= 10 * tol
+
+
+def test_untangle_entangle_elementwise():
+    """untangleSpectra / entangleSpectra alone: the coefficient formulas of
+    Real.hs:182-187 and 231-236 on arbitrary complex data (FP64 exact up to one
+    rounding), and untangle . entangle = id on spectra of real signals."""
+    n = 1001
+    rng = np.random.default_rng(5)
+    z = rng.standard_normal((4, n)) + 1j * rng.standard_normal((4, n))
+    a, b = real.untangle_spectra(to_dev(z, torch.complex128))
+    zr = np.conj(z[:, (-np.arange(n)) % n])
+    assert rel_l2(host(a), (z + zr) / 2) <= 1e-15 and rel_l2(host(b), (z - zr) * (-0.5j)) <= 1e-15
+    sa, sb = np.fft.fft(rng.standard_normal((4, n))), np.fft.fft(rng.standard_normal((4, n)))
+    e = real.entangle_spectra(to_dev(sa, torch.complex128), to_dev(sb, torch.complex128))
+    assert rel_l2(host(e), sa + 1j * sb) <= 1e-15
+    a2, b2 = real.untangle_spectra(e)
+    assert rel_l2(host(a2), sa) <= 1e-14 and rel_l2(host(b2), sb) <= 1e-14

@@ -272,3 +272,32 @@ def test_planted_mistake_is_caught():
     tw_bad = np.exp(+2j * np.pi * np.arange(n // 2) / n)
     bad = np.concatenate([e + tw_bad * o, e - tw_bad * o])
     assert rel(bad, O.dft_bruteforce(-1, x)) > 1e-3
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 10, 97, 100, 1000])
+def test_two_for_one_packing(n):
+    """Real.hs:129-236 against closed forms and the reference's own properties
+    (Fourier.hs:503-536): the untangled spectra of x + i y are the spectra of x
+    and y (numpy and the oracle's transform), untangling an entangled pair of
+    real spectra gives it back, and twoFromSpectrum with the inverse transform
+    undoes twoToSpectrum up to the factor n."""
+    rng = np.random.default_rng(n)
+    x, y = rng.standard_normal((3, n)), rng.standard_normal((3, n))
+    a, b = O.two_to_spectrum(-1, x, y)
+    np.testing.assert_allclose(a, np.fft.fft(x, axis=1), rtol=0, atol=1e-11 * max(n, 1))
+    np.testing.assert_allclose(b, np.fft.fft(y, axis=1), rtol=0, atol=1e-11 * max(n, 1))
+    np.testing.assert_allclose(a, O.transform(-1, x + 0j), rtol=0, atol=1e-12 * max(n, 1))
+    a2, b2 = O.untangle_spectra(O.entangle_spectra(a, b))
+    np.testing.assert_allclose(a2, a, rtol=0, atol=1e-12 * max(n, 1))
+    np.testing.assert_allclose(b2, b, rtol=0, atol=1e-12 * max(n, 1))
+    xs, ys = O.two_from_spectrum(1, a, b)
+    np.testing.assert_allclose(xs, n * x, rtol=0, atol=1e-10 * max(n, 1))
+    np.testing.assert_allclose(ys, n * y, rtol=0, atol=1e-10 * max(n, 1))
+    # the coefficient formulas themselves, on arbitrary (non-Hermitian) input:
+    # (z + conj z_-k) / 2 and (z - conj z_-k) (-i / 2)
+    z = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    ua, ub = O.untangle_spectra(z)
+    zr = np.conj(z[:, (-np.arange(n)) % n])
+    np.testing.assert_allclose(ua, (z + zr) / 2, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(ub, (z - zr) * (-0.5j), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(O.entangle_spectra(a, b), a + 1j * b, rtol=0, atol=1e-12 * max(n, 1))
