@@ -60,6 +60,12 @@ _SIGNATURES = {
     "af_two_to_spectrum": (ctypes.c_int, [_v, _i64, _v, _v, _v, _v, _v, _v, _v]),
     "af_two_from_spectrum": (ctypes.c_int, [_v, _i64, _v, _v, _v, _v, _v, _v, _v]),
     "af_untangle_spectra": (ctypes.c_int, [_i32, _i64, _i64, _v, _v, _v, _v, _v, _v, _v]),
+    "af_execute_2d": (ctypes.c_int, [_v, _v, _i64, _v, _v, _v, _v, _v]),
+    "af_execute_3d": (ctypes.c_int, [_v, _v, _v, _i64, _v, _v, _v, _v, _v]),
+    "af_transpose": (ctypes.c_int, [_i32, _i64, _i64, _i64, _v, _v, _v, _v, _v]),
+    "af_two_to_spectrum_2d": (ctypes.c_int, [_v, _v, _i64, _v, _v, _v, _v, _v, _v, _v]),
+    "af_two_from_spectrum_2d": (ctypes.c_int, [_v, _v, _i64, _v, _v, _v, _v, _v, _v, _v]),
+    "af_untangle_spectra_2d": (ctypes.c_int, [_i32, _i64, _i64, _i64, _v, _v, _v, _v, _v, _v, _v]),
     "af_entangle_spectra": (ctypes.c_int, [_i32, _i64, _i64, _v, _v, _v, _v, _v, _v, _v]),
     "af_plan_tune": (ctypes.c_int, [_v, _i64, _i32, _v, ctypes.c_char_p, ctypes.c_size_t]),
     "af_wisdom_export": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),

@@ -124,6 +124,87 @@ class Transform:
         return out
 
 
+class Transform2d:
+    """``Preprocessed.transform2d (SubTransformPair t0 t1)`` (Preprocessed.hs:141-148):
+    t0 along the last axis, t1 along the one before, over arrays (..., H, W)."""
+
+    def __init__(self, t0: Transform, t1: Transform):
+        self.t0, self.t1 = t0, t1
+
+    def __call__(self, x, out=None):
+        x = _arrays.materialize(x)
+        re, im, shape = _arrays.complex_view(x)
+        if len(shape) < 2 or shape[-1] != self.t0.n or shape[-2] != self.t1.n:
+            raise ValueError(f"array shape {shape} does not end in ({self.t1.n}, {self.t0.n})")
+        prec = _arrays.precision_of_complex(x)
+        if out is None:
+            out = _arrays.empty_like_complex(x)
+        ore, oim = _arrays.ptrs(out)
+        batch = _arrays.batch_of(shape[:-1])
+        _abi.check(_abi.lib().af_execute_2d(self.t0.plan(prec).handle, self.t1.plan(prec).handle, batch, re, im, ore,
+                                            oim, _arrays.stream_handle()), "af_execute_2d")
+        return out
+
+
+class Transform3d:
+    """``Preprocessed.transform3d (SubTransformTriple t0 t1 t2)`` (Preprocessed.hs:150-161):
+    t0, t1, t2 along the last, second-to-last and third-to-last axes."""
+
+    def __init__(self, t0: Transform, t1: Transform, t2: Transform):
+        self.t0, self.t1, self.t2 = t0, t1, t2
+
+    def __call__(self, x, out=None):
+        x = _arrays.materialize(x)
+        re, im, shape = _arrays.complex_view(x)
+        if len(shape) < 3 or shape[-3:] != (self.t2.n, self.t1.n, self.t0.n):
+            raise ValueError(f"array shape {shape} does not end in ({self.t2.n}, {self.t1.n}, {self.t0.n})")
+        prec = _arrays.precision_of_complex(x)
+        if out is None:
+            out = _arrays.empty_like_complex(x)
+        ore, oim = _arrays.ptrs(out)
+        batch = _arrays.batch_of(shape[:-2])
+        _abi.check(_abi.lib().af_execute_3d(self.t0.plan(prec).handle, self.t1.plan(prec).handle,
+                                            self.t2.plan(prec).handle, batch, re, im, ore, oim,
+                                            _arrays.stream_handle()), "af_execute_3d")
+        return out
+
+
+def transform2d(t0: Transform, t1: Transform) -> Transform2d:
+    return Transform2d(t0, t1)
+
+
+def transform3d(t0: Transform, t1: Transform, t2: Transform) -> Transform3d:
+    return Transform3d(t0, t1, t2)
+
+
+def _transpose_last2(x, rows, cols, out_shape):
+    x = _arrays.materialize(x)
+    re, im, shape = _arrays.complex_view(x)
+    prec = _arrays.precision_of_complex(x)
+    out = _arrays.empty_like_complex(x, out_shape)
+    ore, oim = _arrays.ptrs(out)
+    batch = 1
+    for d in shape:
+        batch *= d
+    batch //= max(rows * cols, 1)
+    _abi.check(_abi.lib().af_transpose(prec, batch, rows, cols, re, im, ore, oim, _arrays.stream_handle()),
+               "af_transpose")
+    return out
+
+
+def transpose(x):
+    """LinAlg.transpose of the last two axes: (..., h, w) -> (..., w, h)."""
+    shape = tuple(x[0].shape if isinstance(x, tuple) else x.shape)
+    return _transpose_last2(x, shape[-2], shape[-1], shape[:-2] + (shape[-1], shape[-2]))
+
+
+def cycle_dim3(x):
+    """Fourier.cycleDim3 (Private.hs:414-425): (..., k, m, n) -> (..., n, k, m)."""
+    shape = tuple(x[0].shape if isinstance(x, tuple) else x.shape)
+    k, m, n = shape[-3:]
+    return _transpose_last2(x, k * m, n, shape[:-3] + (n, k, m))
+
+
 def wisdom_export() -> str:
     """The autotuner's choices in this process, as text."""
     need = ctypes.c_size_t(0)

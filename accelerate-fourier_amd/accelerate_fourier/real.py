@@ -219,3 +219,61 @@ def entangle_spectra(c, d, out=None):
     _abi.check(_abi.lib().af_entangle_spectra(prec, _arrays.batch_of(shape), shape[-1], cre, cim, dre, dim, zre, zim,
                                               _arrays.stream_handle()), "af_entangle_spectra")
     return out
+
+
+def two_to_spectrum_2d(t0: Transform, t1: Transform, x, y, out=None):
+    """Real.twoToSpectrum2d (transform2d t0 t1) (Real.hs:138-145) over (..., H, W)."""
+    x, y = _arrays.materialize(x), _arrays.materialize(y)
+    _check_real(x, "x")
+    _check_real(y, "y")
+    _two_shape(x, y, "signals")
+    if x.dim() < 2 or x.shape[-1] != t0.n or x.shape[-2] != t1.n:
+        raise ValueError("the sub-transforms must match the last two axes")
+    prec = _real_prec(x)
+    if out is None:
+        ct = _arrays.complex_dtype(prec)
+        out = (torch.empty(x.shape, dtype=ct, device=x.device), torch.empty(x.shape, dtype=ct, device=x.device))
+    are, aim = _arrays.ptrs(out[0])
+    bre, bim = _arrays.ptrs(out[1])
+    _abi.check(_abi.lib().af_two_to_spectrum_2d(t0.plan(prec).handle, t1.plan(prec).handle,
+                                                _arrays.batch_of(tuple(x.shape)[:-1]), x.data_ptr(), y.data_ptr(),
+                                                are, aim, bre, bim, _arrays.stream_handle()), "af_two_to_spectrum_2d")
+    return out
+
+
+def two_from_spectrum_2d(t0: Transform, t1: Transform, a, b, out=None):
+    """Real.twoFromSpectrum2d (transform2d t0 t1) (Real.hs:199-206)."""
+    a, b = _arrays.materialize(a), _arrays.materialize(b)
+    are, aim, shape = _arrays.complex_view(a, "a")
+    bre, bim, shape_b = _arrays.complex_view(b, "b")
+    if tuple(shape) != tuple(shape_b):
+        raise ValueError("the two spectra must have equal shapes")
+    if len(shape) < 2 or shape[-1] != t0.n or shape[-2] != t1.n:
+        raise ValueError("the sub-transforms must match the last two axes")
+    prec = _arrays.precision_of_complex(a)
+    if out is None:
+        dev = a[0].device if isinstance(a, tuple) else a.device
+        rt = _arrays.real_dtype(prec)
+        out = (torch.empty(shape, dtype=rt, device=dev), torch.empty(shape, dtype=rt, device=dev))
+    _abi.check(_abi.lib().af_two_from_spectrum_2d(t0.plan(prec).handle, t1.plan(prec).handle,
+                                                  _arrays.batch_of(shape[:-1]), are, aim, bre, bim,
+                                                  out[0].data_ptr(), out[1].data_ptr(), _arrays.stream_handle()),
+               "af_two_from_spectrum_2d")
+    return out
+
+
+def untangle_spectra_2d(z, out=None):
+    """Real.untangleSpectra2d (Real.hs:175-180): partners at Cyclic.reverse2d."""
+    z = _arrays.materialize(z)
+    zre, zim, shape = _arrays.complex_view(z, "z")
+    if len(shape) < 2:
+        raise ValueError("untangleSpectra2d needs at least two axes")
+    prec = _arrays.precision_of_complex(z)
+    if out is None:
+        out = (_arrays.empty_like_complex(z), _arrays.empty_like_complex(z))
+    are, aim = _arrays.ptrs(out[0])
+    bre, bim = _arrays.ptrs(out[1])
+    _abi.check(_abi.lib().af_untangle_spectra_2d(prec, _arrays.batch_of(shape[:-1]), shape[-2], shape[-1], zre, zim,
+                                                 are, aim, bre, bim, _arrays.stream_handle()),
+               "af_untangle_spectra_2d")
+    return out

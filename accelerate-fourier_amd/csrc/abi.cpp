@@ -954,6 +954,40 @@ af_status run_c2c(af_plan P, int sign, int64_t batch, const void *in_re, const v
     return launch_all(P->prec, ps, s);
 }
 
+// Transform along the middle axis of an (outer, n, inner) array with the C2C
+// plan P (its own sign).  inner == 1 is the plain batched transform; a plain
+// single-pass core runs one strided pass (lines adjacent, elements `inner`
+// apart, the geometry of a four-step column pass, in place allowed); any other
+// plan transposes to (outer, inner, n), transforms contiguous rows and
+// transposes back.
+af_status run_axis(af_plan P, int64_t outer, int64_t inner, const void *in_re, const void *in_im, void *out_re,
+                   void *out_im, hipStream_t s) {
+    const int64_t n = P->shape.n;
+    if (n == 0 || outer == 0 || inner == 0) return AF_SUCCESS;
+    const int prec = P->prec;
+    if (inner == 1 || n == 1)  // contiguous rows (n == 1: rows of one element)
+        return run_c2c(P, P->desc.sign, outer * (n == 1 ? inner : 1), in_re, in_im, out_re, out_im, nullptr, s);
+    if (P->shape.algo == ALG_STOCKHAM && P->core.sh.kind == CORE_SINGLE && n * inner < ((int64_t)1 << 31)) {
+        int B = choose_B_cols(prec, n);
+        while (B > 1 && B > inner) B /= 2;
+        AfPass d = base_pass(prec, (int)n, P->core.sh.r, P->core.tw, P->core.tw_d, B, inner, outer, P->desc.sign,
+                             P->plain);
+        set_in(d, ep(in_re, in_im, n * inner), inner, 1);
+        set_out(d, ep(out_re, out_im, n * inner), inner, 1);
+        d.in_lines_fast = d.out_lines_fast = 1;
+        HIPCHK(launch_pass(prec, d, s));
+        return AF_SUCCESS;
+    }
+    const size_t eb = (size_t)elem_bytes(P->desc.precision), tot = (size_t)(outer * n * inner);
+    Scratch sc;
+    AFCHK(sc.get(2 * tot * eb, nullptr, s));
+    void *t1 = sc.p, *t2 = (char *)sc.p + tot * eb;
+    HIPCHK(launch_transpose(prec, in_re, in_im, t1, nullptr, n, inner, outer, s));
+    AFCHK(run_c2c(P, P->desc.sign, outer * inner, t1, nullptr, t2, nullptr, nullptr, s));
+    HIPCHK(launch_transpose(prec, t2, nullptr, out_re, out_im, inner, n, outer, s));
+    return AF_SUCCESS;
+}
+
 bool bad_ptrs(const void *a, const void *b) { return !a || !b; }
 
 // Unfused real kinds: the half-length C2C plan P->sub plus the packing kernels.
@@ -1362,6 +1396,95 @@ af_status af_entangle_spectra(int32_t precision, int64_t batch, int64_t n, const
     AFCHK(device_init());
     HIPCHK(launch_two_pack(precision == AF_F64 ? PREC_F64 : PREC_F32, false, c_re, c_im, d_re, d_im, z_re, z_im,
                            nullptr, nullptr, n, batch, (hipStream_t)stream));
+    return AF_SUCCESS;
+}
+
+// ---- multi-dimensional transforms (Preprocessed.hs:138-161, Adhoc.hs:365-380) ----
+
+static bool plans_c2c(std::initializer_list<af_plan> ps) {
+    int prec = -1;
+    for (af_plan p : ps) {
+        if (!p || p->desc.kind != AF_C2C) return false;
+        if (prec >= 0 && p->desc.precision != prec) return false;
+        prec = p->desc.precision;
+    }
+    return true;
+}
+
+af_status af_execute_2d(af_plan inner, af_plan outer, int64_t batch, const void *in_re, const void *in_im,
+                        void *out_re, void *out_im, void *stream) {
+    if (!plans_c2c({inner, outer}) || batch < 0) return AF_INVALID_ARG;
+    const int64_t W = inner->desc.length, H = outer->desc.length;
+    if (batch == 0 || W == 0 || H == 0) return AF_SUCCESS;
+    if (!in_re || !out_re) return AF_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    AFCHK(run_c2c(inner, inner->desc.sign, batch * H, in_re, in_im, out_re, out_im, nullptr, s));
+    return run_axis(outer, batch, W, out_re, out_im, out_re, out_im, s);
+}
+
+af_status af_execute_3d(af_plan p0, af_plan p1, af_plan p2, int64_t batch, const void *in_re, const void *in_im,
+                        void *out_re, void *out_im, void *stream) {
+    if (!plans_c2c({p0, p1, p2}) || batch < 0) return AF_INVALID_ARG;
+    const int64_t N = p0->desc.length, M = p1->desc.length, K = p2->desc.length;
+    if (batch == 0 || N == 0 || M == 0 || K == 0) return AF_SUCCESS;
+    if (!in_re || !out_re) return AF_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    AFCHK(run_c2c(p0, p0->desc.sign, batch * K * M, in_re, in_im, out_re, out_im, nullptr, s));
+    AFCHK(run_axis(p1, batch * K, N, out_re, out_im, out_re, out_im, s));
+    return run_axis(p2, batch, M * N, out_re, out_im, out_re, out_im, s);
+}
+
+af_status af_transpose(int32_t precision, int64_t batch, int64_t rows, int64_t cols, const void *in_re,
+                       const void *in_im, void *out_re, void *out_im, void *stream) {
+    if ((precision != AF_F64 && precision != AF_F32) || batch < 0 || rows < 0 || cols < 0) return AF_INVALID_ARG;
+    if (batch == 0 || rows == 0 || cols == 0) return AF_SUCCESS;
+    if (!in_re || !out_re || in_re == out_re || (in_im && in_im == out_im)) return AF_INVALID_ARG;
+    if ((rows + 31) / 32 > 65535) return AF_UNSUPPORTED;
+    AFCHK(device_init());
+    HIPCHK(launch_transpose(precision == AF_F64 ? PREC_F64 : PREC_F32, in_re, in_im, out_re, out_im, rows, cols, batch,
+                            (hipStream_t)stream));
+    return AF_SUCCESS;
+}
+
+af_status af_two_to_spectrum_2d(af_plan inner, af_plan outer, int64_t batch, const void *x, const void *y,
+                                void *a_re, void *a_im, void *b_re, void *b_im, void *stream) {
+    if (!plans_c2c({inner, outer}) || batch < 0) return AF_INVALID_ARG;
+    const int64_t W = inner->desc.length, H = outer->desc.length;
+    if (batch == 0 || W == 0 || H == 0) return AF_SUCCESS;
+    if (!x || !y || !a_re || !b_re) return AF_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    Scratch sz;
+    AFCHK(sz.get((size_t)(batch * H * W) * elem_bytes(inner->desc.precision), nullptr, s));
+    AFCHK(af_execute_2d(inner, outer, batch, x, y, sz.p, nullptr, stream));
+    HIPCHK(launch_two_pack(inner->prec, true, sz.p, nullptr, nullptr, nullptr, a_re, a_im, b_re, b_im, W, batch * H,
+                           s, H));
+    return AF_SUCCESS;
+}
+
+af_status af_two_from_spectrum_2d(af_plan inner, af_plan outer, int64_t batch, const void *a_re, const void *a_im,
+                                  const void *b_re, const void *b_im, void *x, void *y, void *stream) {
+    if (!plans_c2c({inner, outer}) || batch < 0) return AF_INVALID_ARG;
+    const int64_t W = inner->desc.length, H = outer->desc.length;
+    if (batch == 0 || W == 0 || H == 0) return AF_SUCCESS;
+    if (!x || !y || !a_re || !b_re) return AF_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    Scratch sz;
+    AFCHK(sz.get((size_t)(batch * H * W) * elem_bytes(inner->desc.precision), nullptr, s));
+    HIPCHK(launch_two_pack(inner->prec, false, a_re, a_im, b_re, b_im, sz.p, nullptr, nullptr, nullptr, W, batch * H,
+                           s));
+    return af_execute_2d(inner, outer, batch, sz.p, nullptr, x, y, stream);
+}
+
+af_status af_untangle_spectra_2d(int32_t precision, int64_t batch, int64_t height, int64_t width, const void *z_re,
+                                 const void *z_im, void *a_re, void *a_im, void *b_re, void *b_im, void *stream) {
+    if ((precision != AF_F64 && precision != AF_F32) || batch < 0 || height < 0 || width < 0) return AF_INVALID_ARG;
+    if (batch == 0 || height == 0 || width == 0) return AF_SUCCESS;
+    if (!z_re || !a_re || !b_re) return AF_INVALID_ARG;
+    for (const void *o : {(const void *)a_re, (const void *)a_im, (const void *)b_re, (const void *)b_im})
+        if (o && (o == z_re || o == z_im)) return AF_INVALID_ARG;
+    AFCHK(device_init());
+    HIPCHK(launch_two_pack(precision == AF_F64 ? PREC_F64 : PREC_F32, true, z_re, z_im, nullptr, nullptr, a_re, a_im,
+                           b_re, b_im, width, batch * height, (hipStream_t)stream, height));
     return AF_SUCCESS;
 }
 

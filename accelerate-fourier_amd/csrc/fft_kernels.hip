@@ -321,15 +321,20 @@ __global__ void k_convert(WT *__restrict__ out, const double2 *__restrict__ in, 
 
 // untangleSpectra (Real.hs:168-187): for the spectrum z of x + i y,
 // a_k = (z_k + conj z_-k) / 2 and b_k = (z_k - conj z_-k) (-i / 2).
+// With h > 1 the rows form h x n matrices and the partner of (y, k) is
+// ((-y) mod h, (-k) mod n): untangleSpectra2d with Cyclic.reverse2d
+// (Real.hs:175-180, Cyclic.hs:30-39).
 template <typename T>
 __global__ void k_two_untangle(const void *z_re, const void *z_im, void *a_re, void *a_im, void *b_re, void *b_im,
-                               int64_t n, int64_t rows) {
+                               int64_t n, int64_t rows, int64_t h) {
     using C = typename Cx<T>::C;
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int64_t kr = k ? n - k : 0;
     for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {
-        const C z = ld_c<T>(z_re, z_im, r * n + k), w = ld_c<T>(z_re, z_im, r * n + kr);
+        const int64_t slab = r / h, y = r - slab * h;
+        const int64_t rr = slab * h + (y ? h - y : 0);
+        const C z = ld_c<T>(z_re, z_im, r * n + k), w = ld_c<T>(z_re, z_im, rr * n + kr);
         const T h = T(0.5);
         // (z - conj w) (-i / 2) = ((z.im + w.im) / 2, -(z.re - w.re) / 2)
         st_c<T>(a_re, a_im, r * n + k, mk((z.x + w.x) * h, (z.y - w.y) * h));
@@ -347,6 +352,36 @@ __global__ void k_two_entangle(const void *c_re, const void *c_im, const void *d
     for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {
         const C c = ld_c<T>(c_re, c_im, r * n + k), d = ld_c<T>(d_re, d_im, r * n + k);
         st_c<T>(z_re, z_im, r * n + k, mk(c.x - d.y, c.y + d.x));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched transpose of R x C complex matrices (the reference's
+// LinAlg.transpose over the last two axes; with R = K M and C = N it is
+// cycleDim3, Private.hs:414-425).  32 x 32 tiles through LDS (one pad column
+// against bank conflicts), 32 x 8 threads, matrices of the batch over z.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_transpose(const void *in_re, const void *in_im, void *out_re, void *out_im, int64_t R, int64_t C, int64_t nb) {
+    using C2 = typename Cx<T>::C;
+    __shared__ C2 tile[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int64_t c0 = (int64_t)blockIdx.x * 32, r0 = (int64_t)blockIdx.y * 32;
+    for (int64_t b = blockIdx.z; b < nb; b += gridDim.z) {
+        const int64_t base = b * R * C;
+#pragma unroll
+        for (int k = 0; k < 32; k += 8) {
+            const int64_t r = r0 + ty + k, c = c0 + tx;
+            if (r < R && c < C) tile[ty + k][tx] = ld_c<T>(in_re, in_im, base + r * C + c);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 32; k += 8) {
+            const int64_t c = c0 + ty + k, r = r0 + tx;
+            if (r < R && c < C) st_c<T>(out_re, out_im, base + c * R + r, tile[tx][ty + k]);
+        }
+        __syncthreads();
     }
 }
 
@@ -458,20 +493,33 @@ hipError_t launch_real_conv_vjp(int prec, const AfPass &d, const void *x, const 
 
 hipError_t launch_two_pack(int prec, bool untangle, const void *p_re, const void *p_im, const void *q_re,
                            const void *q_im, void *o1_re, void *o1_im, void *o2_re, void *o2_im, int64_t n,
-                           int64_t rows, hipStream_t s) {
+                           int64_t rows, hipStream_t s, int64_t h) {
     if (n <= 0 || rows <= 0) return hipSuccess;
+    if (h < 1 || rows % h) return hipErrorInvalidValue;
     const dim3 g(grid_for(n, 256), (unsigned)std::min<int64_t>(rows, 65535)), b(256);
     if (untangle) {
         if (prec == PREC_F64)
-            hipLaunchKernelGGL(k_two_untangle<double>, g, b, 0, s, p_re, p_im, o1_re, o1_im, o2_re, o2_im, n, rows);
+            hipLaunchKernelGGL(k_two_untangle<double>, g, b, 0, s, p_re, p_im, o1_re, o1_im, o2_re, o2_im, n, rows, h);
         else
-            hipLaunchKernelGGL(k_two_untangle<float>, g, b, 0, s, p_re, p_im, o1_re, o1_im, o2_re, o2_im, n, rows);
+            hipLaunchKernelGGL(k_two_untangle<float>, g, b, 0, s, p_re, p_im, o1_re, o1_im, o2_re, o2_im, n, rows, h);
     } else {
         if (prec == PREC_F64)
             hipLaunchKernelGGL(k_two_entangle<double>, g, b, 0, s, p_re, p_im, q_re, q_im, o1_re, o1_im, n, rows);
         else
             hipLaunchKernelGGL(k_two_entangle<float>, g, b, 0, s, p_re, p_im, q_re, q_im, o1_re, o1_im, n, rows);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose(int prec, const void *in_re, const void *in_im, void *out_re, void *out_im, int64_t R,
+                            int64_t C, int64_t nb, hipStream_t s) {
+    if (R <= 0 || C <= 0 || nb <= 0) return hipSuccess;
+    if ((C + 31) / 32 > INT32_MAX || (R + 31) / 32 > 65535) return hipErrorInvalidValue;
+    const dim3 g((unsigned)((C + 31) / 32), (unsigned)((R + 31) / 32), (unsigned)std::min<int64_t>(nb, 65535));
+    if (prec == PREC_F64)
+        hipLaunchKernelGGL(k_transpose<double>, g, dim3(256), 0, s, in_re, in_im, out_re, out_im, R, C, nb);
+    else
+        hipLaunchKernelGGL(k_transpose<float>, g, dim3(256), 0, s, in_re, in_im, out_re, out_im, R, C, nb);
     return hipGetLastError();
 }
 

@@ -34,10 +34,17 @@ hipError_t launch_pack(int prec, int op, const void *a, const void *a_im, const 
 
 // Two-for-one real packing, elementwise over rows of n (fft_kernels.hip):
 // untangle = true: (o1, o2) = untangleSpectra p; false: o1 = entangleSpectra (p, q).
-// Each complex array is planar (re, im) or interleaved (im == nullptr).
+// Each complex array is planar (re, im) or interleaved (im == nullptr); with
+// h > 1 the rows form h x n matrices (the 2-D forms, untangling against
+// reverse2d).
 hipError_t launch_two_pack(int prec, bool untangle, const void *p_re, const void *p_im, const void *q_re,
                            const void *q_im, void *o1_re, void *o1_im, void *o2_re, void *o2_im, int64_t n,
-                           int64_t rows, hipStream_t s);
+                           int64_t rows, hipStream_t s, int64_t h = 1);
+
+// out[b][c][r] = in[b][r][c] for nb matrices of R x C complex elements (planar
+// or interleaved endpoints; out must not share memory with in).
+hipError_t launch_transpose(int prec, const void *in_re, const void *in_im, void *out_re, void *out_im, int64_t R,
+                            int64_t C, int64_t nb, hipStream_t s);
 
 // Allow every kernel the full 160 KiB of LDS (called once).
 void init_kernel_attributes();

@@ -175,6 +175,43 @@ af_status af_entangle_spectra(int32_t precision, int64_t batch, int64_t n, const
                               const void *d_re, const void *d_im, void *z_re, void *z_im, void *stream);
 
 /*
+ * Multi-dimensional transforms over the last two / three axes, each axis with
+ * its own C2C plan (same precision), batch leading slabs.
+ * af_execute_2d replaces Preprocessed.transform2d (SubTransformPair t0 t1)
+ * (Preprocessed.hs:141-148; Adhoc.transform2d, Adhoc.hs:365-371, with one
+ * sub-transform for both axes): t0 = `inner` along the last axis (length W),
+ * t1 = `outer` along the one before (length H); arrays (batch, H, W).
+ * af_execute_3d replaces Preprocessed.transform3d (SubTransformTriple t0 t1 t2)
+ * (Preprocessed.hs:150-161; Adhoc.hs:373-380): arrays (batch, K, M, N) with
+ * p0, p1, p2 of lengths N, M, K.  The reference transposes between axes
+ * (LinAlg.transpose, cycleDim3); here axes other than the last run as strided
+ * passes of plain single-pass plans, else through tiled transposes.  In place
+ * allowed; scratch is stream-ordered.
+ */
+af_status af_execute_2d(af_plan inner, af_plan outer, int64_t batch, const void *in_re, const void *in_im,
+                        void *out_re, void *out_im, void *stream);
+af_status af_execute_3d(af_plan p0, af_plan p1, af_plan p2, int64_t batch, const void *in_re, const void *in_im,
+                        void *out_re, void *out_im, void *stream);
+/*
+ * out[b][c][r] = in[b][r][c] for batch matrices of rows x cols complex values:
+ * LinAlg.transpose over the last two axes; with rows = K M and cols = N it is
+ * Fourier.cycleDim3 (Private.hs:414-425).  out must not share memory with in.
+ */
+af_status af_transpose(int32_t precision, int64_t batch, int64_t rows, int64_t cols, const void *in_re,
+                       const void *in_im, void *out_re, void *out_im, void *stream);
+/*
+ * 2-D two-for-one packing: Real.twoToSpectrum2d / twoFromSpectrum2d
+ * (Real.hs:138-145, 199-206) with the 2-D transform above, and
+ * untangleSpectra2d (Real.hs:175-180) against Cyclic.reverse2d (Cyclic.hs:30-39).
+ */
+af_status af_two_to_spectrum_2d(af_plan inner, af_plan outer, int64_t batch, const void *x, const void *y,
+                                void *a_re, void *a_im, void *b_re, void *b_im, void *stream);
+af_status af_two_from_spectrum_2d(af_plan inner, af_plan outer, int64_t batch, const void *a_re, const void *a_im,
+                                  const void *b_re, const void *b_im, void *x, void *y, void *stream);
+af_status af_untangle_spectra_2d(int32_t precision, int64_t batch, int64_t height, int64_t width, const void *z_re,
+                                 const void *z_im, void *a_re, void *a_im, void *b_re, void *b_im, void *stream);
+
+/*
  * Real signal to full spectrum (AF_R2C_FULL plans).
  * Replaces: Real.toSpectrum (Planned.transform sign (n/2)) (Real.hs:49-70).
  * x: batch x n reals; out: batch x n complex (the full, not halved, spectrum).

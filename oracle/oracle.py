@@ -321,3 +321,51 @@ def complex_karatsuba(x, y, cyclic: bool = False):
 
 def num_threads() -> int:
     return int(lib().orc_num_threads())
+
+
+# ---- multi-dimensional transforms: the reference's compositions of 1-D ones ----
+
+def transform2d(sign0: int, sign1: int, x: np.ndarray, precision: str = "f64") -> np.ndarray:
+    """Preprocessed.transform2d (SubTransformPair (transform sign0 w) (transform sign1 h))
+    (Preprocessed.hs:141-148): transpose . t1 . transpose . t0 over the last two axes."""
+    y = transform(sign0, np.asarray(x), precision=precision)
+    y = transform(sign1, np.ascontiguousarray(np.swapaxes(y, -1, -2)), precision=precision)
+    return np.ascontiguousarray(np.swapaxes(y, -1, -2))
+
+
+def cycle_dim3(x: np.ndarray) -> np.ndarray:
+    """Fourier.cycleDim3 (Private.hs:414-425): (..., k, m, n) -> (..., n, k, m)."""
+    return np.ascontiguousarray(np.moveaxis(np.asarray(x), -1, -3))
+
+
+def transform3d(sign0: int, sign1: int, sign2: int, x: np.ndarray, precision: str = "f64") -> np.ndarray:
+    """Preprocessed.transform3d (SubTransformTriple t0 t1 t2) (Preprocessed.hs:150-161):
+    cycleDim3 . t2 . cycleDim3 . t1 . cycleDim3 . t0."""
+    y = transform(sign0, np.asarray(x), precision=precision)
+    y = transform(sign1, cycle_dim3(y), precision=precision)
+    y = transform(sign2, cycle_dim3(y), precision=precision)
+    return cycle_dim3(y)
+
+
+def reverse2d(x: np.ndarray) -> np.ndarray:
+    """Cyclic.reverse2d (Cyclic.hs:30-39): (y, x) -> ((-y) mod h, (-x) mod w)."""
+    x = np.asarray(x)
+    h, w = x.shape[-2:]
+    return x[..., (-np.arange(h)) % h, :][..., (-np.arange(w)) % w]
+
+
+def untangle_spectra_2d(z: np.ndarray):
+    """Real.untangleSpectra2d (Real.hs:175-180) with untangleCoefficient (182-187)."""
+    bc = np.conj(reverse2d(z))
+    return (z + bc) / 2, (z - bc) * (-0.5j)
+
+
+def two_to_spectrum_2d(sign0: int, sign1: int, x: np.ndarray, y: np.ndarray):
+    """Real.twoToSpectrum2d (Real.hs:138-145)."""
+    return untangle_spectra_2d(transform2d(sign0, sign1, np.asarray(x) + 1j * np.asarray(y)))
+
+
+def two_from_spectrum_2d(sign0: int, sign1: int, a: np.ndarray, b: np.ndarray):
+    """Real.twoFromSpectrum2d (Real.hs:199-206) with entangleSpectra2d (208-216)."""
+    z = transform2d(sign0, sign1, entangle_spectra(a, b))
+    return np.real(z), np.imag(z)

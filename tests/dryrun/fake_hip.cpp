@@ -467,6 +467,8 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
         const void *ptr[6];
         for (int i = 0; i < 6; ++i) ptr[i] = *(const void *const *)args[i];
         const int64_t n = *(const int64_t *)args[6], rows = *(const int64_t *)args[7];
+        const int64_t h = unt ? *(const int64_t *)args[8] : 1;
+        if (h < 1 || rows % h) return [=] { problem(name + ": rows not a multiple of the matrix height"); };
         return [=] {
             // pairs (re, im): interleaved when im is null
             for (int pr = 0; pr < 3; ++pr) {
@@ -479,6 +481,17 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
                 check_c(re, im, 0, rows * n, tsz, what.c_str(), name);
             }
             if ((int64_t)grid.x * block.x < n) problem(name + ": grid does not cover the row");
+        };
+    }
+    if (name.find("af::k_transpose<") == 0) {
+        const void *ir = *(const void *const *)args[0], *ii = *(const void *const *)args[1];
+        const void *orr = *(const void *const *)args[2], *oi = *(const void *const *)args[3];
+        const int64_t R = *(const int64_t *)args[4], C = *(const int64_t *)args[5], nb = *(const int64_t *)args[6];
+        return [=] {
+            check_c(ir, ii, 0, nb * R * C, tsz, "in", name);
+            check_c(orr, oi, 0, nb * R * C, tsz, "out", name);
+            if ((int64_t)grid.x * 32 < C || (int64_t)grid.y * 32 < R) problem(name + ": grid does not cover the matrix");
+            if (block.x != 256) problem(name + ": the kernel assumes 32 x 8 threads");
         };
     }
     if (name.find("af::k_convert<") == 0) {

@@ -294,3 +294,47 @@ def test_two_for_one_packing(dry):
             b.free()
             dry.af_plan_destroy(p)
     assert problems(dry, before) == []
+
+
+def test_multidimensional(dry):
+    """2-D and 3-D transforms (strided passes and transposes) and their two-for-one
+    packing, every launch checked."""
+    dry.af_execute_2d.argtypes = [V, V, ctypes.c_int64, V, V, V, V, V]
+    dry.af_execute_3d.argtypes = [V, V, V, ctypes.c_int64, V, V, V, V, V]
+    dry.af_two_to_spectrum_2d.argtypes = [V, V, ctypes.c_int64, V, V, V, V, V, V, V]
+    dry.af_two_from_spectrum_2d.argtypes = [V, V, ctypes.c_int64, V, V, V, V, V, V, V]
+    before = dry.dry_problem_count()
+
+    def plan(n, prec):
+        p = V()
+        assert dry.af_plan_create(ctypes.byref(PlanDesc(n, -1, C2C, prec, 0, 0)), ctypes.byref(p)) == 0
+        return p
+
+    for prec in (0, 1):
+        e = 16 if prec == 0 else 8
+        for h, w in ((1, 1), (7, 5), (256, 1024), (4099, 16), (16, 4099), (65536, 2), (3, 65536), (1000, 3)):
+            ph, pw = plan(h, prec), plan(w, prec)
+            b = Bufs(dry)
+            for planar in (False, True):
+                if planar:
+                    io = (b(2 * h * w * e // 2), b(2 * h * w * e // 2), b(2 * h * w * e // 2), b(2 * h * w * e // 2))
+                else:
+                    io = (b(2 * h * w * e), None, b(2 * h * w * e), None)
+                assert dry.af_execute_2d(pw, ph, 2, *io, None) == 0
+                assert dry.af_execute_2d(pw, ph, 2, io[2], io[3], io[2], io[3], None) == 0  # in place
+            x, y = b(2 * h * w * e // 2), b(2 * h * w * e // 2)
+            a, bb = b(2 * h * w * e), b(2 * h * w * e)
+            assert dry.af_two_to_spectrum_2d(pw, ph, 2, x, y, a, None, bb, None, None) == 0
+            assert dry.af_two_from_spectrum_2d(pw, ph, 2, a, None, bb, None, x, y, None) == 0
+            b.free()
+            dry.af_plan_destroy(ph)
+            dry.af_plan_destroy(pw)
+        for k, m, n in ((2, 3, 5), (3, 4099, 2), (64, 32, 48)):
+            p0, p1, p2 = plan(n, prec), plan(m, prec), plan(k, prec)
+            b = Bufs(dry)
+            xin, xout = b(2 * k * m * n * e), b(2 * k * m * n * e)
+            assert dry.af_execute_3d(p0, p1, p2, 2, xin, None, xout, None, None) == 0
+            b.free()
+            for p in (p0, p1, p2):
+                dry.af_plan_destroy(p)
+    assert problems(dry, before) == []

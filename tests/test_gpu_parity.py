@@ -412,3 +412,82 @@ def test_untangle_entangle_elementwise():
     assert rel_l2(host(e), sa + 1j * sb) <= 1e-15
     a2, b2 = real.untangle_spectra(e)
     assert rel_l2(host(a2), sa) <= 1e-14 and rel_l2(host(b2), sb) <= 1e-14
+
+
+@pytest.mark.parametrize("h,w", [(1, 1), (7, 5), (64, 48), (256, 1024), (1000, 3), (4099, 16), (16, 4099),
+                                 (65536, 2), (3, 65536)])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_transform_2d(h, w, prec):
+    """Preprocessed.transform2d (Preprocessed.hs:141-148) against the oracle's
+    composition of 1-D transforms, both layouts, and the reference's property
+    transpose (transform2d (tw, th) x) = transform2d (th, tw) (transpose x)
+    (Fourier.hs:537-551) on the device."""
+    import oracle
+    ct = torch.complex128 if prec == "f64" else torch.complex64
+    rt = torch.float64 if prec == "f64" else torch.float32
+    tol = 1e-12 if prec == "f64" else 2e-6
+    x = synthetic.complex_uniform(h * 7 + w, (2, h, w))
+    ref = oracle.transform2d(-1, 1, x)
+    t = planned.transform2d(planned.transform(forward, w), planned.transform(inverse, h))
+    got = host(t(to_dev(x, ct)))
+    assert rel_l2(got, ref) <= tol
+    yp = t((to_dev(x.real, rt), to_dev(x.imag, rt)))
+    assert rel_l2(host(yp[0]) + 1j * host(yp[1]), ref) <= tol
+    tv = planned.transform2d(planned.transform(inverse, h), planned.transform(forward, w))
+    lhs = host(planned.transpose(t(to_dev(x, ct))))
+    rhs = host(tv(planned.transpose(to_dev(x, ct))))
+    assert rel_l2(lhs, rhs) <= tol
+
+
+@pytest.mark.parametrize("k,m,n", [(1, 1, 1), (2, 3, 5), (5, 12, 16), (3, 4099, 2), (64, 32, 48), (7, 2, 1000)])
+def test_transform_3d(k, m, n):
+    """Preprocessed.transform3d (Preprocessed.hs:150-161) against the oracle and
+    the property cycleDim3 (transform3d (tw, th, td) x) =
+    transform3d (th, td, tw) (cycleDim3 x) (Fourier.hs:560-575)."""
+    import oracle
+    x = synthetic.complex_uniform(k + m + n, (2, k, m, n))
+    for prec, ct, tol in (("f64", torch.complex128, 1e-12), ("f32", torch.complex64, 2e-6)):
+        t = planned.transform3d(planned.transform(forward, n), planned.transform(inverse, m),
+                                planned.transform(forward, k))
+        got = host(t(to_dev(x, ct)))
+        assert rel_l2(got, oracle.transform3d(-1, 1, -1, x)) <= tol, prec
+        tv = planned.transform3d(planned.transform(inverse, m), planned.transform(forward, k),
+                                 planned.transform(forward, n))
+        lhs = host(planned.cycle_dim3(t(to_dev(x, ct))))
+        rhs = host(tv(planned.cycle_dim3(to_dev(x, ct))))
+        assert rel_l2(lhs, rhs) <= tol, prec
+
+
+def test_transpose_and_cycle_dim3_exact():
+    """The data movements themselves, bit for bit."""
+    x = synthetic.complex_uniform(3, (3, 37, 70))
+    for ct in (torch.complex128, torch.complex64):
+        xd = to_dev(x, ct)
+        assert np.array_equal(host(planned.transpose(xd)), np.swapaxes(host(xd), -1, -2))
+        x3 = xd.reshape(3, 37, 7, 10)
+        assert np.array_equal(host(planned.cycle_dim3(x3)), np.moveaxis(host(x3), -1, -3))
+
+
+@pytest.mark.parametrize("h,w", [(1, 1), (6, 10), (9, 7), (64, 1000)])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_two_for_one_2d(h, w, prec):
+    """twoToSpectrum2d / twoFromSpectrum2d / untangleSpectra2d (Real.hs:138-206)
+    against the oracle."""
+    import oracle
+    rt = torch.float64 if prec == "f64" else torch.float32
+    ct = torch.complex128 if prec == "f64" else torch.complex64
+    tol = 1e-12 if prec == "f64" else 2e-6
+    x = synthetic.normal(h + w, 2 * h * w).reshape(2, h, w)
+    y = synthetic.normal(h * w + 1, 2 * h * w).reshape(2, h, w)
+    ra, rb = oracle.two_to_spectrum_2d(-1, -1, x, y)
+    tw, th = planned.transform(forward, w), planned.transform(forward, h)
+    a, b = real.two_to_spectrum_2d(tw, th, to_dev(x, rt), to_dev(y, rt))
+    assert rel_l2(host(a), ra) <= tol and rel_l2(host(b), rb) <= tol
+    iw, ih = planned.transform(inverse, w), planned.transform(inverse, h)
+    xs, ys = real.two_from_spectrum_2d(iw, ih, to_dev(ra, ct), to_dev(rb, ct))
+    ox, oy = oracle.two_from_spectrum_2d(1, 1, ra, rb)
+    assert rel_l2(host(xs), ox) <= tol and rel_l2(host(ys), oy) <= tol
+    z = synthetic.complex_uniform(h + 2 * w, (2, h, w))
+    ua, ub = real.untangle_spectra_2d(to_dev(z, ct))
+    oa, ob = oracle.untangle_spectra_2d(z)
+    assert rel_l2(host(ua), oa) <= tol and rel_l2(host(ub), ob) <= tol

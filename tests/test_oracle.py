@@ -301,3 +301,35 @@ def test_two_for_one_packing(n):
     np.testing.assert_allclose(ua, (z + zr) / 2, rtol=0, atol=1e-15)
     np.testing.assert_allclose(ub, (z - zr) * (-0.5j), rtol=0, atol=1e-15)
     np.testing.assert_allclose(O.entangle_spectra(a, b), a + 1j * b, rtol=0, atol=1e-12 * max(n, 1))
+
+
+@pytest.mark.parametrize("h,w", [(1, 1), (3, 5), (8, 6), (7, 16)])
+def test_multidimensional(h, w):
+    """Preprocessed.transform2d/3d, cycleDim3, reverse2d and the 2-D two-for-one
+    packing (restated as the reference's compositions of 1-D transforms) against
+    numpy's fft2/fftn, and the reference's transposition properties
+    (Fourier.hs:537-586)."""
+    rng = np.random.default_rng(h * 31 + w)
+    x = rng.standard_normal((2, h, w)) + 1j * rng.standard_normal((2, h, w))
+    np.testing.assert_allclose(O.transform2d(-1, -1, x), np.fft.fft2(x), rtol=0, atol=1e-12 * h * w)
+    np.testing.assert_allclose(O.transform2d(1, 1, x), np.fft.ifft2(x) * h * w, rtol=0, atol=1e-12 * h * w)
+    # transpose (transform2d (tw, th) x) == transform2d (th, tw) (transpose x)
+    np.testing.assert_allclose(np.swapaxes(O.transform2d(-1, 1, x), -1, -2),
+                               O.transform2d(1, -1, np.swapaxes(x, -1, -2)), rtol=0, atol=1e-12 * h * w)
+    d = 3
+    x3 = rng.standard_normal((2, d, h, w)) + 1j * rng.standard_normal((2, d, h, w))
+    np.testing.assert_allclose(O.transform3d(-1, -1, -1, x3), np.fft.fftn(x3, axes=(-3, -2, -1)), rtol=0,
+                               atol=1e-12 * d * h * w)
+    np.testing.assert_allclose(O.cycle_dim3(O.transform3d(-1, 1, -1, x3)),
+                               O.transform3d(1, -1, -1, O.cycle_dim3(x3)), rtol=0, atol=1e-12 * d * h * w)
+    assert O.cycle_dim3(x3).shape == (2, w, d, h)
+    assert np.array_equal(O.cycle_dim3(O.cycle_dim3(O.cycle_dim3(x3))), x3)
+    # reverse2d: the spectrum of conj(signal) is conj(reverse2d(spectrum))
+    np.testing.assert_allclose(np.fft.fft2(np.conj(x)), np.conj(O.reverse2d(np.fft.fft2(x))), rtol=0, atol=1e-11)
+    xr, yr = rng.standard_normal((2, h, w)), rng.standard_normal((2, h, w))
+    a, b = O.two_to_spectrum_2d(-1, -1, xr, yr)
+    np.testing.assert_allclose(a, np.fft.fft2(xr), rtol=0, atol=1e-11 * h * w)
+    np.testing.assert_allclose(b, np.fft.fft2(yr), rtol=0, atol=1e-11 * h * w)
+    xs, ys = O.two_from_spectrum_2d(1, 1, a, b)
+    np.testing.assert_allclose(xs, h * w * xr, rtol=0, atol=1e-10 * h * w)
+    np.testing.assert_allclose(ys, h * w * yr, rtol=0, atol=1e-10 * h * w)
