@@ -81,6 +81,7 @@ _SIGNATURES = {
     "af_karatsuba_i64": (ctypes.c_int, [_i64, _i64, _v, _v, _v, _v]),
     "af_karatsuba_ragged_i64": (ctypes.c_int, [_i64, _p64, _v, _v, _v, _v]),
     "af_karatsuba_cyclic_i64": (ctypes.c_int, [_i64, _i64, _v, _v, _v, _v]),
+    "af_karatsuba_complex_i64": (ctypes.c_int, [_i64, _i64, ctypes.c_int32, _v, _v, _v, _v, _v, _v, _v]),
     "af_batch_create": (ctypes.c_int, [_i64, _p64, _i32, _i32, ctypes.POINTER(_v)]),
     "af_batch_destroy": (ctypes.c_int, [_v]),
     "af_batch_execute": (ctypes.c_int, [_v, _v, _v, _v, _v, _v]),

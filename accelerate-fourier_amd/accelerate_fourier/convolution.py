@@ -4,6 +4,9 @@
 (Convolution/Adhoc.hs:36-64): equal-length rows, result length 2L-1 (0 for
 L = 0), wrap-around Int64 arithmetic, bit-exact on the GPU.
 ``cyclic(karatsuba)`` is ``Convolution.cyclic karatsuba`` (Adhoc.hs:104-111).
+``complex(karatsuba)`` and ``complex(cyclic(karatsuba))`` are
+``Convolution.complex`` over them (Adhoc.hs:114-133): complex Int64 operands
+as (re, im) pairs of int64 tensors.
 """
 from __future__ import annotations
 
@@ -57,14 +60,49 @@ def karatsuba_ragged(offsets, x, y, out=None):
 def cyclic(conv):
     """Convolution.cyclic: wrap the tail of a linear convolution (Adhoc.hs:104-111)."""
     if conv is karatsuba:
-        def cyc(x, y, out=None):
-            _check_pair(x, y)
-            L = x.shape[-1]
-            if out is None:
-                out = torch.empty_like(x)
-            _abi.check(_abi.lib().af_karatsuba_cyclic_i64(_arrays.batch_of(tuple(x.shape)), L, x.data_ptr(),
-                                                          y.data_ptr(), out.data_ptr(), _arrays.stream_handle()),
-                       "af_karatsuba_cyclic_i64")
-            return out
-        return cyc
+        return _cyclic_karatsuba
     raise NotImplementedError("cyclic wraps the GPU Karatsuba only")
+
+
+def complex(conv):
+    """Convolution.complex (Adhoc.hs:114-133): a real convolution made complex
+    by the Gauss form, re = xr*yr - xi*yi, im = (xr+xi)*(yr+yi) - xr*yr - xi*yi.
+    The operands and the result are (re, im) pairs of int64 tensors; wrap-around
+    arithmetic, bit-exact."""
+    if conv is karatsuba:
+        cyc = 0
+    elif conv is _cyclic_karatsuba:
+        cyc = 1
+    else:
+        raise NotImplementedError("complex wraps the GPU Karatsuba and its cyclic form only")
+
+    def cconv(x, y, out=None):
+        if not (isinstance(x, tuple) and isinstance(y, tuple) and len(x) == 2 and len(y) == 2):
+            raise TypeError("complex Int64 operands are (re, im) pairs of int64 tensors")
+        _check_pair(x[0], y[0])
+        _check_pair(x[1], y[1])
+        if x[0].shape != x[1].shape:
+            raise ValueError("real and imaginary parts must have the same shape")
+        L = x[0].shape[-1]
+        zl = L if cyc else (2 * L - 1 if L > 0 else 0)
+        if out is None:
+            shape = tuple(x[0].shape[:-1]) + (zl,)
+            out = (torch.empty(shape, dtype=torch.int64, device=x[0].device),
+                   torch.empty(shape, dtype=torch.int64, device=x[0].device))
+        _abi.check(_abi.lib().af_karatsuba_complex_i64(
+            _arrays.batch_of(tuple(x[0].shape)), L, cyc, x[0].data_ptr(), x[1].data_ptr(), y[0].data_ptr(),
+            y[1].data_ptr(), out[0].data_ptr(), out[1].data_ptr(), _arrays.stream_handle()),
+            "af_karatsuba_complex_i64")
+        return out
+    return cconv
+
+
+def _cyclic_karatsuba(x, y, out=None):
+    _check_pair(x, y)
+    L = x.shape[-1]
+    if out is None:
+        out = torch.empty_like(x)
+    _abi.check(_abi.lib().af_karatsuba_cyclic_i64(_arrays.batch_of(tuple(x.shape)), L, x.data_ptr(),
+                                                  y.data_ptr(), out.data_ptr(), _arrays.stream_handle()),
+               "af_karatsuba_cyclic_i64")
+    return out

@@ -239,6 +239,129 @@ def transform_chirp235(sign: Sign, n: int) -> Transform:
     return Transform(sign, n, _abi.ALG_CHIRP235)
 
 
+def ceiling_power_of_two(n: int) -> int:
+    """NumberTheory.ceilingPowerOfTwo: the least power of two >= n."""
+    return int(_abi.lib().af_ceiling_power_of_two(int(n)))
+
+
+def ceiling_5_smooth(n: int) -> int:
+    """NumberTheory.ceiling5Smooth: the least 5-smooth number >= n."""
+    return int(_abi.lib().af_ceiling_5smooth(int(n)))
+
+
+class Planner:
+    """A planning rule ``Integer -> State PlanMap Plan`` of Planned.hs.
+
+    The device planner knows the reference's three rules: ``planWithMapUpdate``
+    (Rader, or Bluestein where Rader would nest; Planned.hs:215-240),
+    ``planDecomposeWithMapUpdate`` (never Bluestein; 279-293) and
+    ``planChirpWithMapUpdate padLength`` (Bluestein with a pad of
+    ceilingPowerOfTwo or ceiling5Smooth of 2n-1; 731-737).  Each is one
+    ``algorithm`` of the C ABI's plan descriptor.  Calling a planner gives the
+    text of the device plan it makes for a length."""
+
+    def __init__(self, name: str, algorithm: int):
+        self.name = name
+        self.algorithm = algorithm
+
+    def __call__(self, n: int, precision=_abi.F32) -> str:
+        return describe(int(n), precision=precision, algorithm=self.algorithm)
+
+    def __repr__(self):
+        return f"Planner({self.name})"
+
+
+plan_with_map_update = Planner("planWithMapUpdate", _abi.ALG_AUTO)
+plan_decompose_with_map_update = Planner("planDecomposeWithMapUpdate", _abi.ALG_DECOMPOSE)
+
+
+def plan_chirp_with_map_update(pad_length) -> Planner:
+    """Planned.planChirpWithMapUpdate (Planned.hs:731-737) for the two pad
+    functions the reference's own entry points pass (transformChirp2 and
+    transformChirp235, Planned.hs:704-720)."""
+    if pad_length is ceiling_power_of_two:
+        return Planner("planChirpWithMapUpdate ceilingPowerOfTwo", _abi.ALG_CHIRP2)
+    if pad_length is ceiling_5_smooth:
+        return Planner("planChirpWithMapUpdate ceiling5Smooth", _abi.ALG_CHIRP235)
+    raise ValueError("the device planner pads Bluestein transforms with ceiling_power_of_two "
+                     "or ceiling_5_smooth only")
+
+
+def plan(n: int, precision=_abi.F32) -> str:
+    """Planned.plan (Planned.hs:201-203): the default plan for a length, as the
+    text of the device plan."""
+    return plan_with_map_update(n, precision)
+
+
+def transform_with_planner(planner: Planner, sign: Sign, n: int) -> Transform:
+    """Planned.transformWithPlanner (Planned.hs:121-130).  A planner is one of
+    the reference's planning rules above; an arbitrary function from lengths
+    to plans has no device counterpart and is refused."""
+    if not isinstance(planner, Planner):
+        raise TypeError("planner must be plan_with_map_update, plan_decompose_with_map_update "
+                        "or plan_chirp_with_map_update(pad)")
+    return Transform(sign, n, planner.algorithm)
+
+
+class Cache:
+    """Planned.Cache (Planned.hs:300-309): everything a transform of one
+    length and sign precomputes.  The device tables are built on first use for
+    each precision and shared with every other transform of the same length,
+    sign and algorithm in this process."""
+
+    def __init__(self, sign: Sign, n: int, algorithm: int = _abi.ALG_AUTO, adjoint_of: "Cache" = None):
+        self.sign = sign if isinstance(sign, Sign) else Sign(int(sign))
+        self.n = int(n)
+        self.algorithm = algorithm
+        self.adjoint_of = adjoint_of
+
+    def transform(self) -> Transform:
+        if self.adjoint_of is None:
+            return Transform(self.sign, self.n, self.algorithm)
+        return _AdjointTransform(self.adjoint_of.transform())
+
+
+class _AdjointTransform(Transform):
+    """The transform of the opposite sign run as the adjoint of another
+    transform's plan, so the two share every table: the unnormalised DFT
+    matrix is symmetric, so its adjoint is its conjugate, the DFT of the
+    opposite sign."""
+
+    def __init__(self, of: Transform):
+        super().__init__(of.sign.negate(), of.n, of.algorithm)
+        self._of = of
+
+    def __call__(self, x, out=None):
+        return self._of.vjp(x, out)
+
+    # plan() stays the opposite-sign plan of its own: 2-D and 3-D
+    # compositions run plans directly and need the right sign.
+
+    def tune(self, batch, precision="f32", layout="planar"):
+        raise NotImplementedError("tune the forward transform of the pair")
+
+    def vjp(self, g, out=None):
+        return self._of(g, out)
+
+
+def cache(sign: Sign, n: int) -> Cache:
+    """Planned.cache (Planned.hs:316-329)."""
+    return Cache(sign, n)
+
+
+def cache_duplex(n: int):
+    """Planned.cacheDuplex (Planned.hs:331-341): ``(cache inverse n, cache
+    forward n)`` sharing their data.  The inverse runs as the adjoint of the
+    forward transform's device plan, so one set of tables serves both."""
+    fwd = Cache(Sign(-1), n)
+    return Cache(Sign(1), n, adjoint_of=fwd), fwd
+
+
+def transform_with_cache(c: Cache) -> Transform:
+    """Planned.transformWithCache (Planned.hs:137-167)."""
+    return c.transform()
+
+
 def convolve_cyclic(n: int):
     """Planned.convolveCyclic (Planned.hs:779-792): normalised cyclic convolution.
 

@@ -1604,6 +1604,31 @@ af_status af_karatsuba_cyclic_i64(int64_t batch, int64_t len, const int64_t *x, 
     return karatsuba_run(meta, batch, karatsuba_P(len), x, y, z, 1, (hipStream_t)stream);
 }
 
+af_status af_karatsuba_complex_i64(int64_t batch, int64_t len, int32_t cyclic, const int64_t *xr, const int64_t *xi,
+                                   const int64_t *yr, const int64_t *yi, int64_t *zr, int64_t *zi, void *stream) {
+    if (batch < 0 || len < 0 || (cyclic != 0 && cyclic != 1)) return AF_INVALID_ARG;
+    if (batch == 0 || len == 0) return AF_SUCCESS;
+    if (!xr || !xi || !yr || !yi || !zr || !zi) return AF_INVALID_ARG;
+    AFCHK(device_init());
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t nz = cyclic ? len : 2 * len - 1, count = batch * len, zcount = batch * nz;
+    // the three operand pairs, then the three products, in one workspace
+    int64_t *ws = nullptr;
+    HIPCHK(hipMallocAsync((void **)&ws, (size_t)(6 * count + 3 * zcount) * sizeof(int64_t), s));
+    int64_t *x3 = ws, *y3 = ws + 3 * count, *p3 = ws + 6 * count;
+    std::vector<int64_t> meta(9 * batch);
+    for (int64_t q = 0; q < 3 * batch; ++q) {
+        meta[3 * q] = q * len;
+        meta[3 * q + 1] = len;
+        meta[3 * q + 2] = q * nz;
+    }
+    af_status st = HIPST(launch_gauss_split(xr, xi, yr, yi, x3, y3, count, s));
+    if (st == AF_SUCCESS) st = karatsuba_run(meta, 3 * batch, karatsuba_P(len), x3, y3, p3, cyclic, s);
+    if (st == AF_SUCCESS) st = HIPST(launch_gauss_join(p3, zr, zi, zcount, s));
+    (void)hipFreeAsync(ws, s);
+    return st;
+}
+
 af_status af_karatsuba_ragged_i64(int64_t npairs, const int64_t *offsets, const int64_t *x, const int64_t *y,
                                   int64_t *z, void *stream) {
     if (npairs < 0 || (npairs > 0 && !offsets)) return AF_INVALID_ARG;

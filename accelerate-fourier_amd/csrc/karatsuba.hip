@@ -273,6 +273,56 @@ k_karatsuba(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__restric
     }
 }
 
+// Convolution.Adhoc.complex (Convolution/Adhoc.hs:114-133) in the Gauss form:
+// three real products xr*yr, xi*yi and (xr + xi)*(yr + yi).  The split writes
+// the three operand pairs as consecutive blocks of `count` words; the join
+// takes the three products (blocks of `count` words) to
+// re = p0 - p1, im = p2 - (p0 + p1), all in the ring of 64-bit words.
+__global__ void __launch_bounds__(256)
+k_gauss_split(const u64 *__restrict__ xr, const u64 *__restrict__ xi, const u64 *__restrict__ yr,
+              const u64 *__restrict__ yi, u64 *__restrict__ x3, u64 *__restrict__ y3, int64_t count) {
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += step) {
+        const u64 a = xr[i], b = xi[i], c = yr[i], d = yi[i];
+        x3[i] = a;
+        x3[count + i] = b;
+        x3[2 * count + i] = a + b;
+        y3[i] = c;
+        y3[count + i] = d;
+        y3[2 * count + i] = c + d;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_gauss_join(const u64 *__restrict__ p3, u64 *__restrict__ zr, u64 *__restrict__ zi, int64_t count) {
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += step) {
+        const u64 p0 = p3[i], p1 = p3[count + i], p2 = p3[2 * count + i];
+        zr[i] = p0 - p1;
+        zi[i] = p2 - (p0 + p1);
+    }
+}
+
+static unsigned elementwise_grid(int64_t count) {
+    const int64_t g = (count + 255) / 256;
+    return (unsigned)(g < 8192 ? g : 8192);
+}
+
+hipError_t launch_gauss_split(const int64_t *xr, const int64_t *xi, const int64_t *yr, const int64_t *yi, int64_t *x3,
+                              int64_t *y3, int64_t count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gauss_split, dim3(elementwise_grid(count)), dim3(256), 0, s, (const u64 *)xr,
+                       (const u64 *)xi, (const u64 *)yr, (const u64 *)yi, (u64 *)x3, (u64 *)y3, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_gauss_join(const int64_t *p3, int64_t *zr, int64_t *zi, int64_t count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gauss_join, dim3(elementwise_grid(count)), dim3(256), 0, s, (const u64 *)p3, (u64 *)zr,
+                       (u64 *)zi, count);
+    return hipGetLastError();
+}
+
 void init_karatsuba_attributes() {
     (void)hipFuncSetAttribute((const void *)k_karatsuba, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }

@@ -78,5 +78,8 @@ hipError_t launch_fast_rconv(int prec, const AfPass &d, const void *in0, const v
 // Karatsuba (karatsuba.hip)
 hipError_t launch_karatsuba(const int64_t *x, const int64_t *y, int64_t *z, const int64_t *dev_meta, int64_t npairs,
                             int max_P, int cyclic, hipStream_t s);
+hipError_t launch_gauss_split(const int64_t *xr, const int64_t *xi, const int64_t *yr, const int64_t *yi, int64_t *x3,
+                              int64_t *y3, int64_t count, hipStream_t s);
+hipError_t launch_gauss_join(const int64_t *p3, int64_t *zr, int64_t *zi, int64_t count, hipStream_t s);
 
 }  // namespace af

@@ -51,6 +51,12 @@ def parse():
     ap.add_argument("--profile-uneven", action="store_true",
                     help="only the 4096-row uneven batch (FP32 forward, warmup + steps runs), for "
                          "rocprofv3 kernel traces")
+    ap.add_argument("--profile-conv", action="store_true",
+                    help="only the 512 x 8192 FP32 real convolution and its adjoint (warmup + steps "
+                         "calls), for rocprofv3 kernel traces")
+    ap.add_argument("--profile-karatsuba", action="store_true",
+                    help="only the 1024-pair ragged Int64 Karatsuba (warmup + steps calls), for "
+                         "rocprofv3 kernel traces")
     return ap.parse_args()
 
 
@@ -149,6 +155,84 @@ def profiled_traffic(pass_index):
     return None
 
 
+def conv_workload(torch):
+    """Real cyclic convolution and its adjoint, FP32 512 rows x 8192: x, the
+    kernel k (scaled by 1/sqrt(N)) and the cotangent w from normal draws of
+    splitmix64 seeds 3, 4, 5, resident in HBM.  One call is the forward
+    convolution and the vjp for both operands."""
+    from accelerate_fourier import real, synthetic
+    N = 8192
+    xr = torch.from_numpy(synthetic.normal(3, 512 * N).reshape(512, N)).to("cuda", torch.float32)
+    kr = torch.from_numpy(synthetic.normal(4, 512 * N).reshape(512, N) / math.sqrt(N)).to("cuda", torch.float32)
+    wr = torch.from_numpy(synthetic.normal(5, 512 * N).reshape(512, N)).to("cuda", torch.float32)
+    yr, dx, dk = torch.empty_like(xr), torch.empty_like(xr), torch.empty_like(xr)
+
+    def fb():
+        real.convolve(xr, kr, out=yr)
+        real.convolve_vjp(xr, kr, wr, dx, dk)
+    return fb, N
+
+
+def karatsuba_workload(torch, np):
+    """Int64 Karatsuba over 1024 ragged pairs, lengths U[256, 4096] (seed 4),
+    entries U[-2^15, 2^15) (seeds 40, 41), resident in HBM.  Returns the call
+    and the lengths."""
+    from accelerate_fourier import convolution, synthetic
+    lens = synthetic.integers(4, 1024, 256, 4097)
+    off = np.concatenate([[0], np.cumsum(lens)])
+    xa = torch.from_numpy(synthetic.integers(40, int(off[-1]), -2 ** 15, 2 ** 15)).to("cuda")
+    ya = torch.from_numpy(synthetic.integers(41, int(off[-1]), -2 ** 15, 2 ** 15)).to("cuda")
+    za = torch.empty(2 * int(off[-1]) - 1024, dtype=torch.int64, device="cuda")
+    return (lambda: convolution.karatsuba_ragged(off, xa, ya, out=za)), lens, (off, xa, ya, za)
+
+
+def karatsuba_fft_f32(torch, np, lens, data, steps, warmup):
+    """The Karatsuba pairs cast to FP32 and convolved by the fused real FFT
+    convolution: pair p is zero-padded to M = the next power of two >= 2L - 1
+    (one launch per M) with x scaled by 1/M (exact), so the result is the
+    linear convolution.  Against the exact Int64 products (the Karatsuba
+    result of the same call): time, max-abs error, and the fraction of
+    coefficients that rounding the FP32 result to the nearest integer recovers
+    (values reach 2^36 and more, far past FP32's 24-bit significand)."""
+    from accelerate_fourier import real
+    off, xa, ya, za = data
+    xh, yh = xa.cpu().numpy(), ya.cpu().numpy()
+    Ms = np.array([1 << int(np.ceil(np.log2(2 * L - 1))) for L in lens])
+    groups = []
+    for M in sorted(set(Ms.tolist())):
+        idx = np.nonzero(Ms == M)[0]
+        xp = np.zeros((len(idx), M), np.float32)
+        yp = np.zeros((len(idx), M), np.float32)
+        for j, p in enumerate(idx):
+            L = int(lens[p])
+            xp[j, :L] = xh[off[p]:off[p] + L].astype(np.float32) / M
+            yp[j, :L] = yh[off[p]:off[p] + L].astype(np.float32)
+        xd, yd = torch.from_numpy(xp).cuda(), torch.from_numpy(yp).cuda()
+        groups.append((idx, xd, yd, torch.empty_like(xd)))
+
+    def run():
+        for _, xd, yd, zd in groups:
+            real.convolve(xd, yd, out=zd)
+    ms = time_it(run, steps, warmup)
+    run()
+    zx = za.cpu().numpy()
+    err = big = 0.0
+    hit = tot = 0
+    for idx, _, _, zd in groups:
+        zf = zd.cpu().numpy()
+        for j, p in enumerate(idx):
+            L = int(lens[p])
+            got = zf[j, :2 * L - 1].astype(np.float64)
+            ex = zx[2 * off[p] - p:2 * off[p] - p + 2 * L - 1]
+            err = max(err, float(np.max(np.abs(got - ex))))
+            big = max(big, float(np.max(np.abs(ex))))
+            hit += int(np.sum(np.rint(got).astype(np.int64) == ex))
+            tot += 2 * L - 1
+    return {"c5_fft_f32_ms": ms, "c5_fft_f32_launches": len(groups), "c5_fft_f32_max_abs_err": err,
+            "c5_exact_max_abs": big, "c5_fft_f32_max_abs_err_rel_to_max": err / big,
+            "c5_fft_f32_rounded_exact_frac": hit / tot}
+
+
 def uneven_batch(torch, np, steps):
     """The 4096-row uneven batch (four length classes, sum of lengths 2.86e8), FP32
     forward, inputs U[-1, 1) from splitmix64 seed 2 resident in HBM; error of every
@@ -223,6 +307,14 @@ def main():
             b32(x32, out=y)
         torch.cuda.synchronize()
         print(json.dumps({"profile": "done", "uneven_runs": 1 + warmup + steps, "plan": b32.describe()}))
+        return
+    if args.profile_conv or args.profile_karatsuba:
+        fn = conv_workload(torch)[0] if args.profile_conv else karatsuba_workload(torch, np)[0]
+        for _ in range(warmup + steps):
+            fn()
+        torch.cuda.synchronize()
+        print(json.dumps({"profile": "done", "calls": warmup + steps,
+                          "workload": "conv" if args.profile_conv else "karatsuba"}))
         return
     n, batch = 65536, 64
     flops = 5.0 * n * math.log2(n) * batch
@@ -314,32 +406,19 @@ def main():
         extras["c1_f64_256x1024_us"] = ms1 * 1e3
         extras["c1_f64_GBps"] = 2 * 16 * 1024 * 256 / (ms1 * 1e-3) / 1e9
         # real convolution and its adjoint, FP32 512 x 8192
-        from accelerate_fourier import real
-        N = 8192
-        xr = torch.from_numpy(synthetic.normal(3, 512 * N).reshape(512, N)).to("cuda", torch.float32)
-        kr = torch.from_numpy(synthetic.normal(4, 512 * N).reshape(512, N) / math.sqrt(N)).to("cuda", torch.float32)
-        wr = torch.from_numpy(synthetic.normal(5, 512 * N).reshape(512, N)).to("cuda", torch.float32)
-        yr, dx, dk = torch.empty_like(xr), torch.empty_like(xr), torch.empty_like(xr)
-
-        def fb():
-            real.convolve(xr, kr, out=yr)
-            real.convolve_vjp(xr, kr, wr, dx, dk)
+        fb, N = conv_workload(torch)
         msc = time_it(fb, steps, warmup)
         extras["c4_conv_fwd_bwd_ms"] = msc
         extras["c4_conv_per_s"] = 512 / (msc * 1e-3)
         extras["c4_hbm_frac"] = 32 * 512 * N / (msc * 1e-3) / 1e9 / HBM_PEAK_GBS
         # Karatsuba, 1024 pairs, lengths U[256, 4096]
-        from accelerate_fourier import convolution
-        lens = synthetic.integers(4, 1024, 256, 4097)
-        off = np.concatenate([[0], np.cumsum(lens)])
-        xa = torch.from_numpy(synthetic.integers(40, int(off[-1]), -2 ** 15, 2 ** 15)).to("cuda")
-        ya = torch.from_numpy(synthetic.integers(41, int(off[-1]), -2 ** 15, 2 ** 15)).to("cuda")
-        za = torch.empty(2 * int(off[-1]) - 1024, dtype=torch.int64, device="cuda")
-        msk = time_it(lambda: convolution.karatsuba_ragged(off, xa, ya, out=za), max(steps // 4, 3), 1)
+        kar, lens, kdata = karatsuba_workload(torch, np)
+        msk = time_it(kar, max(steps // 4, 3), 1)
         extras["c5_karatsuba_ms"] = msk
         extras["c5_pairs_per_s"] = 1024 / (msk * 1e-3)
         extras["c5_schoolbook_equiv_GMACs"] = float(np.sum(lens.astype(np.float64) ** 2)) / (msk * 1e-3) / 1e9
-        del xa, ya, za
+        extras.update(karatsuba_fft_f32(torch, np, lens, kdata, steps, warmup))
+        del kar, kdata
         extras.update(uneven_batch(torch, np, steps))
         out["extras"] = extras
 

@@ -286,6 +286,18 @@ af_status af_karatsuba_ragged_i64(int64_t npairs, const int64_t *offsets,
 /* Convolution.Adhoc.cyclic karatsuba (Convolution/Adhoc.hs:104-111): batch x len outputs. */
 af_status af_karatsuba_cyclic_i64(int64_t batch, int64_t len, const int64_t *x, const int64_t *y,
                                   int64_t *z, void *stream);
+/*
+ * Complex Int64 convolution, planar operands (re and im arrays of batch x len
+ * each), in the Gauss form of three real Karatsuba products:
+ * re = xr*yr - xi*yi, im = (xr + xi)*(yr + yi) - xr*yr - xi*yi, wrap-around
+ * arithmetic.  cyclic = 0: linear, batch x (2 len - 1) outputs;
+ * cyclic = 1: Convolution.cyclic of it, batch x len outputs.
+ * Replaces: Convolution.Adhoc.complex karatsuba and
+ * Convolution.Adhoc.complex (cyclic karatsuba) (Convolution/Adhoc.hs:114-133).
+ */
+af_status af_karatsuba_complex_i64(int64_t batch, int64_t len, int32_t cyclic,
+                                   const int64_t *xr, const int64_t *xi, const int64_t *yr,
+                                   const int64_t *yi, int64_t *zr, int64_t *zi, void *stream);
 
 /* ---- Uneven batches: many lengths in one graph launch ---- */
 

@@ -474,6 +474,39 @@ void orc_complex_karatsuba_f64(int cyclic, int64_t len, int64_t rows,
     free(xm); free(ym); free(a); free(b); free(c);
 }
 
+/* complex (Adhoc.hs:114-133) over the Int64 Karatsuba: the same three real
+ * products, in wrap-around arithmetic */
+void orc_complex_karatsuba_i64(int cyclic, int64_t len, int64_t rows,
+                               const int64_t *xre, const int64_t *xim,
+                               const int64_t *yre, const int64_t *yim,
+                               int64_t *ore, int64_t *oim) {
+    if (len == 0) return;
+    int64_t zlen = 2 * len - 1, olen = cyclic ? len : zlen;
+    u64w *xm = (u64w *)malloc(sizeof(u64w) * (size_t)len), *ym = (u64w *)malloc(sizeof(u64w) * (size_t)len);
+    u64w *a = (u64w *)malloc(sizeof(u64w) * (size_t)zlen), *b = (u64w *)malloc(sizeof(u64w) * (size_t)zlen),
+         *c = (u64w *)malloc(sizeof(u64w) * (size_t)zlen);
+    for (int64_t r = 0; r < rows; ++r) {
+        const u64w *xr = (const u64w *)(xre + r * len), *xi = (const u64w *)(xim + r * len);
+        const u64w *yr = (const u64w *)(yre + r * len), *yi = (const u64w *)(yim + r * len);
+        for (int64_t k = 0; k < len; ++k) { xm[k] = xr[k] + xi[k]; ym[k] = yr[k] + yi[k]; }
+        kara_u64_one(len, xr, yr, a);
+        kara_u64_one(len, xi, yi, b);
+        kara_u64_one(len, xm, ym, c);
+        if (cyclic) {
+            for (int64_t k = 0; k < len; ++k) {
+                a[k] = a[k] + ((k + len < zlen) ? a[k + len] : 0);
+                b[k] = b[k] + ((k + len < zlen) ? b[k + len] : 0);
+                c[k] = c[k] + ((k + len < zlen) ? c[k + len] : 0);
+            }
+        }
+        for (int64_t k = 0; k < olen; ++k) {
+            ((u64w *)ore)[r * olen + k] = a[k] - b[k];
+            ((u64w *)oim)[r * olen + k] = c[k] - (a[k] + b[k]);
+        }
+    }
+    free(xm); free(ym); free(a); free(b); free(c);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Transforms, instantiated twice                                            */
 /* ------------------------------------------------------------------------ */

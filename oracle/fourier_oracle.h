@@ -95,6 +95,12 @@ void orc_complex_karatsuba_f64(int cyclic, int64_t len, int64_t rows,
                                const double *yre, const double *yim,
                                double *ore, double *oim);
 
+/* the same wrapper around the Int64 Karatsuba, wrap-around arithmetic */
+void orc_complex_karatsuba_i64(int cyclic, int64_t len, int64_t rows,
+                               const int64_t *xre, const int64_t *xim,
+                               const int64_t *yre, const int64_t *yim,
+                               int64_t *ore, int64_t *oim);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus

@@ -67,6 +67,7 @@ def lib():
             "orc_karatsuba_ragged_i64": (None, [_i64, _p, _p, _p, _p]),
             "orc_cyclic_karatsuba_i64": (None, [_i64, _i64, _p, _p, _p]),
             "orc_complex_karatsuba_f64": (None, [_int, _i64, _i64, _p, _p, _p, _p, _p, _p]),
+            "orc_complex_karatsuba_i64": (None, [_int, _i64, _i64, _p, _p, _p, _p, _p, _p]),
             "orc_num_threads": (_int, []),
         }
         for name, (res, args) in sig.items():
@@ -317,6 +318,20 @@ def complex_karatsuba(x, y, cyclic: bool = False):
     lib().orc_complex_karatsuba_f64(1 if cyclic else 0, n, rows, _ptr(a), _ptr(b), _ptr(c), _ptr(d),
                                     _ptr(ore), _ptr(oim))
     return ore + 1j * oim
+
+
+def complex_karatsuba_i64(xr, xi, yr, yi, cyclic: bool = False):
+    """Convolution.complex karatsuba (Adhoc.hs:114-133) on Int64 (re, im) parts:
+    returns (re, im), wrap-around arithmetic."""
+    xr, xi, yr, yi = (_c(v, np.int64) for v in (xr, xi, yr, yi))
+    n = xr.shape[-1]
+    rows = int(np.prod(xr.shape[:-1])) if xr.ndim > 1 else 1
+    ol = n if cyclic else max(2 * n - 1, 0)
+    ore = np.zeros(xr.shape[:-1] + (ol,), dtype=np.int64)
+    oim = np.zeros(xr.shape[:-1] + (ol,), dtype=np.int64)
+    lib().orc_complex_karatsuba_i64(1 if cyclic else 0, n, rows, _ptr(xr), _ptr(xi), _ptr(yr), _ptr(yi),
+                                    _ptr(ore), _ptr(oim))
+    return ore, oim
 
 
 def num_threads() -> int:

@@ -507,7 +507,23 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
             if (N1 > 0 && N1 * N2 != count) problem(name + ": N1*N2 differs from count");
         };
     }
-    return [] {};  // kernels without a footprint model (packing, Karatsuba)
+    if (name.find("af::k_gauss_split") == 0 || name.find("af::k_gauss_join") == 0) {
+        // split (xr, xi, yr, yi, x3, y3, count): four inputs of count words, two
+        // outputs of 3 count; join (p3, zr, zi, count): one input of 3 count, two outputs
+        const bool split = name.find("split") != std::string::npos;
+        const int na = split ? 6 : 3;
+        const void *ptr[6];
+        for (int i = 0; i < na; ++i) ptr[i] = *(const void *const *)args[i];
+        const int64_t count = *(const int64_t *)args[na];
+        return [=] {
+            for (int i = 0; i < na; ++i) {
+                const int64_t words = (split ? i >= 4 : i == 0) ? 3 * count : count;
+                in_alloc(ptr[i], 0, words * 8, split ? (i < 4 ? "operand" : "split output") : (i ? "z" : "p3"), name);
+            }
+            if (block.x != 256) problem(name + ": block differs from 256");
+        };
+    }
+    return [] {};  // kernels without a footprint model (packing)
 }
 
 }  // namespace

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from accelerate_fourier import _abi, planned
+from accelerate_fourier.sign import Sign
 
 L = None
 
@@ -186,3 +187,31 @@ def test_number_theory_matches_oracle(lib, orc):
     for n in list(range(1, 300)) + [199999, 2000005]:
         assert lib.af_ceiling_5smooth(n) == orc.ceiling_5smooth(n)
         assert lib.af_ceiling_power_of_two(n) == orc.ceiling_power_of_two(n)
+
+
+def test_planners_and_caches_host(orc):
+    """Planned.plan / planWithMapUpdate / planDecomposeWithMapUpdate /
+    planChirpWithMapUpdate (Planned.hs:201-293, 731-737) select the device
+    planner's algorithms; the chirp pads follow the oracle's planner, and
+    transformWithPlanner / cacheDuplex build transforms of the right sign and
+    algorithm without a device."""
+    assert planned.plan(1024) == planned.describe(1024)
+    assert planned.plan_with_map_update(10007).startswith("c2c.f32:bluestein10007(")
+    assert "unsupported" in planned.plan_decompose_with_map_update(10007)
+    assert planned.plan_decompose_with_map_update(65521).startswith("c2c.f32:rader65521(")
+    for pad, alg in ((planned.ceiling_power_of_two, orc.CHIRP2), (planned.ceiling_5_smooth, orc.CHIRP235)):
+        for n in (3, 47, 1000):
+            s = planned.plan_chirp_with_map_update(pad)(n)
+            m = int(re.search(r"\((?:stockham|fourstep)(\d+)", s).group(1))
+            assert m == pad(2 * n - 1)
+            assert str(m) in orc.plan_string(n, alg)
+    with pytest.raises(ValueError):
+        planned.plan_chirp_with_map_update(lambda n: n)
+    with pytest.raises(TypeError):
+        planned.transform_with_planner(lambda n: None, Sign(-1), 8)
+    t = planned.transform_with_planner(planned.plan_decompose_with_map_update, Sign(1), 97)
+    assert (t.sign.value, t.n, t.algorithm) == (1, 97, _abi.ALG_DECOMPOSE)
+    ci, cf = planned.cache_duplex(12)
+    ti, tf = planned.transform_with_cache(ci), planned.transform_with_cache(cf)
+    assert (ti.sign.value, tf.sign.value, ti.n, tf.n) == (1, -1, 12, 12)
+    assert planned.transform_with_cache(planned.cache(Sign(1), 5)).sign.value == 1

@@ -60,6 +60,7 @@ def dry():
     lib.af_karatsuba_ragged_i64.argtypes = [ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), V, V, V, V]
     lib.af_karatsuba_i64.argtypes = [ctypes.c_int64, ctypes.c_int64, V, V, V, V]
     lib.af_karatsuba_cyclic_i64.argtypes = [ctypes.c_int64, ctypes.c_int64, V, V, V, V]
+    lib.af_karatsuba_complex_i64.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32] + [V] * 7
     return lib
 
 
@@ -209,6 +210,11 @@ def test_karatsuba(dry):
             x, y = b(batch * L * 8), b(batch * L * 8)
             assert dry.af_karatsuba_i64(batch, L, x, y, b(batch * (2 * L - 1) * 8), None) == 0
             assert dry.af_karatsuba_cyclic_i64(batch, L, x, y, b(batch * L * 8), None) == 0
+            if L in (1, 65, 4096, 4608):
+                xi, yi = b(batch * L * 8), b(batch * L * 8)
+                for cyc, nz in ((0, 2 * L - 1), (1, L)):
+                    assert dry.af_karatsuba_complex_i64(batch, L, cyc, x, xi, y, yi, b(batch * nz * 8),
+                                                        b(batch * nz * 8), None) == 0
     lens = np.array([256, 4096, 1, 77, 3000, 4096, 2049], dtype=np.int64)
     offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
     arr = (ctypes.c_int64 * len(offs))(*[int(v) for v in offs])

@@ -491,3 +491,65 @@ def test_two_for_one_2d(h, w, prec):
     ua, ub = real.untangle_spectra_2d(to_dev(z, ct))
     oa, ob = oracle.untangle_spectra_2d(z)
     assert rel_l2(host(ua), oa) <= tol and rel_l2(host(ub), ob) <= tol
+
+
+@pytest.mark.parametrize("n", [12, 97, 1000, 4099, 65521])
+def test_cache_duplex_and_planners(n):
+    """cacheDuplex (Planned.hs:331-341): the inverse of the pair runs as the
+    adjoint of the forward plan and equals the inverse transform; every planner
+    of transformWithPlanner (Planned.hs:121-130) gives the DFT."""
+    x = synthetic.complex_uniform(n + 3, (3, n))
+    for sign in (-1, 1):
+        ref = orc_transform(sign, x)
+        for prec, ct in (("f64", torch.complex128), ("f32", torch.complex64)):
+            tol = tol_f64(n) if prec == "f64" else 1e-6
+            ci, cf = planned.cache_duplex(n)
+            t = planned.transform_with_cache(cf if sign == -1 else ci)
+            assert rel_l2(host(t(to_dev(x, ct))), ref) <= tol
+            for pl in (planned.plan_with_map_update, planned.plan_decompose_with_map_update,
+                       planned.plan_chirp_with_map_update(planned.ceiling_power_of_two),
+                       planned.plan_chirp_with_map_update(planned.ceiling_5_smooth)):
+                if "unsupported" in pl(n):
+                    continue
+                tp = planned.transform_with_planner(pl, Sign(sign), n)
+                got = host(tp(to_dev(x, ct)))
+                assert rel_l2(got, ref) <= (tol + 4e-15 * n if prec == "f64" else 1e-6), pl
+
+
+@pytest.mark.parametrize("L,m", [(1, 2 ** 15), (2, 2 ** 15), (65, 2 ** 15), (777, 2 ** 62), (4096, 2 ** 15)])
+def test_karatsuba_complex_bit_exact(L, m):
+    """Convolution.complex karatsuba and Convolution.complex (cyclic karatsuba)
+    (Adhoc.hs:104-133) on Int64 (re, im) pairs, bit for bit against the oracle."""
+    import oracle
+    xr, xi, yr, yi = (synthetic.integers(500 + 4 * L + i, 2 * L, -m, m).reshape(2, L) for i in range(4))
+    dev = [to_dev(v, torch.int64) for v in (xr, xi, yr, yi)]
+    for cyc in (False, True):
+        conv = convolution.complex(convolution.cyclic(convolution.karatsuba) if cyc else convolution.karatsuba)
+        zr, zi = conv((dev[0], dev[1]), (dev[2], dev[3]))
+        rr, ri = oracle.complex_karatsuba_i64(xr, xi, yr, yi, cyclic=cyc)
+        assert np.array_equal(host(zr), rr) and np.array_equal(host(zi), ri), cyc
+
+
+def test_karatsuba_pairs_by_fp32_fft_convolution():
+    """The Karatsuba pairs cast to FP32 and convolved by the zero-padded real
+    FFT convolution (real.convolve at M = the next power of two >= 2L-1, inputs
+    scaled by 1/M, exact): for small coefficients rounding recovers the exact
+    Int64 product; for the 2^15 coefficients of the benchmark the max-abs error
+    stays within eps_32 * log2(M) * |x|_2 |y|_2."""
+    import oracle
+    for m, lens in ((8, [256, 300, 1000, 1024]), (2 ** 15, [256, 2049, 4096])):
+        for i, L in enumerate(lens):
+            x = synthetic.integers(700 + i, L, -m, m)
+            y = synthetic.integers(800 + i, L, -m, m)
+            M = 1 << int(np.ceil(np.log2(2 * L - 1)))
+            xp = np.zeros((1, M), np.float32)
+            yp = np.zeros((1, M), np.float32)
+            xp[0, :L] = x.astype(np.float32) / M
+            yp[0, :L] = y.astype(np.float32)
+            got = host(real.convolve(to_dev(xp, torch.float32), to_dev(yp, torch.float32)))[0, :2 * L - 1]
+            exact = oracle.schoolbook(x[None, :], y[None, :])[0]
+            err = np.max(np.abs(got.astype(np.float64) - exact))
+            bound = 2.0 ** -23 * np.log2(M) * np.linalg.norm(x) * np.linalg.norm(y)
+            assert err <= bound, (m, L, err, bound)
+            if m == 8:
+                assert np.array_equal(np.rint(got).astype(np.int64), exact), (L, err)

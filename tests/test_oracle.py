@@ -333,3 +333,28 @@ def test_multidimensional(h, w):
     xs, ys = O.two_from_spectrum_2d(1, 1, a, b)
     np.testing.assert_allclose(xs, h * w * xr, rtol=0, atol=1e-10 * h * w)
     np.testing.assert_allclose(ys, h * w * yr, rtol=0, atol=1e-10 * h * w)
+
+
+def test_complex_karatsuba_i64_matches_four_products():
+    """Convolution.complex karatsuba (Adhoc.hs:114-133) on Int64: the Gauss
+    three-product form equals the textbook four products
+    (xr*yr - xi*yi, xr*yi + xi*yr) of the schoolbook convolution, in
+    wrap-around arithmetic (coefficients up to 2^62), linear and cyclic."""
+    def school(a, b):
+        return O.schoolbook(a, b)
+    for L, m in ((1, 99), (2, 99), (33, 2 ** 15), (64, 2 ** 62), (129, 2 ** 62)):
+        r = np.random.default_rng(300 + L)
+        xr, xi, yr, yi = (r.integers(-m, m, (3, L)) for _ in range(4))
+        re, im = O.complex_karatsuba_i64(xr, xi, yr, yi)
+        assert np.array_equal(re, school(xr, yr) - school(xi, yi))
+        assert np.array_equal(im, school(xr, yi) + school(xi, yr))
+        cre, cim = O.complex_karatsuba_i64(xr, xi, yr, yi, cyclic=True)
+        for lin, cyc in ((re, cre), (im, cim)):
+            w = lin[:, :L].copy()
+            w[:, :L - 1] += lin[:, L:]
+            assert np.array_equal(cyc, w)
+    # a sign slip in the Gauss recombination fails the comparison
+    r = np.random.default_rng(90)
+    xr, xi, yr, yi = (r.integers(-99, 99, (1, 8)) for _ in range(4))
+    bad_im = O.karatsuba(xr + xi, yr + yi) - O.karatsuba(xr, yr) + O.karatsuba(xi, yi)
+    assert not np.array_equal(bad_im, school(xr, yi) + school(xi, yr))
