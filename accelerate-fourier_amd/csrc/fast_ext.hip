@@ -64,14 +64,23 @@ __device__ __forceinline__ void ext_tile(const AfPass &d, int64_t tile, typename
 
 template <typename T, int L>
 __device__ __forceinline__ void ext_stage_table(const AfPass &d, typename Cx<T>::W *stab) {
-    using W = typename Cx<T>::W;
     // read before use: the first barrier of the stages orders it
-    if (FastTables<L>::STAGE_LDS)
-        for (int i = threadIdx.x; i < L; i += blockDim.x) stab[i] = gld<W>(d.tw, i);
+    fast_tables_fill<T, L>(d, stab);
+}
+
+// Waves per SIMD that the LDS of one workgroup allows (at most 8): the
+// register budget is held to it, so that registers never lower the occupancy
+// below what LDS permits.
+template <typename T, int L, int B>
+constexpr int ext_waves_per_simd() {
+    constexpr int lds = fast_ld(L, B) * B * (int)sizeof(typename Cx<T>::C) + FastTables<L>::STAGE_ENTRIES * 16;
+    constexpr int wg = (160 * 1024) / lds;
+    constexpr int w = wg * (B * L / 16 / 64) / 4;
+    return w < 1 ? 1 : w > 8 ? 8 : w;
 }
 
 template <typename T, int L, int B, int SG, bool CONV>
-__global__ void __launch_bounds__(B * L / 16)
+__global__ void __launch_bounds__(B * L / 16) __attribute__((amdgpu_waves_per_eu(ext_waves_per_simd<T, L, B>())))
 k_fast_ext(const AfPass d) {
     using C = typename Cx<T>::C;
     using W = typename Cx<T>::W;
@@ -84,7 +93,7 @@ k_fast_ext(const AfPass d) {
 }
 
 template <typename T, int L, int B, int SG, bool CONV>
-__global__ void __launch_bounds__(B * L / 16)
+__global__ void __launch_bounds__(B * L / 16) __attribute__((amdgpu_waves_per_eu(ext_waves_per_simd<T, L, B>())))
 k_fast_ext_grouped(const AfPass *__restrict__ descs, const int64_t *__restrict__ prefix, const int ndesc) {
     using C = typename Cx<T>::C;
     using W = typename Cx<T>::W;
@@ -152,7 +161,7 @@ bool fast_ext_eligible(int prec, const AfPass &d) {
     const int64_t pout = (d.nlines - 1) * d.out_ls + (int64_t)(d.L - 1) * d.out_es;
     if (pin >= ((int64_t)1 << 32) || pout >= ((int64_t)1 << 32)) return false;
     if ((d.flags & AFP_POSTTW) && (d.ftN >> d.ftShift) * 16 + 16 >= ((int64_t)1 << 32)) return false;
-    const int64_t lds = (int64_t)fast_ld(d.L, d.B) * d.B * (prec == PREC_F64 ? 16 : 8) + (d.L <= 1024 ? d.L : 0) * 16;
+    const int64_t lds = (int64_t)fast_ld(d.L, d.B) * d.B * (prec == PREC_F64 ? 16 : 8) + (d.L <= 1024 ? d.L : 256) * 16;
     return lds <= 160 * 1024;
 }
 

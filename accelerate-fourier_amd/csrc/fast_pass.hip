@@ -31,8 +31,7 @@ k_fast(const AfPass d) {
     C *lds = (C *)smem_fast;
     W *stab = (W *)(smem_fast + (size_t)LD * B * sizeof(C));
     // stage table into LDS; the first barrier of the stages orders it before any use
-    if (FT::STAGE_LDS)
-        for (int i = threadIdx.x; i < L; i += NT) stab[i] = ((const W *)d.tw)[i];
+    fast_tables_fill<T, L>(d, stab);
     const FastRow r = fast_row<T, B>(d, xcd_tile(blockIdx.x, gridDim.x));
     C v[16];
     fast_load<T, L, B>(d, r, v);
@@ -75,9 +74,10 @@ __device__ __forceinline__ void fast_rows_load(const T *src, int64_t row0, typen
 
 // Inverse transform of the B lines in buf (natural order), stored to HBM as
 // real pairs from row0 on.  buf is free again when it returns.
-template <typename T, int L, int B>
+template <typename T, int L, int B, bool RT = false>
 __device__ __forceinline__ void fast_inverse_store(const AfPass &d, typename Cx<T>::C *buf,
-                                                   const typename Cx<T>::W *stab, T *dst, int64_t row0) {
+                                                   const typename Cx<T>::W *stab, T *dst, int64_t row0,
+                                                   const FastRegTw<L> *rt = nullptr) {
     using C = typename Cx<T>::C;
     constexpr int NB = L / 16;
     constexpr int LD = fast_ld(L, B);
@@ -87,7 +87,7 @@ __device__ __forceinline__ void fast_inverse_store(const AfPass &d, typename Cx<
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = buf[b * LD + fpad(i + j * NB)];
     lds_barrier();
-    fast_fft_lds<T, L, B, 1>(d, v, buf, stab);
+    fast_fft_lds<T, L, B, 1, RT>(d, v, buf, stab, rt);
     const Rsrc r = mk_rsrc(dst + row0 * 2 * L, (uint32_t)(B * L * sizeof(C)));
     const uint32_t o0 = (uint32_t)((b * L + i) * sizeof(C));
 #pragma unroll
@@ -193,6 +193,13 @@ __device__ __forceinline__ void fast_spec_product2(const typename Cx<T>::C *A, c
 // FP32: one line buffer; the first signal's spectrum is kept in the registers of the
 // threads that own its pairs, so the buffer serves every transform in turn and
 // twice as many workgroups fit per CU as with two buffers.
+// FP32 at lengths 2048 and 4096 takes the last-stage register twiddles
+// (FastRegTw); FP64, already at one workgroup per CU there, keeps the table
+// loads.
+template <typename T, int L> __host__ __device__ constexpr bool rconv_regtw() {
+    return !FastTables<L>::STAGE_LDS && sizeof(T) == 4;
+}
+
 template <typename T, int L, int B>
 __host__ __device__ constexpr size_t fast_rconv_lds_bytes() {
     return (sizeof(T) == 4 ? 1 : 2) * (size_t)fast_ld(L, B) * B * sizeof(typename Cx<T>::C) +
@@ -217,43 +224,46 @@ k_fast_rconv(const AfPass d, const T *__restrict__ in0, const T *__restrict__ in
     C *A = (C *)smem_rc;
     C *Bb = ONE ? A : A + LD * B;
     W *stab = (W *)(Bb + LD * B);
-    if (FT::STAGE_LDS)
-        for (int i = threadIdx.x; i < L; i += NT) stab[i] = ((const W *)d.tw)[i];
+    fast_tables_fill<T, L>(d, stab);
     const W *aux = (const W *)d.aux_tw;
     const int64_t row0 = (int64_t)blockIdx.x * B;
+    constexpr bool RT = rconv_regtw<T, L>();
+    FastRegTw<L> rt;
+    if constexpr (RT) fast_regtw_init<L, B>(d, rt);
     C v[16];
     if constexpr (ONE) {
         C p0[RP::NP], p1[RP::NP];
         fast_rows_load<T, L, B>(in0, row0, v);
-        fast_fft_lds<T, L, B, -1>(d, v, A, stab);
+        fast_fft_lds<T, L, B, -1, RT>(d, v, A, stab, &rt);
         fast_pairs_read<T, L, B>(A, p0, p1);
+        // (loading in1 before the first transform, to overlap it, measured slower)
         fast_rows_load<T, L, B>(in1, row0, v);
         lds_barrier();  // every pair is read before the second transform writes A
-        fast_fft_lds<T, L, B, -1>(d, v, A, stab);
+        fast_fft_lds<T, L, B, -1, RT>(d, v, A, stab, &rt);
         fast_spec_product<T, L, B>(p0, p1, A, aux, VJP);
         lds_barrier();
-        fast_inverse_store<T, L, B>(d, A, stab, out0, row0);
+        fast_inverse_store<T, L, B, RT>(d, A, stab, out0, row0, &rt);
         if (VJP) {
             fast_rows_load<T, L, B>(in2, row0, v);
-            fast_fft_lds<T, L, B, -1>(d, v, A, stab);
+            fast_fft_lds<T, L, B, -1, RT>(d, v, A, stab, &rt);
             fast_spec_product<T, L, B>(p0, p1, A, aux, true);
             lds_barrier();
-            fast_inverse_store<T, L, B>(d, A, stab, out1, row0);
+            fast_inverse_store<T, L, B, RT>(d, A, stab, out1, row0, &rt);
         }
     } else {
         fast_rows_load<T, L, B>(in0, row0, v);
-        fast_fft_lds<T, L, B, -1>(d, v, A, stab);
+        fast_fft_lds<T, L, B, -1, RT>(d, v, A, stab, &rt);
         fast_rows_load<T, L, B>(in1, row0, v);
-        fast_fft_lds<T, L, B, -1>(d, v, Bb, stab);
+        fast_fft_lds<T, L, B, -1, RT>(d, v, Bb, stab, &rt);
         fast_spec_product2<T, L, B>(A, Bb, Bb, aux, VJP);
         lds_barrier();
-        fast_inverse_store<T, L, B>(d, Bb, stab, out0, row0);
+        fast_inverse_store<T, L, B, RT>(d, Bb, stab, out0, row0, &rt);
         if (VJP) {
             fast_rows_load<T, L, B>(in2, row0, v);
-            fast_fft_lds<T, L, B, -1>(d, v, Bb, stab);
+            fast_fft_lds<T, L, B, -1, RT>(d, v, Bb, stab, &rt);
             fast_spec_product2<T, L, B>(A, Bb, Bb, aux, true);
             lds_barrier();
-            fast_inverse_store<T, L, B>(d, Bb, stab, out1, row0);
+            fast_inverse_store<T, L, B, RT>(d, Bb, stab, out1, row0, &rt);
         }
     }
 }
@@ -312,7 +322,7 @@ bool fast_pass_eligible(int prec, const AfPass &d) {
     const int64_t span_o = ((int64_t)(d.B - 1) * d.out_ls + (int64_t)(d.L - 1) * d.out_es) * (d.out_im ? eb : 2 * eb) + eb;
     if (span_i >= ((int64_t)1 << 32) || span_o >= ((int64_t)1 << 32)) return false;
     if ((d.flags & AFP_POSTTW) && (d.ftN >> d.ftShift) * 16 + 16 >= ((int64_t)1 << 32)) return false;
-    const int64_t lds = (int64_t)fast_ld(d.L, d.B) * d.B * (prec == PREC_F64 ? 16 : 8) + (d.L <= 1024 ? d.L : 0) * 16;
+    const int64_t lds = (int64_t)fast_ld(d.L, d.B) * d.B * (prec == PREC_F64 ? 16 : 8) + (d.L <= 1024 ? d.L : 256) * 16;
     if (lds > 160 * 1024) return false;
     return (int64_t)d.B * d.L / 16 <= 1024;
 }

@@ -30,15 +30,36 @@ __device__ __host__ constexpr int fpad(int e) { return e + (e >> 4); }
 __device__ __host__ constexpr int fast_ld(int L, int B) { return B > 1 ? fpad(L) + 1 : fpad(L); }
 
 // Stage twiddles come from an LDS copy of the W_L table when it is small
-// (L <= 1024: at most 16 KiB), otherwise from one FP64 load per butterfly and
-// FP64 powers.  Four-step twiddles come from FP64 recurrences: one coarse x
+// (L <= 1024: at most 16 KiB).  Longer lines (2048, 4096: three stages) take
+// the second stage's W_L^(k j L/256) = W_256^(k j) from an LDS copy of W_256
+// (4 KiB) and the last stage's from one FP64 load per butterfly and FP64
+// powers (or from registers, FastRegTw).  Four-step twiddles come from FP64 recurrences: one coarse x
 // fine table product per butterfly and a running FP64 product over its
 // outputs, each value rounded once to the working precision.  (LDS copies of
 // the four-step tables, two 16-byte LDS reads and a compensated product per
 // element, measured slower on MI355X.)
 template <int L> struct FastTables {
     static constexpr bool STAGE_LDS = L <= 1024;
-    static constexpr int STAGE_ENTRIES = STAGE_LDS ? L : 0;
+    static constexpr int STAGE_ENTRIES = STAGE_LDS ? L : 256;
+};
+
+// The LDS stage table of the line length (W_L, or W_256 beyond 1024) from
+// the W_L table of the working precision; read after the first stage's
+// barrier.
+template <typename T, int L>
+__device__ __forceinline__ void fast_tables_fill(const AfPass &d, typename Cx<T>::W *stab) {
+    using W = typename Cx<T>::W;
+    constexpr int step = FastTables<L>::STAGE_LDS ? 1 : L / 256;
+    for (int i = threadIdx.x; i < FastTables<L>::STAGE_ENTRIES; i += blockDim.x) stab[i] = ((const W *)d.tw)[i * step];
+}
+
+// The last-stage twiddles of the fused convolutions' transforms at lengths
+// 2048 and 4096 held in registers: each thread forms the 15 powers w^j of its
+// butterflies' twiddle bases once, in FP64, and every transform of the
+// workgroup reuses them (conjugated for the inverse) instead of a table load
+// and a chain of FP64 products per transform.
+template <int L> struct FastRegTw {
+    double2 w[15];  // butterfly q, power j at q (R - 1) + j - 1
 };
 
 template <typename T, int L, int B>
@@ -331,6 +352,9 @@ __device__ __forceinline__ void fast_stages(const AfPass &d, const FastRow &r, t
                 if (FT::STAGE_LDS) {
 #pragma unroll
                     for (int j = 1; j < R; ++j) v[q * R + j] = twmul(v[q * R + j], stab[j * k * tws], s, false);
+                } else if (st == 1) {  // the second of three stages: p R = 256, W_256 in LDS
+#pragma unroll
+                    for (int j = 1; j < R; ++j) v[q * R + j] = twmul(v[q * R + j], stab[k * j], s, false);
                 } else {
                     // w^j for j = 1..R-1 from one FP64 load and FP64 powers
                     double2 w1 = ((const double2 *)d.tw_d)[k * tws];
@@ -382,12 +406,40 @@ __device__ __forceinline__ void fast_stages(const AfPass &d, const FastRow &r, t
     }
 }
 
+// The register twiddles of the thread's last-stage butterflies, from the
+// FP64 W_L table.
+template <int L, int B>
+__device__ __forceinline__ void fast_regtw_init(const AfPass &d, FastRegTw<L> &rt) {
+    using PL = P2Plan<L>;
+    static_assert(PL::NST == 3 && L > 1024, "register twiddles serve the three-stage lengths 2048 and 4096");
+    constexpr int NT = B * L / 16;
+    constexpr int st = PL::NST - 1;
+    constexpr int R = PL::radix(st);
+    constexpr int p = PL::p(st);
+    constexpr int PER = 16 / R;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        int b, i;
+        bf_map<B, L / R>((int)threadIdx.x + q * NT, false, b, i);
+        const int k = i % p;
+        const double2 w1 = ((const double2 *)d.tw_d)[k * (L / (p * R))];
+        double2 wj = w1;
+#pragma unroll
+        for (int j = 1; j < R; ++j) {
+            rt.w[q * (R - 1) + j - 1] = wj;
+            if (j + 1 < R) wj = cmul(wj, w1);
+        }
+    }
+}
+
 // All stages of B lines (one radix-16 first-stage butterfly per thread, its
 // inputs in v); each line ends in lds in natural order.  Consecutive threads
 // take consecutive elements of a line in every stage.  Ends with a barrier.
-template <typename T, int L, int B, int SG>
+// RT: the stage twiddles come from rt (fast_regtw_init) for lengths whose
+// W_L table is not in LDS.
+template <typename T, int L, int B, int SG, bool RT = false>
 __device__ __forceinline__ void fast_fft_lds(const AfPass &d, typename Cx<T>::C (&v)[16], typename Cx<T>::C *lds,
-                                             const typename Cx<T>::W *stab) {
+                                             const typename Cx<T>::W *stab, const FastRegTw<L> *rt = nullptr) {
     using PL = P2Plan<L>;
     using FT = FastTables<L>;
     constexpr int NT = B * L / 16;
@@ -426,6 +478,16 @@ __device__ __forceinline__ void fast_fft_lds(const AfPass &d, typename Cx<T>::C 
                 if (FT::STAGE_LDS) {
 #pragma unroll
                     for (int j = 1; j < R; ++j) v[q * R + j] = twmul(v[q * R + j], stab[j * k * tws], s, false);
+                } else if (st == 1) {  // the second of three stages: p R = 256, W_256 in LDS
+#pragma unroll
+                    for (int j = 1; j < R; ++j) v[q * R + j] = twmul(v[q * R + j], stab[k * j], s, false);
+                } else if (RT) {
+#pragma unroll
+                    for (int j = 1; j < R; ++j) {
+                        double2 w = rt->w[q * (R - 1) + j - 1];
+                        if (SG > 0) w.y = -w.y;
+                        v[q * R + j] = dmul(v[q * R + j], w, false);
+                    }
                 } else {
                     double2 w1 = ((const double2 *)d.tw_d)[k * tws];
                     if (SG > 0) w1.y = -w1.y;

@@ -251,7 +251,7 @@ void check_fast_launch(const AfPass &d, int tsz, int L, int B, int sg, dim3 grid
     if (d.sign != sg) problem(k + ": descriptor sign differs from the instantiation");
     if ((int64_t)grid.x != d.ntiles) problem(k + ": grid differs from ntiles");
     if ((int64_t)block.x != (int64_t)B * L / 16) problem(k + ": block differs from B*L/16");
-    const int64_t need = (int64_t)fast_ld_h(L, B) * B * 2 * tsz + (L <= 1024 ? L : 0) * 16;
+    const int64_t need = (int64_t)fast_ld_h(L, B) * B * 2 * tsz + (L <= 1024 ? L : 256) * 16;
     if ((int64_t)shmem != need) problem(k + ": LDS size differs from the kernel's layout");
     if (shmem > 160 * 1024) problem(k + ": LDS above 160 KiB");
     if (d.flags & ~(AFP_POSTTW | AFP_SCALE)) problem(k + ": flags the kernel ignores");
@@ -280,7 +280,7 @@ void check_ext_desc(const AfPass &d, int tsz, int L, int B, int sg, bool conv, d
     if (((d.flags & AFP_CONV) != 0) != conv) problem(k + ": CONV differs from the instantiation");
     if (conv && d.in_lines_fast) problem(k + ": convolution lines must be rows");
     if ((int64_t)block.x != (int64_t)B * L / 16) problem(k + ": block differs from B*L/16");
-    const int64_t need = (int64_t)fast_ld_h(L, B) * B * 2 * tsz + (L <= 1024 ? L : 0) * 16;
+    const int64_t need = (int64_t)fast_ld_h(L, B) * B * 2 * tsz + (L <= 1024 ? L : 256) * 16;
     if ((int64_t)shmem != need) problem(k + ": LDS size differs from the kernel's layout");
     if (shmem > 160 * 1024) problem(k + ": LDS above 160 KiB");
     if (d.flags & ~(AFP_POSTTW | AFP_SCALE | AFP_CONV | AFP_SPEC_PLAIN)) problem(k + ": flags the kernel ignores");
@@ -291,7 +291,7 @@ void check_ext_desc(const AfPass &d, int tsz, int L, int B, int sg, bool conv, d
     const int64_t span_o = ((int64_t)(B - 1) * d.out_ls + (int64_t)(L - 1) * d.out_es) * (d.out_im ? eb : 2 * eb) + eb;
     if (span_i >= ((int64_t)1 << 32) || span_o >= ((int64_t)1 << 32)) problem(k + ": tile span above 32 bits");
     in_alloc(d.tw_d, 0, (int64_t)L * 16, "tw_d", k);
-    if (L <= 1024) in_alloc(d.tw, 0, (int64_t)L * 16, "tw", k);
+    in_alloc(d.tw, 0, (int64_t)L * 16, "tw", k);  // the LDS stage table, or W_256 from it
     if (d.flags & AFP_POSTTW) {
         const int64_t T = (int64_t)1 << d.ftShift;
         in_alloc(d.ft_coarse_d, 0, (((d.ftN - 1) >> d.ftShift) + 1) * 16, "ft_coarse_d", k);
@@ -399,13 +399,15 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
             if ((int64_t)grid.x * B != d.nrows) problem(name + ": grid * B differs from the rows");
             if ((int64_t)block.x != (int64_t)B * L / 16) problem(name + ": block differs from B*L/16");
             // FP32: one line buffer (first spectrum in registers); FP64: two
-            const int64_t need = (tsz == 4 ? 1 : 2) * (int64_t)fast_ld_h(L, B) * B * 2 * tsz + (L <= 1024 ? L : 0) * 16;
+            // + the W_L stage table (L <= 1024) or W_256
+            const int64_t need = (tsz == 4 ? 1 : 2) * (int64_t)fast_ld_h(L, B) * B * 2 * tsz + (L <= 1024 ? L : 256) * 16;
             if ((int64_t)shmem != need) problem(name + ": LDS size differs from the kernel's layout");
             for (int i = 0; i < (vjp ? 3 : 2); ++i) in_alloc(in[i], 0, d.nrows * row, "input", name);
             for (int i = 0; i < (vjp ? 2 : 1); ++i) in_alloc(out[i], 0, d.nrows * row, "output", name);
             in_alloc(d.aux_tw, 0, (int64_t)L * 16, "aux_tw", name);
-            if (L <= 1024) in_alloc(d.tw, 0, (int64_t)L * 16, "tw", name);
-            else in_alloc(d.tw_d, 0, (int64_t)L * 16, "tw_d", name);
+            // W_L (float4 / double2 entries): the LDS stage table, or W_256 from it
+            in_alloc(d.tw, 0, (int64_t)L * 16, "tw", name);
+            if (L > 1024) in_alloc(d.tw_d, 0, (int64_t)L * 16, "tw_d", name);
         };
     }
     if (name.find("af::k_karatsuba") == 0) {
