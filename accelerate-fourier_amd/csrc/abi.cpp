@@ -1110,11 +1110,11 @@ af_status af_plan_tune(af_plan plan, int64_t batch, int32_t planar, void *stream
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (st == AF_SUCCESS) st = HIPST(hipEventCreate(&e0));
     if (st == AF_SUCCESS) st = HIPST(hipEventCreate(&e1));
-    // average microseconds per launch of d: two warm-ups, then the best of three runs of five
+    // average microseconds per launch of d: two warm-ups, then the best of five runs of five
     auto time_pass = [&](const AfPass &d, double &us) -> af_status {
         for (int w = 0; w < 2; ++w) HIPCHK(launch_pass(prec, d, s));
         us = 1e30;
-        for (int rep = 0; rep < 3; ++rep) {
+        for (int rep = 0; rep < 5; ++rep) {
             HIPCHK(hipEventRecord(e0, s));
             for (int k = 0; k < 5; ++k) HIPCHK(launch_pass(prec, d, s));
             HIPCHK(hipEventRecord(e1, s));
@@ -1145,7 +1145,19 @@ af_status af_plan_tune(af_plan plan, int64_t batch, int32_t planar, void *stream
                 if (!tile_launchable(prec, d)) continue;
                 double t = 0;
                 st = time_pass(d, t);
-                if (st == AF_SUCCESS && t < tb * 0.98) {  // a clear win only
+                // the incumbent is timed again right after each candidate (its
+                // best time only improves), and a candidate must beat it by 3%:
+                // run-to-run noise does not flip the choice
+                AfPass cur = base;
+                if (best[i].B) {
+                    reorder_radices(cur, best[i].order);
+                    set_tile(cur, prec, best[i].B);
+                }
+                double tc = 0;
+                if (st == AF_SUCCESS) st = time_pass(cur, tc);
+                tb = std::min(tb, tc);
+                if (best[i].B == 0) t0 = std::min(t0, tc);
+                if (st == AF_SUCCESS && t < tb * 0.97) {
                     tb = t;
                     best[i].B = B;
                     best[i].order = o;

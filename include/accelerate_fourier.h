@@ -131,7 +131,8 @@ af_status af_execute_pass(af_plan plan, int32_t index, int64_t batch,
  * arrays, 0: interleaved) on `stream`, every tile line count 1..64 that some
  * kernel can run for each launch of the transform, and for launches on the
  * mixed-radix kernel also three orders of their radices (as planned, reversed,
- * odd radices first); keeps what is fastest by more than 2% in the plan, where
+ * odd radices first); keeps what beats the incumbent (timed again after each
+ * candidate) by more than 3% in the plan, where
  * af_execute, af_vjp_execute and af_execute_pass with the same batch and
  * layout use it, and in the process's wisdom.  Allocates and frees its own
  * zero-filled buffers and synchronizes the stream.  `report` (may be NULL)

@@ -553,3 +553,51 @@ def test_karatsuba_pairs_by_fp32_fft_convolution():
             assert err <= bound, (m, L, err, bound)
             if m == 8:
                 assert np.array_equal(np.rint(got).astype(np.int64), exact), (L, err)
+
+
+def test_karatsuba_full_config_bit_exact():
+    """The full Karatsuba workload: 1024 ragged pairs, lengths U[256, 4096]
+    (seed 4), coefficients U[-2^15, 2^15) (seeds 40, 41), bit for bit against
+    the oracle's ragged Karatsuba."""
+    import oracle
+    lens = synthetic.integers(4, 1024, 256, 4097)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    x = synthetic.integers(40, int(off[-1]), -2 ** 15, 2 ** 15)
+    y = synthetic.integers(41, int(off[-1]), -2 ** 15, 2 ** 15)
+    z = host(convolution.karatsuba_ragged(off, to_dev(x, torch.int64), to_dev(y, torch.int64)))
+    assert np.array_equal(z, oracle.karatsuba_ragged(off, x, y))
+
+
+def test_uneven_batch_full_config():
+    """The full uneven batch (4096 rows, four length classes, seed 2), FP32:
+    every row within 1e-6 relative L2 of the FP64 batch on the same inputs, and
+    two rows of each class (the shortest and the longest) against the oracle in
+    FP64 at the oracle's own bound.  Inputs U[-1, 1) drawn on the device by a
+    seeded torch generator (2.9e8 values)."""
+    import oracle
+    from accelerate_fourier.batch import UnevenBatch
+    lens, cls = synthetic.uneven_lengths(2)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    tot = int(off[-1])
+    g = torch.Generator(device=DEV)
+    g.manual_seed(2)
+    v = torch.rand(2, tot, dtype=torch.float64, device=DEV, generator=g) * 2 - 1
+    x64 = torch.complex(v[0], v[1])
+    del v
+    y64 = UnevenBatch(lens, forward, "f64")(x64)
+    y32 = UnevenBatch(lens, forward, "f32")(x64.to(torch.complex64))
+    sizes = [int(n) for n in lens]
+    err = y32.to(torch.complex128) - y64
+    d = torch.stack([torch.linalg.vector_norm(r) for r in torch.split(err, sizes)])
+    nrm = torch.stack([torch.linalg.vector_norm(r) for r in torch.split(y64, sizes)])
+    rel = (d / nrm).cpu().numpy()
+    del err
+    assert rel.max() <= 1e-6, (int(rel.argmax()), float(rel.max()))
+    for c in range(len(synthetic.UNEVEN_CLASSES)):
+        rows = np.nonzero(cls == c)[0]
+        for r in (rows[np.argmin(lens[rows])], rows[np.argmax(lens[rows])]):
+            n = int(lens[r])
+            ref = oracle.transform(-1, host(x64[off[r]:off[r + 1]]))
+            got = host(y64[off[r]:off[r + 1]])
+            assert rel_l2(got, ref) <= tol_f64(n), (c, int(r), n)
+    del x64, y64, y32
