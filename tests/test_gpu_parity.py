@@ -601,3 +601,72 @@ def test_uneven_batch_full_config():
             got = host(y64[off[r]:off[r + 1]])
             assert rel_l2(got, ref) <= tol_f64(n), (c, int(r), n)
     del x64, y64, y32
+
+
+def test_concurrent_plans_and_executes():
+    """Plans are immutable after creation and creation is thread-safe: plans
+    made concurrently from host threads (the C ABI releases the interpreter
+    during the call), and one plan executed concurrently from four threads,
+    each on its own stream, give every caller its own correct result (plans
+    with workspace: four-step 65536, Bluestein 100003, Rader 4099)."""
+    import ctypes
+    import threading
+    import oracle
+
+    handles, errs = [None] * 6, []
+    lens = [65536, 100003, 4099, 1000, 65521, 77]
+
+    def create(i):
+        try:
+            d = _abi.PlanDesc(lens[i], -1, _abi.C2C, _abi.F32, _abi.ALG_AUTO, 0)
+            h = ctypes.c_void_p()
+            _abi.check(_abi.lib().af_plan_create(ctypes.byref(d), ctypes.byref(h)), "af_plan_create")
+            handles[i] = h
+        except Exception as e:  # reported below
+            errs.append(e)
+    th = [threading.Thread(target=create, args=(i,)) for i in range(len(lens))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errs, errs
+    for i, n in enumerate(lens):
+        x = synthetic.complex_uniform(950 + i, (2, n))
+        xd = to_dev(x, torch.complex64)
+        y = torch.empty_like(xd)
+        re, im, _ = _arrays_view(xd)
+        ore, oim, _ = _arrays_view(y)
+        _abi.check(_abi.lib().af_execute(handles[i], 2, re, im, ore, oim, None, None), "af_execute")
+        assert rel_l2(host(y), orc_transform(-1, x)) <= 1e-6, n
+        _abi.lib().af_plan_destroy(handles[i])
+
+    for n in (65536, 100003, 4099):
+        t = planned.transform(forward, n)
+        xs = [synthetic.complex_uniform(900 + i, (3, n)) for i in range(4)]
+        t(to_dev(xs[0], torch.complex64))
+        outs, errs = [None] * 4, []
+
+        def work(i):
+            try:
+                s = torch.cuda.Stream()
+                with torch.cuda.stream(s):
+                    xd = torch.from_numpy(xs[i]).to(DEV, torch.complex64)
+                    for _ in range(3):
+                        y = t(xd)
+                    s.synchronize()
+                    outs[i] = y.cpu().numpy()
+            except Exception as e:  # reported below
+                errs.append(e)
+        th = [threading.Thread(target=work, args=(i,)) for i in range(4)]
+        for w in th:
+            w.start()
+        for w in th:
+            w.join()
+        assert not errs, errs
+        for i in range(4):
+            assert rel_l2(outs[i], orc_transform(-1, xs[i])) <= 1e-6, (n, i)
+
+
+def _arrays_view(x):
+    from accelerate_fourier import _arrays
+    return _arrays.complex_view(x)
