@@ -48,10 +48,11 @@ __global__ void __launch_bounds__(1024) k_pass(const AfPass d) {
 }
 
 // tile_prefix[i] = first global tile of descriptor i; tile_prefix[ndesc] = total.
-// Not inlined: the register saves of the call are paid once per workgroup,
-// which loops over many tiles, and the descriptor in LDS is read directly.
+// The descriptor in LDS is read directly.  Inlined: as a call, its register
+// saves took 196 bytes of scratch per lane (FP32) in every workgroup, about as
+// much memory traffic as the tile data of a workgroup with one or two tiles.
 template <typename T, bool FULL>
-__device__ __noinline__ void pass_tiles_grouped(const AfPass *__restrict__ descs, const int64_t *__restrict__ tile_prefix,
+__device__ __forceinline__ void pass_tiles_grouped(const AfPass *__restrict__ descs, const int64_t *__restrict__ tile_prefix,
                                                 const int ndesc, typename Cx<T>::C *lds, TileLines &tl,
                                                 typename Cx<T>::C *s_x0, typename Cx<T>::C *s_dc) {
     // the tile's descriptor is copied into LDS, where the tile reads it
