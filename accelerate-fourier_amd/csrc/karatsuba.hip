@@ -55,16 +55,19 @@ __device__ __forceinline__ u64 readlane64(u64 v, int l) {
 
 // Base-3 digits of t over `levels` levels (level 1 = most significant digit):
 // base has bit (levels - lev) set where the digit is 2, freem where it is 1.
+// (Accumulated in locals with selects: conditional updates through the
+// references became a scratch array indexed by the digit.)
 __device__ __forceinline__ void digits3(int t, int levels, int &base, int &freem) {
-    base = 0;
-    freem = 0;
+    int b = 0, f = 0;
     for (int lev = levels; lev >= 1; --lev) {
         const int dig = t % 3;
         t /= 3;
         const int bit = 1 << (levels - lev);
-        if (dig == 2) base |= bit;
-        if (dig == 1) freem |= bit;
+        b |= dig == 2 ? bit : 0;
+        f |= dig == 1 ? bit : 0;
     }
+    base = b;
+    freem = f;
 }
 
 constexpr int kKarThreads = 1024;
