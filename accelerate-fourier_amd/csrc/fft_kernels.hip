@@ -137,9 +137,13 @@ __device__ __forceinline__ void spectral_product(typename Cx<T>::C *A, typename 
 
 template <typename T>
 __global__ void __launch_bounds__(1024)
-k_real_conv(const AfPass d, const T *__restrict__ x, const T *__restrict__ k, T *__restrict__ y) {
+k_real_conv(const AfPass dv, const T *__restrict__ x, const T *__restrict__ k, T *__restrict__ y) {
     extern __shared__ __align__(16) unsigned char smem[];
     using C = typename Cx<T>::C;
+    // the descriptor read in place from the kernel arguments (as in k_pass):
+    // taken by value, all of its fields were held in registers and spilled
+    (void)dv;
+    const AfPass &d = *opaque((const AfPass *)(const void *)__builtin_amdgcn_kernarg_segment_ptr());
     const int64_t N = 2 * (int64_t)d.L;
     C *A = (C *)smem;
     C *Bb = A + pidx(d.ld, d.padshift);
@@ -160,10 +164,12 @@ k_real_conv(const AfPass d, const T *__restrict__ x, const T *__restrict__ k, T 
 // dk = fromSpectrum(S_w conj S_x), dx = fromSpectrum(S_w conj S_k)
 template <typename T>
 __global__ void __launch_bounds__(1024)
-k_real_conv_vjp(const AfPass d, const T *__restrict__ x, const T *__restrict__ k, const T *__restrict__ w,
+k_real_conv_vjp(const AfPass dv, const T *__restrict__ x, const T *__restrict__ k, const T *__restrict__ w,
                 T *__restrict__ dx, T *__restrict__ dk) {
     extern __shared__ __align__(16) unsigned char smem[];
     using C = typename Cx<T>::C;
+    (void)dv;  // read in place, as in k_real_conv
+    const AfPass &d = *opaque((const AfPass *)(const void *)__builtin_amdgcn_kernarg_segment_ptr());
     const int64_t N = 2 * (int64_t)d.L;
     C *A = (C *)smem;
     C *Bb = A + pidx(d.ld, d.padshift);
