@@ -1,6 +1,6 @@
 """The generated gfx950 code of every kernel source keeps a wait state between
 a vector-memory store of more than 8 bytes and a vector instruction that
-overwrites the store's data registers.
+overwrites the store's data registers, and spills nothing to scratch.
 
 A buffer store that takes its wavefront-uniform offset from an SGPR (soffset)
 gets no such wait state from the compiler, and on MI355X those stores then
@@ -83,11 +83,40 @@ def test_scanner_finds_a_planted_clobber():
     assert len(hits) == 1 and "v[64:67]" in hits[0][1]
 
 
+_ASM_TEXTS = {}
+
+
+def _asm_texts():
+    """The gfx950 assembly of every kernel source, compiled once per session."""
+    if not _ASM_TEXTS:
+        with tempfile.TemporaryDirectory() as d:
+            with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+                outs = list(ex.map(lambda s: _asm(s, d), SOURCES))
+            for src, o in zip(SOURCES, outs):
+                _ASM_TEXTS[src] = open(o).read()
+    return _ASM_TEXTS
+
+
 def test_no_store_data_clobber():
-    with tempfile.TemporaryDirectory() as d:
-        with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-            outs = list(ex.map(lambda s: _asm(s, d), SOURCES))
-        hits = []
-        for o in outs:
-            hits += clobbered_stores(open(o).read())
+    hits = []
+    for text in _asm_texts().values():
+        hits += clobbered_stores(text)
     assert not hits, f"{len(hits)} stores with clobbered data, e.g. {hits[:3]}"
+
+
+# The FP64 generic pass kernels keep a few registers' worth of spills at the
+# 128-register budget of 1024-thread workgroups; no other kernel may use
+# scratch (private memory traffic per lane, paid in every workgroup).
+_SCRATCH_ALLOWED = {"_ZN2af6k_passIdLb0EEEv6AfPass": 64, "_ZN2af6k_passIdLb1EEEv6AfPass": 64,
+                    "_ZN2af14k_pass_groupedIdLb0EEEvPK6AfPassPKli": 64,
+                    "_ZN2af14k_pass_groupedIdLb1EEEvPK6AfPassPKli": 64}
+
+
+def test_no_kernel_spills():
+    over = []
+    for text in _asm_texts().values():
+        for m in re.finditer(r"\.amdhsa_kernel (\S+)\n(.*?)\.end_amdhsa_kernel", text, re.S):
+            size = re.search(r"\.amdhsa_private_segment_fixed_size (\d+)", m.group(2))
+            if size and int(size.group(1)) > _SCRATCH_ALLOWED.get(m.group(1), 0):
+                over.append((m.group(1), int(size.group(1))))
+    assert not over, over
