@@ -1560,6 +1560,50 @@ int karatsuba_P(int64_t Lp) {
 
 const int kKaratsubaMaxP = 4608;
 
+// Pinned host staging for the small per-call tables copied to the device (the
+// Karatsuba pair table): a pool of hipHostMalloc buffers, each reused once the
+// event recorded after its copy reports completion, so a call neither copies
+// from pageable memory nor waits on the stream.  The buffers live until the
+// process ends.
+struct Staging {
+    void *host = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+};
+std::mutex g_stage_mu;
+std::vector<Staging *> g_stage_free;
+
+af_status stage_to_device(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    Staging *b = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_stage_mu);
+        for (size_t i = 0; i < g_stage_free.size(); ++i)
+            if (g_stage_free[i]->cap >= bytes && hipEventQuery(g_stage_free[i]->done) == hipSuccess) {
+                b = g_stage_free[i];
+                g_stage_free.erase(g_stage_free.begin() + (std::ptrdiff_t)i);
+                break;
+            }
+    }
+    if (!b) {
+        std::unique_ptr<Staging> nb(new Staging);
+        nb->cap = std::max<size_t>(bytes, (size_t)64 << 10);
+        HIPCHK(hipHostMalloc(&nb->host, nb->cap, hipHostMallocDefault));
+        const hipError_t e = hipEventCreateWithFlags(&nb->done, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            (void)hipHostFree(nb->host);
+            HIPCHK(e);
+        }
+        b = nb.release();
+    }
+    std::memcpy(b->host, src, bytes);
+    af_status st = HIPST(hipMemcpyAsync(dst, b->host, bytes, hipMemcpyHostToDevice, s));
+    if (st == AF_SUCCESS) st = HIPST(hipEventRecord(b->done, s));
+    if (st != AF_SUCCESS) (void)hipStreamSynchronize(s);  // the buffer is free again either way
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    g_stage_free.push_back(b);
+    return st;
+}
+
 af_status karatsuba_run(const std::vector<int64_t> &meta_in, int64_t npairs, int maxP, const int64_t *x,
                         const int64_t *y, int64_t *z, int cyclic, hipStream_t s) {
     AFCHK(device_init());
@@ -1576,10 +1620,7 @@ af_status karatsuba_run(const std::vector<int64_t> &meta_in, int64_t npairs, int
         for (int j = 0; j < 3; ++j) meta[3 * i + j] = meta_in[3 * order[i] + j];
     void *dm = nullptr;
     HIPCHK(hipMallocAsync(&dm, meta.size() * 8, s));
-    af_status st = AF_SUCCESS;
-    { const af_status h__ = HIPST(hipMemcpyAsync(dm, meta.data(), meta.size() * 8, hipMemcpyHostToDevice, s)); if (st == AF_SUCCESS) st = h__; }
-    // the host vector dies with this frame: wait for the copy before returning
-    if (st == AF_SUCCESS) st = HIPST(hipStreamSynchronize(s));
+    af_status st = stage_to_device(dm, meta.data(), meta.size() * 8, s);
     if (st == AF_SUCCESS) st = HIPST(launch_karatsuba(x, y, z, (const int64_t *)dm, npairs, maxP, cyclic, s));
     (void)hipFreeAsync(dm, s);
     return st;

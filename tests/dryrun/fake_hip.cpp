@@ -629,6 +629,16 @@ hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) {
     return hipSuccess;
 }
 hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
+// pinned host memory: plain host memory here (never a kernel operand)
+hipError_t hipHostMalloc(void **p, size_t n, unsigned int) {
+    *p = std::malloc(n ? n : 1);
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+hipError_t hipHostFree(void *p) {
+    std::free(p);
+    return hipSuccess;
+}
 hipError_t hipMemcpy(void *dst, const void *src, size_t n, hipMemcpyKind kind) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (kind == hipMemcpyHostToDevice || kind == hipMemcpyDeviceToDevice) in_alloc(dst, 0, n, "memcpy dst", "hipMemcpy");
