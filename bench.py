@@ -369,6 +369,15 @@ def main():
     }
     if tune_report:
         out["config"]["autotune"] = [l for l in tune_report.strip().split("\n")]
+    # a second denominator: the box's own device-to-device copy of the same
+    # 32 MiB array (reads and writes 32 MiB each)
+    src = torch.empty(batch * n * 2, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    msc = time_it(lambda: dst.copy_(src), steps, warmup)
+    copy_gbps = alg_bytes / (msc * 1e-3) / 1e9
+    out["roofline"]["device_copy_GBps"] = copy_gbps
+    out["roofline"]["achieved_over_device_copy"] = achieved / copy_gbps
+    del src, dst
     y32 = f32.result()
 
     if not args.skip_extras:
@@ -380,9 +389,7 @@ def main():
         y64 = f64.result()
         extras["fp32_rel_l2_vs_fp64"] = rel_l2(y32, y64)
         extras["fp32_max_abs_vs_fp64"] = float(np.max(np.abs(y32 - y64)))
-        import oracle
-        xs = x[:2]
-        extras["fp64_rel_l2_vs_oracle_2rows"] = rel_l2(y64[:2], oracle.transform(-1, xs))
+        extras["fp64_rel_l2_vs_numpy_2rows"] = rel_l2(y64[:2], np.fft.fft(x[:2], axis=-1))
         extras["fp64_same_workload_ms"] = ms64
         extras["fp32_over_fp64_speedup"] = ms64 / ms
         del f64
@@ -406,6 +413,15 @@ def main():
         ms1 = time_it(c1.run, steps, warmup)
         extras["c1_f64_256x1024_us"] = ms1 * 1e3
         extras["c1_f64_GBps"] = 2 * 16 * 1024 * 256 / (ms1 * 1e-3) / 1e9
+        # forward, then inverse, then scale by 1/N: the round trip and the
+        # forward spectrum against numpy's FP64 FFT
+        y1 = c1.result()
+        c1i = C2C(1024, 256, _abi.F64, sign=1)
+        c1i.set_io(y1)
+        c1i.run()
+        extras["c1_forward_rel_l2_vs_numpy"] = rel_l2(y1, np.fft.fft(x1, axis=-1))
+        extras["c1_roundtrip_rel_l2"] = rel_l2(c1i.result() / 1024, x1)
+        del c1, c1i
         # real convolution and its adjoint, FP32 512 x 8192
         fb, N = conv_workload(torch)
         msc = time_it(fb, steps, warmup)
@@ -440,6 +456,20 @@ def main():
                                "cores": threads, "kind": "port",
                                "sample": f"{rows} of the 64 rows x 65536, FP32 oracle (C restatement of the "
                                          f"reference's split-radix plan), {reps} repetitions over ~10 s"}
+        # a separate, labelled datum: scipy's pocketfft on the same rows and threads
+        try:
+            import scipy.fft as sfft
+            x16 = xs.astype(np.complex64)
+            sfft.fft(x16, axis=-1, workers=threads)
+            t0 = time.perf_counter()
+            reps = 0
+            while time.perf_counter() - t0 < 2.0:
+                sfft.fft(x16, axis=-1, workers=threads)
+                reps += 1
+            dt = (time.perf_counter() - t0) / reps
+            out["cpu_baseline"]["scipy_pocketfft_f32_GFLOPs"] = 5.0 * n * math.log2(n) * rows / dt / 1e9
+        except ImportError:
+            pass
     print(json.dumps(out))
 
 
