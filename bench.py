@@ -410,6 +410,8 @@ def main():
         x1 = synthetic.complex_uniform(0, (256, 1024))
         c1 = C2C(1024, 256, _abi.F64)
         c1.set_io(x1)
+        if not args.no_tune:
+            extras["c1_autotune"] = c1.tune().strip().split("\n")
         ms1 = time_it(c1.run, steps, warmup)
         extras["c1_f64_256x1024_us"] = ms1 * 1e3
         extras["c1_f64_GBps"] = 2 * 16 * 1024 * 256 / (ms1 * 1e-3) / 1e9
