@@ -521,7 +521,7 @@ struct af_plan_s {
     Core core;
     Owner own;
     int32_t *in_map = nullptr, *out_map = nullptr;  // Rader gathers / scatters
-    void *chirp = nullptr;                          // Bluestein chirp, table format, length n
+    void *chirp = nullptr;                          // Bluestein chirp, plain complex values, length n
     void *spec = nullptr;                           // convolution spectrum, plain complex of the precision
     void *aux_tw = nullptr;                         // real packing twiddles W_n^k, k < n/2
     af_plan dual = nullptr;                         // the opposite sign (adjoint of Rader / Bluestein)
@@ -678,8 +678,9 @@ af_status build_bluestein(af_plan P, hipStream_t s) {
     af_status st = AF_SUCCESS;
     // chirp c_k = exp(sign i pi k^2 / n), k < n; kernel h = conj(c) wrapped over M
     { const af_status h__ = HIPST(gen_chirp_d(cd, n, n, P->desc.sign, 0, 0, s)); if (st == AF_SUCCESS) st = h__; }
-    if (st == AF_SUCCESS) st = P->own.alloc(&P->chirp, n * 16);
-    if (st == AF_SUCCESS) st = HIPST(convert_to_w(P->prec, P->chirp, cd, n, 1.0, 0, 0, s));
+    // plain complex values of the precision (FP32: each value rounded once from FP64)
+    if (st == AF_SUCCESS) st = P->own.alloc(&P->chirp, n * elem_bytes(P->desc.precision));
+    if (st == AF_SUCCESS) st = HIPST(convert_to_plain(P->prec, P->chirp, cd, n, 1.0, 0, 0, s));
     if (st == AF_SUCCESS) st = HIPST(gen_chirp_d(hb, n, M, P->desc.sign, M, 1, s));
     if (st == AF_SUCCESS) st = fft_f64_forward(M, hb, hf, s);
     if (st == AF_SUCCESS) st = P->own.alloc(&P->spec, M * elem_bytes(P->desc.precision));

@@ -66,7 +66,7 @@ struct AfPass {
     int64_t in_es, in_ls, in_bs;
     int64_t in_len;               // positions >= in_len read as zero
     const int32_t *in_map;        // gather: address r * in_bs + in_map[pos]
-    const void *in_mul;           // multiply by in_mul[pos] (twiddle-table format)
+    const void *in_mul;           // multiply by in_mul[pos] (plain complex values of the precision)
     // stage twiddles W_L^m, m < L, forward sign, twiddle-table format
     const void *tw;
     // convolution spectrum, element (l, e) of row r at spec[r*spec_bs + l*spec_ls + e]

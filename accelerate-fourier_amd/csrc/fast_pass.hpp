@@ -193,7 +193,7 @@ __device__ __forceinline__ void fast_load(const AfPass &d, const FastRow &r, typ
 
 // First-stage load of the extended kernel: as fast_load, and element e of
 // line l (row position pos = l in_ls + e in_es) reads as zero from
-// pos >= in_len on and is multiplied by in_mul[pos] (twiddle-table format)
+// pos >= in_len on and is multiplied by in_mul[pos] (plain complex values)
 // when in_mul is set -- the Bluestein chirp and zero padding.
 template <typename T, int L, int B>
 __device__ __forceinline__ void fast_load_ext(const AfPass &d, const FastRow &r, typename Cx<T>::C (&v)[16]) {
@@ -217,7 +217,7 @@ __device__ __forceinline__ void fast_load_ext(const AfPass &d, const FastRow &r,
             const uint32_t off = o0 + (uint32_t)j * step;
             if (r.in_planar) x = mk(bld<T, 0>(r.in_re, off, 0), bld<T, 0>(r.in_im, off, 0));
             else x = bld<C, 0>(r.in_re, off, 0);
-            if (mul) x = vmul(x, gld<W>(d.in_mul, pos), false);
+            if (mul) x = cmul(x, gld<C>(d.in_mul, pos));
         }
         v[j] = x;
     }
@@ -277,7 +277,7 @@ __device__ __forceinline__ void fast_store_ext(const AfPass &d, const FastRow &r
         }
         const uint32_t pos = p0 + (uint32_t)j * pstep;
         if (pos < plen) {
-            if (mul) x = vmul(x, gld<W>(d.out_mul, pos), false);
+            if (mul) x = cmul(x, gld<C>(d.out_mul, pos));
             const uint32_t off = o0 + (uint32_t)j * step;
             if (PLANAR) {
                 bst<T>(r.out_re, off, 0, x.x);

@@ -493,7 +493,7 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
                 if (pos < d.in_len) {
                     const int64_t a = d.in_map ? (int64_t)gld<int32_t>(d.in_map, pos) : pos;
                     v = ld_c<T>(d.in_re, d.in_im, rb + a);
-                    if (d.in_mul) v = vmul(v, gld<W>(d.in_mul, pos), plain);
+                    if (d.in_mul) v = cmul(v, gld<C>(d.in_mul, pos));
                 }
             }
         }
@@ -583,7 +583,7 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
             // the lines of a four-step pass number N / L, so l * e < N: no reduction
             v = cmul(v, ft_twiddle_t(d, (uint64_t)((uint32_t)l * (uint32_t)e), T(0)));
         }
-        if (d.out_mul) v = vmul(v, gld<W>(d.out_mul, o), plain);
+        if (d.out_mul) v = cmul(v, gld<C>(d.out_mul, o));
         if (flags & AFP_SCALE) v = cscale(v, T(d.scale));
         const int64_t ob = r * d.out_bs;
         if (rader_any) {

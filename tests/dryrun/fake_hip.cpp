@@ -182,7 +182,7 @@ void check_pass(const AfPass &d, int tsz, const std::string &k) {
             if (d.in_map) in_alloc(d.in_map, 0, (maxpos + 1) * 4, "in_map", k);
             if (amin < 0) problem(k + ": negative input index");
             check_c(d.in_re, d.in_im, std::min<int64_t>(amin, 0), R1 * d.in_bs + amax + 1, tsz, "in", k);
-            if (d.in_mul) in_alloc(d.in_mul, 0, (maxpos + 1) * 16, "in_mul", k);
+            if (d.in_mul) in_alloc(d.in_mul, 0, (maxpos + 1) * csz, "in_mul", k);  // plain complex values
         }
     }
     if (d.flags & (AFP_RADER | AFP_X0_CAPTURE)) check_c(d.in_re, d.in_im, 0, R1 * d.in_bs + 1, tsz, "in x0", k);
@@ -212,7 +212,7 @@ void check_pass(const AfPass &d, int tsz, const std::string &k) {
     if (maxo >= 0) {
         if (amin < 0) problem(k + ": negative output index");
         check_c(d.out_re, d.out_im, std::min<int64_t>(amin, 0), R1 * d.out_bs + amax + 1, tsz, "out", k);
-        if (d.out_mul) in_alloc(d.out_mul, 0, (maxo + 1) * 16, "out_mul", k);
+        if (d.out_mul) in_alloc(d.out_mul, 0, (maxo + 1) * csz, "out_mul", k);
     }
     if (d.flags & AFP_POSTTW) {
         if (d.ftN < 1) problem(k + ": POSTTW without ftN");
