@@ -24,8 +24,6 @@ __global__ void __launch_bounds__(B * L / 16)
 k_fast(const AfPass d) {
     using C = typename Cx<T>::C;
     using W = typename Cx<T>::W;
-    using FT = FastTables<L>;
-    constexpr int NT = B * L / 16;
     constexpr int LD = fast_ld(L, B);
     extern __shared__ __align__(16) unsigned char smem_fast[];
     C *lds = (C *)smem_fast;
@@ -215,9 +213,7 @@ k_fast_rconv(const AfPass d, const T *__restrict__ in0, const T *__restrict__ in
              T *__restrict__ out0, T *__restrict__ out1) {
     using C = typename Cx<T>::C;
     using W = typename Cx<T>::W;
-    using FT = FastTables<L>;
     using RP = RconvPairs<L, B>;
-    constexpr int NT = B * L / 16;
     constexpr int LD = fast_ld(L, B);
     constexpr bool ONE = sizeof(T) == 4;  // one line buffer (FP32) or two (FP64)
     extern __shared__ __align__(16) unsigned char smem_rc[];

@@ -198,7 +198,6 @@ __device__ __forceinline__ void fast_load(const AfPass &d, const FastRow &r, typ
 template <typename T, int L, int B>
 __device__ __forceinline__ void fast_load_ext(const AfPass &d, const FastRow &r, typename Cx<T>::C (&v)[16]) {
     using C = typename Cx<T>::C;
-    using W = typename Cx<T>::W;
     constexpr int NB = L / 16;
     int b, i;
     bf_map<B, NB>(threadIdx.x, d.in_lines_fast != 0, b, i);
@@ -263,7 +262,6 @@ __device__ __forceinline__ void fast_store_ext(const AfPass &d, const FastRow &r
                                                const double2 tw_step, const T sc, const uint32_t o0,
                                                const uint32_t step, const uint32_t p0, const uint32_t pstep) {
     using C = typename Cx<T>::C;
-    using W = typename Cx<T>::W;
     const uint32_t plen = d.out_len < (int64_t)UINT32_MAX ? (uint32_t)d.out_len : UINT32_MAX;
     const bool mul = d.out_mul != nullptr;
 #pragma unroll
@@ -306,7 +304,6 @@ __device__ __forceinline__ double2 fast_ft(const AfPass &d, uint32_t a) {
 template <typename T, int L, int B, int SG, bool EXT = false>
 __device__ __forceinline__ void fast_stages(const AfPass &d, const FastRow &r, typename Cx<T>::C (&v)[16],
                                             typename Cx<T>::C *lds, const typename Cx<T>::W *stab) {
-    using C = typename Cx<T>::C;
     using PL = P2Plan<L>;
     using FT = FastTables<L>;
     constexpr int NT = B * L / 16;

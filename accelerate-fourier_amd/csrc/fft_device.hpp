@@ -312,7 +312,6 @@ __device__ __forceinline__ void stage(typename Cx<T>::C *lds, const int L, const
 
 template <typename T, typename D>
 __device__ __forceinline__ void run_stages(typename Cx<T>::C *lds, const D &d, const T s) {
-    using W = typename Cx<T>::W;
     const void *tw = d.tw;
     const bool plain = (d.flags & AFP_PLAIN_TW) != 0;
     for (int st = 0; st < d.nst; ++st) {

@@ -181,30 +181,38 @@ k_karatsuba(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__restric
                 // lane 0), it sees y_(l-i) for i <= l and 0 otherwise.  The blocks are zero
                 // beyond s, so lane l's z_l is the shifted sum and z_(l+64) is the rotated sum
                 // minus the shifted one.
+                // (steps run in groups of four up to s rounded up: x_i is zero for
+                // i >= s, so the extra steps add nothing, and the groups unroll)
+                const int s4 = (s + 3) & ~3;
                 u64 lo = 0, hi = 0;
                 if (exact_f64) {
                     // FP64 fused multiply-adds: exact, see above
                     const double Xd = (double)(int64_t)X;
                     double yr = (double)(int64_t)Y, ys = yr, dt = 0.0, dl = 0.0;
-#pragma unroll 4
-                    for (int i = 0; i < s; ++i) {
-                        const double xi = __longlong_as_double((long long)readlane64((u64)__double_as_longlong(Xd), i));
-                        dt = fma(xi, yr, dt);
-                        dl = fma(xi, ys, dl);
-                        yr = __longlong_as_double((long long)rot1((u64)__double_as_longlong(yr)));
-                        ys = __longlong_as_double((long long)shr1((u64)__double_as_longlong(ys)));
+                    for (int i0 = 0; i0 < s4; i0 += 4) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const double xi =
+                                __longlong_as_double((long long)readlane64((u64)__double_as_longlong(Xd), i0 + k));
+                            dt = fma(xi, yr, dt);
+                            dl = fma(xi, ys, dl);
+                            yr = __longlong_as_double((long long)rot1((u64)__double_as_longlong(yr)));
+                            ys = __longlong_as_double((long long)shr1((u64)__double_as_longlong(ys)));
+                        }
                     }
                     lo = (u64)(int64_t)dl;
                     hi = (u64)(int64_t)(dt - dl);
                 } else {
                     u64 yr = Y, ys = Y, t = 0;
-#pragma unroll 4
-                    for (int i = 0; i < s; ++i) {
-                        const u64 xi = readlane64(X, i);
-                        t += xi * yr;
-                        lo += xi * ys;
-                        yr = rot1(yr);
-                        ys = shr1(ys);
+                    for (int i0 = 0; i0 < s4; i0 += 4) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const u64 xi = readlane64(X, i0 + k);
+                            t += xi * yr;
+                            lo += xi * ys;
+                            yr = rot1(yr);
+                            ys = shr1(ys);
+                        }
                     }
                     hi = t - lo;
                 }
