@@ -119,10 +119,11 @@ __device__ __forceinline__ void fast_pairs_read(const typename Cx<T>::C *A, type
     }
 }
 
-// Per pair: untangle both half-length spectra (the first from registers, the
-// second from Bb) to full spectra, multiply (by the conjugate of the second
-// for the adjoint's correlations) and entangle the product back into Bb as the
-// half-length spectrum of the inverse.  Each pair has one owner thread.
+// Per pair: the product of the two full spectra (by the conjugate of the
+// second for the adjoint's correlations), entangled back into Bb as the
+// half-length spectrum of the inverse, in the closed form of rconv_pair (the
+// first spectrum from registers, the second from Bb).  Each pair has one owner
+// thread.
 template <typename T, int L, int B>
 __device__ __forceinline__ void fast_spec_product(const typename Cx<T>::C (&p0)[RconvPairs<L, B>::NP],
                                                   const typename Cx<T>::C (&p1)[RconvPairs<L, B>::NP],
@@ -137,20 +138,8 @@ __device__ __forceinline__ void fast_spec_product(const typename Cx<T>::C (&p0)[
         const int b = idx / RP::H, e = idx - b * RP::H;
         const int er = (L - e) & (L - 1);
         const int ie = b * LD + fpad(e), ir = b * LD + fpad(er);
-        const C a0 = p0[q], a1 = p1[q], b0 = Bb[ie], b1 = Bb[ir];
-        C sa_lo, sa_hi, sb_lo, sb_hi;
-        untangle<T>(a0, a1, aux[e], false, sa_lo, sa_hi);
-        untangle<T>(b0, b1, aux[e], false, sb_lo, sb_hi);
-        if (conj_b) { sb_lo = cconj(sb_lo); sb_hi = cconj(sb_hi); }
-        const C z0 = entangle<T>(cmul(sa_lo, sb_lo), cmul(sa_hi, sb_hi), aux[e], false);
-        C z1 = z0;
-        if (er != e) {
-            C ta_lo, ta_hi, tb_lo, tb_hi;
-            untangle<T>(a1, a0, aux[er], false, ta_lo, ta_hi);
-            untangle<T>(b1, b0, aux[er], false, tb_lo, tb_hi);
-            if (conj_b) { tb_lo = cconj(tb_lo); tb_hi = cconj(tb_hi); }
-            z1 = entangle<T>(cmul(ta_lo, tb_lo), cmul(ta_hi, tb_hi), aux[er], false);
-        }
+        C z0, z1;
+        rconv_pair<T>(p0[q], p1[q], Bb[ie], Bb[ir], aux[e], false, conj_b, z0, z1);
         Bb[ie] = z0;
         if (er != e) Bb[ir] = z1;
     }
@@ -169,20 +158,8 @@ __device__ __forceinline__ void fast_spec_product2(const typename Cx<T>::C *A, c
         const int b = idx / H, e = idx - b * H;
         const int er = (L - e) & (L - 1);
         const int ie = b * LD + fpad(e), ir = b * LD + fpad(er);
-        const C a0 = A[ie], a1 = A[ir], b0 = Bb[ie], b1 = Bb[ir];
-        C sa_lo, sa_hi, sb_lo, sb_hi;
-        untangle<T>(a0, a1, aux[e], false, sa_lo, sa_hi);
-        untangle<T>(b0, b1, aux[e], false, sb_lo, sb_hi);
-        if (conj_b) { sb_lo = cconj(sb_lo); sb_hi = cconj(sb_hi); }
-        const C z0 = entangle<T>(cmul(sa_lo, sb_lo), cmul(sa_hi, sb_hi), aux[e], false);
-        C z1 = z0;
-        if (er != e) {
-            C ta_lo, ta_hi, tb_lo, tb_hi;
-            untangle<T>(a1, a0, aux[er], false, ta_lo, ta_hi);
-            untangle<T>(b1, b0, aux[er], false, tb_lo, tb_hi);
-            if (conj_b) { tb_lo = cconj(tb_lo); tb_hi = cconj(tb_hi); }
-            z1 = entangle<T>(cmul(ta_lo, tb_lo), cmul(ta_hi, tb_hi), aux[er], false);
-        }
+        C z0, z1;
+        rconv_pair<T>(A[ie], A[ir], Bb[ie], Bb[ir], aux[e], false, conj_b, z0, z1);
         dst[ie] = z0;
         if (er != e) dst[ir] = z1;
     }

@@ -419,6 +419,43 @@ __device__ __forceinline__ typename Cx<T>::C entangle(typename Cx<T>::C p0, type
     return cadd(cadd(p0, p1), mk(-t.y, t.x));
 }
 
+// The spectrum product of the real convolution on one pair (e, r = L - e) of
+// half-length spectra, untangle -> product -> entangle in closed form.  With
+// w = W_{2L}^e, the untangled values are S_e = E + w O, S_{e+L} = E - w O for
+// E = (Z_e + conj Z_r) / 2, D = (Z_e - conj Z_r) / 2, O = -i D; putting the
+// products of two such spectra through the entangling gives
+//   z_e = 2 A_e B_e - c (A_e - conj A_r)(B_e - conj B_r),   c = (1 + w^2) / 2,
+//   z_r = 2 A_r B_r - conj of the same correction,
+// and with the second spectrum conjugated (the adjoint's correlations)
+//   z_e = 2 A_e conj B_e - conj(c) (A_e + conj A_r) conj(B_e - conj B_r),
+//   z_r = 2 A_r conj B_r + conj of that correction:
+// four complex products and one by c per pair instead of three twiddle
+// products and two spectrum products per element.  c = (cos^2, cos sin) of
+// w's angle, formed from both parts of the table entry.
+__device__ __forceinline__ float2 half_one_plus_sq(float4 w, bool plain) {
+    if (plain) return make_float2(w.x * w.x, w.x * w.y);
+    return make_float2(fmaf(w.x, w.x, 2.0f * w.x * w.z), fmaf(w.x, w.y, fmaf(w.x, w.w, w.z * w.y)));
+}
+__device__ __forceinline__ double2 half_one_plus_sq(double2 w, bool) { return make_double2(w.x * w.x, w.x * w.y); }
+
+template <typename T>
+__device__ __forceinline__ void rconv_pair(typename Cx<T>::C ae, typename Cx<T>::C ar, typename Cx<T>::C be,
+                                           typename Cx<T>::C br, typename Cx<T>::W w, bool plain, bool conj_b,
+                                           typename Cx<T>::C &ze, typename Cx<T>::C &zr) {
+    using C = typename Cx<T>::C;
+    const C c = half_one_plus_sq(w, plain);
+    const C db = csub(be, cconj(br));
+    if (!conj_b) {
+        const C q = cmul(c, cmul(csub(ae, cconj(ar)), db));
+        ze = csub(cscale(cmul(ae, be), T(2)), q);
+        zr = csub(cscale(cmul(ar, br), T(2)), cconj(q));
+    } else {
+        const C q = cmul(cconj(c), cmul(cadd(ae, cconj(ar)), cconj(db)));
+        ze = csub(cscale(cmul(ae, cconj(be)), T(2)), q);
+        zr = cadd(cscale(cmul(ar, cconj(br)), T(2)), cconj(q));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // One tile of a pass: load -> stages -> [spectrum product -> inverse stages]
 // -> store.

@@ -115,21 +115,9 @@ __device__ __forceinline__ void spectral_product(typename Cx<T>::C *A, typename 
     const W *aux = (const W *)d.aux_tw;
     for (int e = threadIdx.x; e <= L / 2; e += blockDim.x) {
         const int er = (L - e) % L;
-        const C a0 = A[pidx(e, ps)], a1 = A[pidx(er, ps)];
-        const C b0 = Bb[pidx(e, ps)], b1 = Bb[pidx(er, ps)];
-        C sa_lo, sa_hi, sb_lo, sb_hi;
-        untangle<T>(a0, a1, aux[e], plain, sa_lo, sa_hi);
-        untangle<T>(b0, b1, aux[e], plain, sb_lo, sb_hi);
-        if (conj_b) { sb_lo = cconj(sb_lo); sb_hi = cconj(sb_hi); }
-        const C z0 = entangle<T>(cmul(sa_lo, sb_lo), cmul(sa_hi, sb_hi), aux[e], plain);
-        C z1 = z0;
-        if (er != e) {
-            C ta_lo, ta_hi, tb_lo, tb_hi;
-            untangle<T>(a1, a0, aux[er], plain, ta_lo, ta_hi);
-            untangle<T>(b1, b0, aux[er], plain, tb_lo, tb_hi);
-            if (conj_b) { tb_lo = cconj(tb_lo); tb_hi = cconj(tb_hi); }
-            z1 = entangle<T>(cmul(ta_lo, tb_lo), cmul(ta_hi, tb_hi), aux[er], plain);
-        }
+        C z0, z1;
+        rconv_pair<T>(A[pidx(e, ps)], A[pidx(er, ps)], Bb[pidx(e, ps)], Bb[pidx(er, ps)], aux[e], plain, conj_b, z0,
+                      z1);
         dst[pidx(e, ps)] = z0;
         if (er != e) dst[pidx(er, ps)] = z1;
     }
