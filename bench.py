@@ -284,7 +284,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: every box has one MI355X, and the "
-                                   "transform path shards only across independent rows (see DESIGN.md)",
+                                   "transform path shards only across independent rows",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import numpy as np
