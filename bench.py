@@ -283,8 +283,8 @@ def rel_l2(a, b):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU: every box has one MI355X, and the "
-                                   "transform path shards only across independent rows",
+        print(json.dumps({"error": "one GPU only: the transform path shards across independent rows "
+                                   "without communication, so its figure is the one-GPU throughput",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import numpy as np
