@@ -707,7 +707,7 @@ const char *af_status_string(af_status s) {
     switch (s) {
     case AF_SUCCESS: return "success";
     case AF_INVALID_ARG: return "invalid argument (a precondition of the reference is violated)";
-    case AF_UNSUPPORTED: return "unsupported by this build";
+    case AF_UNSUPPORTED: return "unsupported by this library";
     case AF_NO_DEVICE: return "no gfx950 device";
     case AF_HIP_ERROR: return "HIP runtime error";
     case AF_OUT_OF_MEMORY: return "out of device memory";

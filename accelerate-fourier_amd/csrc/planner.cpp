@@ -211,7 +211,7 @@ bool plan_shape(int prec, int64_t n, int algorithm, PlanShape &out, std::string 
             }
         }
         if (algorithm == AF_ALG_DECOMPOSE) {
-            why = "length needs nested Rader transforms, which this build only serves by Bluestein";
+            why = "length needs nested Rader transforms, which this library only serves by Bluestein";
             return false;
         }
     }

@@ -369,7 +369,7 @@ def main():
     }
     if tune_report:
         out["config"]["autotune"] = [l for l in tune_report.strip().split("\n")]
-    # a second denominator: the box's own device-to-device copy of the same
+    # a second denominator: the device's own device-to-device copy of the same
     # 32 MiB array (reads and writes 32 MiB each)
     src = torch.empty(batch * n * 2, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)

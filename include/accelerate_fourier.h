@@ -34,7 +34,7 @@ extern "C" {
 typedef enum {
     AF_SUCCESS = 0,
     AF_INVALID_ARG = 1,   /* a documented precondition of the reference is violated */
-    AF_UNSUPPORTED = 2,   /* a valid request this build cannot serve */
+    AF_UNSUPPORTED = 2,   /* a valid request this library cannot serve */
     AF_NO_DEVICE = 3,     /* no gfx950 device is visible */
     AF_HIP_ERROR = 4,     /* a HIP runtime call failed */
     AF_OUT_OF_MEMORY = 5,

@@ -719,7 +719,7 @@ hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
     return hipSuccess;
 }
 
-// ---- reporting (for the driver) ----
+// ---- reporting (read by tests/test_dryrun.py) ----
 void dry_context(const char *c) {
     std::lock_guard<std::mutex> lk(g_mu);
     g_context = c ? c : "";
