@@ -148,7 +148,7 @@ bool fast_ext_eligible(int prec, const AfPass &d) {
     if (conv && (d.in_lines_fast || !d.spec)) return false;
     if (!ext_has_shape(prec, d.L, d.B, conv)) return false;
     if (!d.tw_d) return false;
-    if (d.L <= 1024 && !d.tw) return false;
+    if (!d.tw) return false;  // the LDS stage table (W_L, or W_256 from it) at every length
     if ((d.flags & AFP_POSTTW) && (!d.ft_coarse_d || !d.ft_fine_d || (d.ftN & (d.ftN - 1)))) return false;
     if (d.nlines % d.B) return false;
     if (d.ntiles >= ((int64_t)1 << 31)) return false;

@@ -285,7 +285,8 @@ bool fast_pass_eligible(int prec, const AfPass &d) {
     if (d.in_map || d.out_map || d.in_mul || d.out_mul) return false;
     if (d.in_len != INT64_MAX || d.out_len != INT64_MAX) return false;
     if (d.L < 256 || d.L > 4096 || (d.L & (d.L - 1))) return false;
-    if (!d.tw_d || ((d.flags & AFP_POSTTW) && (!d.ft_coarse_d || !d.ft_fine_d))) return false;
+    // the W_L table of the working precision feeds the LDS stage table at every length
+    if (!d.tw || !d.tw_d || ((d.flags & AFP_POSTTW) && (!d.ft_coarse_d || !d.ft_fine_d))) return false;
     if ((d.flags & AFP_POSTTW) && (d.ftN & (d.ftN - 1))) return false;
     if (d.nlines % d.B) return false;
     if (d.ntiles >= ((int64_t)1 << 31)) return false;
