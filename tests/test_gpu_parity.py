@@ -37,7 +37,7 @@ def host(t):
 # lengths covering every device plan: single-pass Stockham (all radices),
 # four-step, Rader (single and four-step core) and Bluestein (single, four-step)
 LENGTHS_SMALL = list(range(0, 41)) + [48, 60, 64, 77, 97, 100, 121, 127, 128, 169, 255, 256, 257]
-LENGTHS_MID = [1000, 1024, 2310, 4096, 4099, 8191, 8192, 10007, 15015, 16384]
+LENGTHS_MID = [1000, 1024, 2048, 2310, 4096, 4099, 8191, 8192, 10007, 15015, 16384]
 LENGTHS_BIG = [16385 * 0 + 20000, 32768, 65521, 65536, 100003]
 
 
@@ -150,17 +150,26 @@ def test_real_spectrum(n, prec):
     assert rel_l2(host(y), oracle.from_spectrum(ref)) <= tol
 
 
+@pytest.mark.parametrize("n", [100, 1024, 2048, 4096, 8192])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_real_convolution(prec):
+def test_real_convolution(n, prec):
+    """The fused real convolution and its adjoint against the oracle's
+    fromSpectrum(toSpectrum x * toSpectrum k) and the correlations
+    fromSpectrum(S_w conj S_k), fromSpectrum(S_w conj S_x): the generic kernel
+    (n = 100) and the power-of-two one at half lengths 512 ... 4096 (the
+    register-twiddle form from 2048 on)."""
     import oracle
     rt = torch.float64 if prec == "f64" else torch.float32
     tol = 1e-12 if prec == "f64" else 1e-6
-    n = 1024
-    x = synthetic.normal(3, 4 * n).reshape(4, n)
-    k = synthetic.normal(4, 4 * n).reshape(4, n) / np.sqrt(n)
+    x = synthetic.normal(3 + n, 4 * n).reshape(4, n)
+    k = synthetic.normal(4 + n, 4 * n).reshape(4, n) / np.sqrt(n)
+    w = synthetic.normal(5 + n, 4 * n).reshape(4, n)
     y = real.convolve(to_dev(x, rt), to_dev(k, rt))
-    ref = oracle.from_spectrum(oracle.to_spectrum(x) * oracle.to_spectrum(k))
-    assert rel_l2(host(y), ref) <= tol
+    Sx, Sk, Sw = oracle.to_spectrum(x), oracle.to_spectrum(k), oracle.to_spectrum(w)
+    assert rel_l2(host(y), oracle.from_spectrum(Sx * Sk)) <= tol
+    dx, dk = real.convolve_vjp(to_dev(x, rt), to_dev(k, rt), to_dev(w, rt))
+    assert rel_l2(host(dx), oracle.from_spectrum(Sw * np.conj(Sk))) <= tol
+    assert rel_l2(host(dk), oracle.from_spectrum(Sw * np.conj(Sx))) <= tol
 
 
 @pytest.mark.parametrize("n", [1, 2, 7, 64, 100, 97, 1024, 20000])
