@@ -392,6 +392,14 @@ def main():
         extras["fp64_rel_l2_vs_numpy_2rows"] = rel_l2(y64[:2], np.fft.fft(x[:2], axis=-1))
         extras["fp64_same_workload_ms"] = ms64
         extras["fp32_over_fp64_speedup"] = ms64 / ms
+        # external comparator (timing only): the vendor FFT behind torch.fft on ROCm,
+        # same 64 x 65536 Complex Float, interleaved, out of place
+        xt = torch.from_numpy(x).to("cuda", torch.complex64)
+        yt = torch.empty_like(xt)
+        msr = time_it(lambda: torch.fft.fft(xt, dim=-1, out=yt), steps, warmup)
+        extras["comparator_torch_fft_rocfft_64x65536_f32_ms"] = msr
+        extras["comparator_rel_l2_vs_this"] = rel_l2(yt.cpu().numpy(), y32)
+        del xt, yt
         del f64
         # one 2^24 transform
         n24 = 1 << 24
