@@ -414,6 +414,11 @@ def main():
         y24 = g.result()
         extras["c2b_2p24_rel_l2_vs_numpy_fp64"] = rel_l2(y24[0], np.fft.fft(x24[0]))
         del g
+        xt = torch.from_numpy(x24).to("cuda", torch.complex64)
+        yt = torch.empty_like(xt)
+        extras["comparator_torch_fft_rocfft_2p24_f32_ms"] = time_it(lambda: torch.fft.fft(xt, dim=-1, out=yt),
+                                                                     max(steps // 2, 3), 2)
+        del xt, yt
         # plain FP64, 256 x 1024
         x1 = synthetic.complex_uniform(0, (256, 1024))
         c1 = C2C(1024, 256, _abi.F64)
@@ -432,6 +437,11 @@ def main():
         extras["c1_forward_rel_l2_vs_numpy"] = rel_l2(y1, np.fft.fft(x1, axis=-1))
         extras["c1_roundtrip_rel_l2"] = rel_l2(c1i.result() / 1024, x1)
         del c1, c1i
+        xt = torch.from_numpy(x1).to("cuda", torch.complex128)
+        yt = torch.empty_like(xt)
+        extras["comparator_torch_fft_rocfft_c1_f64_us"] = time_it(lambda: torch.fft.fft(xt, dim=-1, out=yt),
+                                                                   steps, warmup) * 1e3
+        del xt, yt
         # real convolution and its adjoint, FP32 512 x 8192
         fb, N = conv_workload(torch)
         msc = time_it(fb, steps, warmup)
