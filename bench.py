@@ -448,6 +448,17 @@ def main():
         extras["c4_conv_fwd_bwd_ms"] = msc
         extras["c4_conv_per_s"] = 512 / (msc * 1e-3)
         extras["c4_hbm_frac"] = 32 * 512 * N / (msc * 1e-3) / 1e9 / HBM_PEAK_GBS
+        # external comparator (timing only): the same forward + gradient step with
+        # torch.fft (rocFFT) and autograd on tensors of the same shape
+        xt = torch.randn(512, N, device="cuda").requires_grad_()
+        kt = (torch.randn(512, N, device="cuda") / math.sqrt(N)).requires_grad_()
+        wt = torch.randn(512, N, device="cuda")
+
+        def torch_fb():
+            yt = torch.fft.irfft(torch.fft.rfft(xt) * torch.fft.rfft(kt), n=N) * N
+            (yt * wt).sum().backward()
+        extras["comparator_torch_fft_rocfft_c4_fwd_bwd_ms"] = time_it(torch_fb, steps, warmup)
+        del xt, kt, wt
         # Karatsuba, 1024 pairs, lengths U[256, 4096]
         kar, lens, kdata = karatsuba_workload(torch, np)
         msk = time_it(kar, max(steps // 4, 3), 1)
