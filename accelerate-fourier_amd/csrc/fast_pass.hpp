@@ -30,10 +30,10 @@ __device__ __host__ constexpr int fpad(int e) { return e + (e >> 4); }
 __device__ __host__ constexpr int fast_ld(int L, int B) { return B > 1 ? fpad(L) + 1 : fpad(L); }
 
 // Stage twiddles come from an LDS copy of the W_L table when it is small
-// (L <= 1024: at most 16 KiB).  Longer lines (2048, 4096: three stages) take
-// the second stage's W_L^(k j L/256) = W_256^(k j) from an LDS copy of W_256
-// (4 KiB) and the last stage's from one FP64 load per butterfly and FP64
-// powers (or from registers, FastRegTw).  Four-step twiddles come from FP64 recurrences: one coarse x
+// (L <= 1024: at most 16 KiB).  Longer lines (2048 ... 8192: three or four
+// stages) take the second stage's W_L^(k j L/256) = W_256^(k j) from an LDS
+// copy of W_256 (4 KiB) and the later stages' from one FP64 load per butterfly
+// and FP64 powers (or, in the fused convolution, from registers: FastRegTw).  Four-step twiddles come from FP64 recurrences: one coarse x
 // fine table product per butterfly and a running FP64 product over its
 // outputs, each value rounded once to the working precision.  (LDS copies of
 // the four-step tables, two 16-byte LDS reads and a compensated product per
