@@ -118,6 +118,7 @@ k_fast_ext_grouped(const AfPass *__restrict__ descs, const int64_t *__restrict__
 #define AF_EXT_SHAPES(X)                                                                               \
     X(float, 256, 16, false) X(float, 512, 16, false) X(float, 1024, 16, false)                        \
     X(float, 2048, 8, false) X(float, 4096, 4, false) X(float, 8192, 1, false)                         \
+    X(float, 16384, 1, false)                                                                          \
     X(float, 256, 16, true) X(float, 512, 8, true) X(float, 1024, 4, true) X(float, 2048, 2, true)     \
     X(float, 4096, 1, true)                                                                            \
     X(double, 256, 8, false) X(double, 512, 8, false) X(double, 1024, 8, false)                        \

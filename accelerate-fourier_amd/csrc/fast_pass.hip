@@ -1,4 +1,4 @@
-// Specialised pass kernel for power-of-two line lengths (256 ... 8192).
+// Specialised pass kernel for power-of-two line lengths (256 ... 16384).
 //
 // The generic pass (fft_device.hpp) loads a tile into LDS, runs a runtime list
 // of stages and stores it from LDS.  For power-of-two lines the stage list is
@@ -284,7 +284,7 @@ bool fast_pass_eligible(int prec, const AfPass &d) {
     if (d.flags & ~kFastFlags) return false;
     if (d.in_map || d.out_map || d.in_mul || d.out_mul) return false;
     if (d.in_len != INT64_MAX || d.out_len != INT64_MAX) return false;
-    if (d.L < 256 || d.L > 8192 || (d.L & (d.L - 1))) return false;
+    if (d.L < 256 || d.L > 16384 || (d.L & (d.L - 1))) return false;
     // the W_L table of the working precision feeds the LDS stage table at every length
     if (!d.tw || !d.tw_d || ((d.flags & AFP_POSTTW) && (!d.ft_coarse_d || !d.ft_fine_d))) return false;
     if ((d.flags & AFP_POSTTW) && (d.ftN & (d.ftN - 1))) return false;
@@ -301,7 +301,7 @@ bool fast_pass_eligible(int prec, const AfPass &d) {
     return (int64_t)d.B * d.L / 16 <= 1024;
 }
 
-// Tile shapes with a power-of-two kernel: lengths 256 ... 8192 and line
+// Tile shapes with a power-of-two kernel: lengths 256 ... 16384 and line
 // counts B from one wavefront (B * L / 16 >= 64 threads) up to what one tile
 // holds in LDS.  X(type, L, B).
 #define AF_FAST_SHAPES(X)                                                                          \
@@ -310,6 +310,7 @@ bool fast_pass_eligible(int prec, const AfPass &d) {
     X(float, 1024, 1) X(float, 1024, 2) X(float, 1024, 4) X(float, 1024, 8) X(float, 1024, 16)     \
     X(float, 2048, 1) X(float, 2048, 2) X(float, 2048, 4) X(float, 2048, 8)                        \
     X(float, 4096, 1) X(float, 4096, 2) X(float, 4096, 4) X(float, 8192, 1) X(float, 8192, 2)      \
+    X(float, 16384, 1)                                                                             \
     X(double, 256, 4) X(double, 256, 8) X(double, 256, 16)                                         \
     X(double, 512, 2) X(double, 512, 4) X(double, 512, 8) X(double, 512, 16)                       \
     X(double, 1024, 1) X(double, 1024, 2) X(double, 1024, 4) X(double, 1024, 8)                    \
@@ -324,7 +325,7 @@ static bool fast_has_shape(int prec, int L, int B) {
 }
 
 int fast_tile_lines(int prec, int64_t L, int B) {
-    if (L < 256 || L > 8192 || (L & (L - 1))) return B;
+    if (L < 256 || L > 16384 || (L & (L - 1))) return B;
     for (int b = B; b >= 1; b /= 2)
         if (fast_has_shape(prec, (int)L, b)) return b;
     return B;

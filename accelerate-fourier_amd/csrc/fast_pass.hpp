@@ -10,7 +10,8 @@
 namespace af {
 
 template <int L> struct P2Plan {
-    static constexpr int LOG2 = L == 256 ? 8 : L == 512 ? 9 : L == 1024 ? 10 : L == 2048 ? 11 : L == 4096 ? 12 : 13;
+    static constexpr int LOG2 = L == 256 ? 8 : L == 512 ? 9 : L == 1024 ? 10 : L == 2048 ? 11 : L == 4096 ? 12
+                                : L == 8192 ? 13 : 14;
     static constexpr int NST = (LOG2 + 3) / 4;               // stages
     static constexpr int LASTR = 1 << (LOG2 - 4 * (NST - 1)); // radix of the last stage (16 if LOG2 % 4 == 0)
     static constexpr int radix(int s) { return s == NST - 1 ? LASTR : 16; }
@@ -30,7 +31,7 @@ __device__ __host__ constexpr int fpad(int e) { return e + (e >> 4); }
 __device__ __host__ constexpr int fast_ld(int L, int B) { return B > 1 ? fpad(L) + 1 : fpad(L); }
 
 // Stage twiddles come from an LDS copy of the W_L table when it is small
-// (L <= 1024: at most 16 KiB).  Longer lines (2048 ... 8192: three or four
+// (L <= 1024: at most 16 KiB).  Longer lines (2048 ... 16384: three or four
 // stages) take the second stage's W_L^(k j L/256) = W_256^(k j) from an LDS
 // copy of W_256 (4 KiB) and the later stages' from one FP64 load per butterfly
 // and FP64 powers (or, in the fused convolution, from registers: FastRegTw).  Four-step twiddles come from FP64 recurrences: one coarse x
@@ -313,7 +314,7 @@ __device__ __forceinline__ void fast_stages(const AfPass &d, const FastRow &r, t
     constexpr T s = T(SG);
     uint32_t oes, ols, ospan;
     fast_strides<T>(d, false, oes, ols, ospan);
-#pragma unroll
+#pragma clang loop unroll(full)  // (plain `unroll` left four-stage lines rolled, with scratch)
     for (int st = 0; st < PL::NST; ++st) {
         const int R = PL::radix(st);
         const int p = PL::p(st);
@@ -443,7 +444,7 @@ __device__ __forceinline__ void fast_fft_lds(const AfPass &d, typename Cx<T>::C 
     constexpr int LD = fast_ld(L, B);
     constexpr T s = T(SG);
     const int t = threadIdx.x;
-#pragma unroll
+#pragma clang loop unroll(full)  // (plain `unroll` left four-stage lines rolled, with scratch)
     for (int st = 0; st < PL::NST; ++st) {
         const int R = PL::radix(st);
         const int p = PL::p(st);
