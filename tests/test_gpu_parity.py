@@ -136,7 +136,7 @@ def test_config1_plain_f64():
     assert rel_l2(host(back) / 1024, x) <= 1e-12
 
 
-@pytest.mark.parametrize("n", [2, 4, 6, 8, 10, 16, 30, 64, 100, 1024, 8192])
+@pytest.mark.parametrize("n", [2, 4, 6, 8, 10, 16, 30, 64, 100, 1024, 8192, 16384, 32768])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_real_spectrum(n, prec):
     import oracle
@@ -150,14 +150,15 @@ def test_real_spectrum(n, prec):
     assert rel_l2(host(y), oracle.from_spectrum(ref)) <= tol
 
 
-@pytest.mark.parametrize("n", [100, 1024, 2048, 4096, 8192])
+@pytest.mark.parametrize("n", [100, 1024, 2048, 4096, 8192, 16384, 32768])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_real_convolution(n, prec):
     """The fused real convolution and its adjoint against the oracle's
     fromSpectrum(toSpectrum x * toSpectrum k) and the correlations
     fromSpectrum(S_w conj S_k), fromSpectrum(S_w conj S_x): the generic kernel
-    (n = 100) and the power-of-two one at half lengths 512 ... 4096 (the
-    register-twiddle form from 2048 on)."""
+    (n = 100, and half lengths 8192 and 16384 above the power-of-two one) and
+    the power-of-two one at half lengths 512 ... 4096 (the register-twiddle
+    form from 2048 on)."""
     import oracle
     rt = torch.float64 if prec == "f64" else torch.float32
     tol = 1e-12 if prec == "f64" else 1e-6
