@@ -66,6 +66,54 @@ def complex_view(x, what="input"):
     return x.data_ptr(), None, tuple(x.shape)
 
 
+def _check_written(t, what):
+    """A tensor a kernel writes into: on the GPU, contiguous, no lazy view bit."""
+    _check_tensor(t, what)
+    if t.is_conj() or t.is_neg():
+        raise ValueError(f"{what} is a lazily conjugated or negated view; pass a plain tensor")
+
+
+def check_complex_out(out, shape, prec, what="out"):
+    """A caller-supplied complex result: a planar (re, im) pair of real tensors
+    or one complex tensor, of the given shape and precision, on the GPU and
+    contiguous.  Raises instead of letting a kernel write past its end."""
+    shape = tuple(shape)
+    if isinstance(out, tuple):
+        if len(out) != 2:
+            raise ValueError(f"{what}: a planar result is a (re, im) pair")
+        for t, nm in zip(out, (".re", ".im")):
+            _check_written(t, what + nm)
+            if t.is_complex() or t.dtype != real_dtype(prec):
+                raise TypeError(f"{what + nm} must be {real_dtype(prec)}")
+            if tuple(t.shape) != shape:
+                raise ValueError(f"{what + nm} has shape {tuple(t.shape)}, expected {shape}")
+        return out
+    _check_written(out, what)
+    if out.dtype != complex_dtype(prec):
+        raise TypeError(f"{what} must be {complex_dtype(prec)}")
+    if tuple(out.shape) != shape:
+        raise ValueError(f"{what} has shape {tuple(out.shape)}, expected {shape}")
+    return out
+
+
+def check_real_out(out, shape, dtype, what="out"):
+    """A caller-supplied real result of the given shape and dtype."""
+    _check_written(out, what)
+    if out.dtype != dtype:
+        raise TypeError(f"{what} must be {dtype}")
+    if tuple(out.shape) != tuple(shape):
+        raise ValueError(f"{what} has shape {tuple(out.shape)}, expected {tuple(shape)}")
+    return out
+
+
+def check_same(t, ref, what, ref_what):
+    """Operand t must match ref in shape and dtype (kernels index both by ref's extents)."""
+    _check_tensor(t, what)
+    if tuple(t.shape) != tuple(ref.shape) or t.dtype != ref.dtype:
+        raise ValueError(f"{what} must have the shape and dtype of {ref_what} "
+                         f"({tuple(ref.shape)}, {ref.dtype}); got ({tuple(t.shape)}, {t.dtype})")
+
+
 def empty_like_complex(x, shape=None):
     if isinstance(x, tuple):
         s = shape or tuple(x[0].shape)

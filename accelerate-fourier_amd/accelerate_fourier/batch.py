@@ -38,11 +38,14 @@ class UnevenBatch:
             self.handle = None
 
     def describe(self) -> str:
-        buf = ctypes.create_string_buffer(4096)
+        """Plan counts per algorithm on the first line, then one line per
+        grouped launch: its kernel key, row count and first row."""
+        buf = ctypes.create_string_buffer(1 << 16)
         _abi.check(_abi.lib().af_batch_describe(self.handle, buf, len(buf)), "af_batch_describe")
         return buf.value.decode()
 
     def __call__(self, x, out=None):
+        x = _arrays.materialize(x)
         re, im, shape = _arrays.complex_view(x)
         if len(shape) != 1 or shape[0] != self.total:
             raise ValueError(f"expected a flat array of {self.total} elements")
@@ -50,6 +53,7 @@ class UnevenBatch:
             raise TypeError("precision differs from the batch's")
         if out is None:
             out = _arrays.empty_like_complex(x)
+        _arrays.check_complex_out(out, shape, self.prec)
         ore, oim = _arrays.ptrs(out)
         _abi.check(_abi.lib().af_batch_execute(self.handle, re, im, ore, oim, _arrays.stream_handle()),
                    "af_batch_execute")

@@ -88,6 +88,7 @@ class Transform:
         prec = _arrays.precision_of_complex(x)
         if out is None:
             out = _arrays.empty_like_complex(x)
+        _arrays.check_complex_out(out, shape, prec)
         ore, oim = _arrays.ptrs(out)
         batch = _arrays.batch_of(shape)
         p = self.plan(prec)
@@ -117,6 +118,7 @@ class Transform:
         prec = _arrays.precision_of_complex(g)
         if out is None:
             out = _arrays.empty_like_complex(g)
+        _arrays.check_complex_out(out, shape, prec)
         ore, oim = _arrays.ptrs(out)
         p = self.plan(prec)
         _abi.check(_abi.lib().af_vjp_execute(p.handle, _arrays.batch_of(shape), re, im, ore, oim, None,
@@ -139,6 +141,7 @@ class Transform2d:
         prec = _arrays.precision_of_complex(x)
         if out is None:
             out = _arrays.empty_like_complex(x)
+        _arrays.check_complex_out(out, shape, prec)
         ore, oim = _arrays.ptrs(out)
         batch = _arrays.batch_of(shape[:-1])
         _abi.check(_abi.lib().af_execute_2d(self.t0.plan(prec).handle, self.t1.plan(prec).handle, batch, re, im, ore,
@@ -161,6 +164,7 @@ class Transform3d:
         prec = _arrays.precision_of_complex(x)
         if out is None:
             out = _arrays.empty_like_complex(x)
+        _arrays.check_complex_out(out, shape, prec)
         ore, oim = _arrays.ptrs(out)
         batch = _arrays.batch_of(shape[:-2])
         _abi.check(_abi.lib().af_execute_3d(self.t0.plan(prec).handle, self.t1.plan(prec).handle,
@@ -380,6 +384,7 @@ def convolve_cyclic(n: int):
             raise TypeError("operands must have the same precision")
         if out is None:
             out = _arrays.empty_like_complex(x)
+        _arrays.check_complex_out(out, xs, prec)
         ore, oim = _arrays.ptrs(out)
         p = get_plan(n, -1, _abi.CONV_CYCLIC, prec)
         _abi.check(_abi.lib().af_convolve_cyclic(p.handle, _arrays.batch_of(xs), xr, xi, yr, yi, ore, oim, None,

@@ -49,6 +49,7 @@ def to_spectrum(sub: Transform, x, out=None, planar=False):
             out = (torch.empty_like(x), torch.empty_like(x))
         else:
             out = torch.empty(x.shape, dtype=_arrays.complex_dtype(prec), device=x.device)
+    _arrays.check_complex_out(out, x.shape, prec)
     ore, oim = _arrays.ptrs(out)
     p = get_plan(n, sub.sign.value, _abi.R2C_FULL, prec, flags=sub.flags)
     _abi.check(_abi.lib().af_to_spectrum(p.handle, _arrays.batch_of(tuple(x.shape)), x.data_ptr(), ore, oim, None,
@@ -68,6 +69,7 @@ def from_spectrum(sub: Transform, spec, out=None):
     if out is None:
         dev = spec[0].device if isinstance(spec, tuple) else spec.device
         out = torch.empty(shape, dtype=_arrays.real_dtype(prec), device=dev)
+    _arrays.check_real_out(out, shape, _arrays.real_dtype(prec))
     p = get_plan(n, sub.sign.value, _abi.C2R_FULL, prec, flags=sub.flags)
     _abi.check(_abi.lib().af_from_spectrum(p.handle, _arrays.batch_of(shape), re, im, out.data_ptr(), None,
                                            _arrays.stream_handle()), "af_from_spectrum")
@@ -79,10 +81,13 @@ def vjp_to_spectrum(g, out=None):
     g = _arrays.materialize(g)
     re, im, shape = _arrays.complex_view(g, "cotangent")
     n = shape[-1]
+    if n % 2:
+        raise ValueError("toSpectrum needs an even length (Real.hs:45-47)")
     prec = _arrays.precision_of_complex(g)
     if out is None:
         dev = g[0].device if isinstance(g, tuple) else g.device
         out = torch.empty(shape, dtype=_arrays.real_dtype(prec), device=dev)
+    _arrays.check_real_out(out, shape, _arrays.real_dtype(prec))
     p = get_plan(n, -1, _abi.R2C_FULL, prec)
     _abi.check(_abi.lib().af_vjp_to_spectrum(p.handle, _arrays.batch_of(shape), re, im, out.data_ptr(), None,
                                              _arrays.stream_handle()), "af_vjp_to_spectrum")
@@ -94,9 +99,12 @@ def vjp_from_spectrum(g, out=None):
     g = _arrays.materialize(g)
     _check_real(g, "cotangent")
     n = g.shape[-1]
+    if n % 2:
+        raise ValueError("fromSpectrum needs an even length (Real.hs:88-90)")
     prec = _real_prec(g)
     if out is None:
         out = torch.empty(g.shape, dtype=_arrays.complex_dtype(prec), device=g.device)
+    _arrays.check_complex_out(out, g.shape, prec)
     ore, oim = _arrays.ptrs(out)
     p = get_plan(n, 1, _abi.C2R_FULL, prec)
     _abi.check(_abi.lib().af_vjp_from_spectrum(p.handle, _arrays.batch_of(tuple(g.shape)), g.data_ptr(), ore, oim,
@@ -107,6 +115,7 @@ def vjp_from_spectrum(g, out=None):
 def convolve(x, k, out=None):
     """fromSpectrum (transform inverse (n/2)) (toSpectrum (transform forward (n/2)) x *
     toSpectrum (transform forward (n/2)) k): n times the cyclic convolution, one fused launch."""
+    x, k = _arrays.materialize(x), _arrays.materialize(k)
     _check_real(x, "x")
     _check_real(k, "k")
     if x.shape != k.shape or x.dtype != k.dtype:
@@ -115,6 +124,7 @@ def convolve(x, k, out=None):
     prec = _real_prec(x)
     if out is None:
         out = torch.empty_like(x)
+    _arrays.check_real_out(out, x.shape, x.dtype)
     p = get_plan(n, -1, _abi.REAL_CONV, prec)
     _abi.check(_abi.lib().af_real_convolve(p.handle, _arrays.batch_of(tuple(x.shape)), x.data_ptr(), k.data_ptr(),
                                            out.data_ptr(), _arrays.stream_handle()), "af_real_convolve")
@@ -123,17 +133,37 @@ def convolve(x, k, out=None):
 
 def convolve_vjp(x, k, w, dx=None, dk=None):
     """Gradients of sum(w * convolve(x, k)) with respect to x and k (one fused launch)."""
+    x, k, w = _arrays.materialize(x), _arrays.materialize(k), _arrays.materialize(w)
     for t, nm in ((x, "x"), (k, "k"), (w, "w")):
         _check_real(t, nm)
+    # the fused adjoint indexes k, w, dx and dk by x's rows and length
+    _arrays.check_same(k, x, "k", "x")
+    _arrays.check_same(w, x, "w", "x")
     n = x.shape[-1]
     prec = _real_prec(x)
     dx = torch.empty_like(x) if dx is None else dx
     dk = torch.empty_like(k) if dk is None else dk
+    _arrays.check_real_out(dx, x.shape, x.dtype, "dx")
+    _arrays.check_real_out(dk, x.shape, x.dtype, "dk")
     p = get_plan(n, -1, _abi.REAL_CONV, prec)
     _abi.check(_abi.lib().af_vjp_real_convolve(p.handle, _arrays.batch_of(tuple(x.shape)), x.data_ptr(),
                                                k.data_ptr(), w.data_ptr(), dx.data_ptr(), dk.data_ptr(),
                                                _arrays.stream_handle()), "af_vjp_real_convolve")
     return dx, dk
+
+
+def _check_pair_out(out, shape, prec):
+    if not (isinstance(out, tuple) and len(out) == 2):
+        raise TypeError("out is a pair of complex arrays (a, b)")
+    _arrays.check_complex_out(out[0], shape, prec, "out[0]")
+    _arrays.check_complex_out(out[1], shape, prec, "out[1]")
+
+
+def _check_real_pair_out(out, shape, prec):
+    if not (isinstance(out, tuple) and len(out) == 2):
+        raise TypeError("out is a pair of real arrays (x, y)")
+    _arrays.check_real_out(out[0], shape, _arrays.real_dtype(prec), "out[0]")
+    _arrays.check_real_out(out[1], shape, _arrays.real_dtype(prec), "out[1]")
 
 
 def _two_shape(a, b, what):
@@ -157,6 +187,7 @@ def two_to_spectrum(sub: Transform, x, y, out=None):
     if out is None:
         ct = _arrays.complex_dtype(prec)
         out = (torch.empty(x.shape, dtype=ct, device=x.device), torch.empty(x.shape, dtype=ct, device=x.device))
+    _check_pair_out(out, x.shape, prec)
     are, aim = _arrays.ptrs(out[0])
     bre, bim = _arrays.ptrs(out[1])
     p = sub.plan(prec)
@@ -183,6 +214,7 @@ def two_from_spectrum(sub: Transform, a, b, out=None):
         dev = a[0].device if isinstance(a, tuple) else a.device
         rt = _arrays.real_dtype(prec)
         out = (torch.empty(shape, dtype=rt, device=dev), torch.empty(shape, dtype=rt, device=dev))
+    _check_real_pair_out(out, shape, prec)
     p = sub.plan(prec)
     _abi.check(_abi.lib().af_two_from_spectrum(p.handle, _arrays.batch_of(shape), are, aim, bre, bim,
                                                out[0].data_ptr(), out[1].data_ptr(), _arrays.stream_handle()),
@@ -198,6 +230,7 @@ def untangle_spectra(z, out=None):
     prec = _arrays.precision_of_complex(z)
     if out is None:
         out = (_arrays.empty_like_complex(z), _arrays.empty_like_complex(z))
+    _check_pair_out(out, shape, prec)
     are, aim = _arrays.ptrs(out[0])
     bre, bim = _arrays.ptrs(out[1])
     _abi.check(_abi.lib().af_untangle_spectra(prec, _arrays.batch_of(shape), shape[-1], zre, zim, are, aim, bre, bim,
@@ -213,8 +246,11 @@ def entangle_spectra(c, d, out=None):
     if tuple(shape) != tuple(shape_d):
         raise ValueError("the two spectra must have equal shapes")
     prec = _arrays.precision_of_complex(c)
+    if _arrays.precision_of_complex(d) != prec:
+        raise TypeError("the two spectra must have the same precision")
     if out is None:
         out = _arrays.empty_like_complex(c)
+    _arrays.check_complex_out(out, shape, prec)
     zre, zim = _arrays.ptrs(out)
     _abi.check(_abi.lib().af_entangle_spectra(prec, _arrays.batch_of(shape), shape[-1], cre, cim, dre, dim, zre, zim,
                                               _arrays.stream_handle()), "af_entangle_spectra")
@@ -233,6 +269,7 @@ def two_to_spectrum_2d(t0: Transform, t1: Transform, x, y, out=None):
     if out is None:
         ct = _arrays.complex_dtype(prec)
         out = (torch.empty(x.shape, dtype=ct, device=x.device), torch.empty(x.shape, dtype=ct, device=x.device))
+    _check_pair_out(out, x.shape, prec)
     are, aim = _arrays.ptrs(out[0])
     bre, bim = _arrays.ptrs(out[1])
     _abi.check(_abi.lib().af_two_to_spectrum_2d(t0.plan(prec).handle, t1.plan(prec).handle,
@@ -255,6 +292,7 @@ def two_from_spectrum_2d(t0: Transform, t1: Transform, a, b, out=None):
         dev = a[0].device if isinstance(a, tuple) else a.device
         rt = _arrays.real_dtype(prec)
         out = (torch.empty(shape, dtype=rt, device=dev), torch.empty(shape, dtype=rt, device=dev))
+    _check_real_pair_out(out, shape, prec)
     _abi.check(_abi.lib().af_two_from_spectrum_2d(t0.plan(prec).handle, t1.plan(prec).handle,
                                                   _arrays.batch_of(shape[:-1]), are, aim, bre, bim,
                                                   out[0].data_ptr(), out[1].data_ptr(), _arrays.stream_handle()),
@@ -271,6 +309,7 @@ def untangle_spectra_2d(z, out=None):
     prec = _arrays.precision_of_complex(z)
     if out is None:
         out = (_arrays.empty_like_complex(z), _arrays.empty_like_complex(z))
+    _check_pair_out(out, shape, prec)
     are, aim = _arrays.ptrs(out[0])
     bre, bim = _arrays.ptrs(out[1])
     _abi.check(_abi.lib().af_untangle_spectra_2d(prec, _arrays.batch_of(shape[:-1]), shape[-2], shape[-1], zre, zim,

@@ -1549,17 +1549,16 @@ af_status af_convolve_cyclic(af_plan plan, int64_t batch, const void *x_re, cons
 
 namespace {
 
-int karatsuba_P(int64_t Lp) {
+// P = 2^d s >= Lp with s <= 64: the padded length the LDS kernel works on
+int64_t karatsuba_P(int64_t Lp) {
     int d = 0;
     int64_t s = Lp;
     while (s > 64) {
         ++d;
         s = (Lp + ((int64_t)1 << d) - 1) >> d;
     }
-    return (int)(s << d);
+    return s << d;
 }
-
-const int kKaratsubaMaxP = 4608;
 
 // Pinned host staging for the small per-call tables copied to the device (the
 // Karatsuba pair table): a pool of hipHostMalloc buffers, each reused once the
@@ -1605,25 +1604,117 @@ af_status stage_to_device(void *dst, const void *src, size_t bytes, hipStream_t 
     return st;
 }
 
-af_status karatsuba_run(const std::vector<int64_t> &meta_in, int64_t npairs, int maxP, const int64_t *x,
-                        const int64_t *y, int64_t *z, int cyclic, hipStream_t s) {
+// Pairs (meta: input offset, length, output offset per pair) whose padded
+// length fits the LDS kernel, in one launch, longest first for load balance.
+af_status karatsuba_leaves(std::vector<int64_t> meta, const int64_t *x, const int64_t *y, int64_t *z, int cyclic,
+                           hipStream_t s) {
+    const int64_t npairs = (int64_t)meta.size() / 3;
+    if (npairs == 0) return AF_SUCCESS;
+    int64_t maxP = 0;
+    std::vector<int64_t> order(npairs);
+    for (int64_t i = 0; i < npairs; ++i) {
+        order[i] = i;
+        maxP = std::max(maxP, karatsuba_P(meta[3 * i + 1]));
+    }
+    if (maxP > kKaratsubaMaxP) return AF_INTERNAL;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return meta[3 * a + 1] > meta[3 * b + 1]; });
+    std::vector<int64_t> sorted(3 * npairs);
+    for (int64_t i = 0; i < npairs; ++i)
+        for (int j = 0; j < 3; ++j) sorted[3 * i + j] = meta[3 * order[i] + j];
+    void *dm = nullptr;
+    HIPCHK(hipMallocAsync(&dm, sorted.size() * 8, s));
+    af_status st = stage_to_device(dm, sorted.data(), sorted.size() * 8, s);
+    if (st == AF_SUCCESS)
+        st = HIPST(launch_karatsuba(x, y, z, (const int64_t *)dm, npairs, (int)maxP, cyclic, s));
+    (void)hipFreeAsync(dm, s);
+    return st;
+}
+
+// Any lengths.  Pairs longer than the LDS kernel's limit get their top
+// Karatsuba levels in global memory: split level by level (k_kar_split) until
+// every piece fits, the pieces go through the LDS kernel in one launch, and the
+// joins (k_kar_join) run back up, the root joins writing the outputs (wrapped
+// when cyclic).
+af_status karatsuba_run(const std::vector<int64_t> &meta_in, int64_t npairs, const int64_t *x, const int64_t *y,
+                        int64_t *z, int cyclic, hipStream_t s) {
     AFCHK(device_init());
     std::call_once(g_attr_flag, set_kernel_attributes);
     if (npairs == 0) return AF_SUCCESS;
-    if (maxP > kKaratsubaMaxP) return AF_UNSUPPORTED;
-    // longest pairs first, for load balance
-    std::vector<int64_t> order(npairs);
-    for (int64_t i = 0; i < npairs; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int64_t a, int64_t b) { return meta_in[3 * a + 1] > meta_in[3 * b + 1]; });
-    std::vector<int64_t> meta(3 * npairs);
-    for (int64_t i = 0; i < npairs; ++i)
-        for (int j = 0; j < 3; ++j) meta[3 * i + j] = meta_in[3 * order[i] + j];
-    void *dm = nullptr;
-    HIPCHK(hipMallocAsync(&dm, meta.size() * 8, s));
-    af_status st = stage_to_device(dm, meta.data(), meta.size() * 8, s);
-    if (st == AF_SUCCESS) st = HIPST(launch_karatsuba(x, y, z, (const int64_t *)dm, npairs, maxP, cyclic, s));
-    (void)hipFreeAsync(dm, s);
+    // a node of the top levels: a root (a caller's pair, offsets into x, y, z)
+    // or a piece (offsets into the workspace arrays WX, WY, WZ)
+    struct Node {
+        int64_t L, xo, zo;
+        bool root;
+    };
+    struct Split {
+        Node n;
+        int64_t cxo, zco;  // children's operands in WX/WY, children's products in WZ
+    };
+    std::vector<int64_t> small, pieces;  // leaf meta: roots, workspace pieces
+    std::vector<Node> cur;
+    for (int64_t p = 0; p < npairs; ++p) {
+        const int64_t off = meta_in[3 * p], L = meta_in[3 * p + 1], zo = meta_in[3 * p + 2];
+        if (L <= 0) continue;
+        if (karatsuba_P(L) <= kKaratsubaMaxP) small.insert(small.end(), {off, L, zo});
+        else cur.push_back({L, off, zo, true});
+    }
+    std::vector<std::vector<Split>> levels;
+    int64_t wx = 0, wz = 0;
+    while (!cur.empty()) {
+        std::vector<Split> lv;
+        std::vector<Node> next;
+        for (const Node &n : cur) {
+            const int64_t h = (n.L + 1) / 2;
+            Split sp{n, wx, wz};
+            wx += 3 * h;
+            wz += 3 * (2 * h - 1);
+            for (int c = 0; c < 3; ++c) {
+                const Node ch{h, sp.cxo + c * h, sp.zco + c * (2 * h - 1), false};
+                if (karatsuba_P(h) <= kKaratsubaMaxP) pieces.insert(pieces.end(), {ch.xo, ch.L, ch.zo});
+                else next.push_back(ch);
+            }
+            lv.push_back(sp);
+        }
+        levels.push_back(std::move(lv));
+        cur.swap(next);
+    }
+    af_status st = karatsuba_leaves(small, x, y, z, cyclic, s);
+    if (st != AF_SUCCESS || levels.empty()) return st;
+    int64_t *ws = nullptr;
+    HIPCHK(hipMallocAsync((void **)&ws, (size_t)(2 * wx + wz) * sizeof(int64_t), s));
+    int64_t *WX = ws, *WY = ws + wx, *WZ = ws + 2 * wx;
+    std::vector<KarSplitTask> splits;
+    std::vector<KarJoinTask> joins;
+    std::vector<size_t> split_at, join_at;  // first task of each level
+    for (const auto &lv : levels) {
+        split_at.push_back(splits.size());
+        join_at.push_back(joins.size());
+        for (const Split &sp : lv) {
+            const Node &n = sp.n;
+            splits.push_back({n.root ? x + n.xo : WX + n.xo, n.root ? y + n.xo : WY + n.xo, n.L, WX + sp.cxo,
+                              WY + sp.cxo});
+            joins.push_back({WZ + sp.zco, n.L, n.root ? z + n.zo : WZ + n.zo, n.root ? (int64_t)cyclic : 0});
+        }
+    }
+    split_at.push_back(splits.size());
+    join_at.push_back(joins.size());
+    const size_t sb = splits.size() * sizeof(KarSplitTask), jb = joins.size() * sizeof(KarJoinTask);
+    std::vector<unsigned char> tab(sb + jb);
+    std::memcpy(tab.data(), splits.data(), sb);
+    std::memcpy(tab.data() + sb, joins.data(), jb);
+    unsigned char *dt = nullptr;
+    st = HIPST(hipMallocAsync((void **)&dt, tab.size(), s));
+    if (st == AF_SUCCESS) st = stage_to_device(dt, tab.data(), tab.size(), s);
+    const KarSplitTask *dsplit = (const KarSplitTask *)dt;
+    const KarJoinTask *djoin = (const KarJoinTask *)(dt + sb);
+    for (size_t l = 0; st == AF_SUCCESS && l < levels.size(); ++l)
+        st = HIPST(launch_kar_split(dsplit + split_at[l], (int64_t)(split_at[l + 1] - split_at[l]), s));
+    if (st == AF_SUCCESS) st = karatsuba_leaves(pieces, WX, WY, WZ, 0, s);
+    for (size_t l = levels.size(); st == AF_SUCCESS && l-- > 0;)
+        st = HIPST(launch_kar_join(djoin + join_at[l], (int64_t)(join_at[l + 1] - join_at[l]), s));
+    if (dt) (void)hipFreeAsync(dt, s);
+    (void)hipFreeAsync(ws, s);
     return st;
 }
 
@@ -1641,7 +1732,7 @@ af_status af_karatsuba_i64(int64_t batch, int64_t len, const int64_t *x, const i
         meta[3 * r + 1] = len;
         meta[3 * r + 2] = r * (2 * len - 1);
     }
-    return karatsuba_run(meta, batch, karatsuba_P(len), x, y, z, 0, (hipStream_t)stream);
+    return karatsuba_run(meta, batch, x, y, z, 0, (hipStream_t)stream);
 }
 
 af_status af_karatsuba_cyclic_i64(int64_t batch, int64_t len, const int64_t *x, const int64_t *y, int64_t *z,
@@ -1655,7 +1746,7 @@ af_status af_karatsuba_cyclic_i64(int64_t batch, int64_t len, const int64_t *x, 
         meta[3 * r + 1] = len;
         meta[3 * r + 2] = r * len;
     }
-    return karatsuba_run(meta, batch, karatsuba_P(len), x, y, z, 1, (hipStream_t)stream);
+    return karatsuba_run(meta, batch, x, y, z, 1, (hipStream_t)stream);
 }
 
 af_status af_karatsuba_complex_i64(int64_t batch, int64_t len, int32_t cyclic, const int64_t *xr, const int64_t *xi,
@@ -1677,7 +1768,7 @@ af_status af_karatsuba_complex_i64(int64_t batch, int64_t len, int32_t cyclic, c
         meta[3 * q + 2] = q * nz;
     }
     af_status st = HIPST(launch_gauss_split(xr, xi, yr, yi, x3, y3, count, s));
-    if (st == AF_SUCCESS) st = karatsuba_run(meta, 3 * batch, karatsuba_P(len), x3, y3, p3, cyclic, s);
+    if (st == AF_SUCCESS) st = karatsuba_run(meta, 3 * batch, x3, y3, p3, cyclic, s);
     if (st == AF_SUCCESS) st = HIPST(launch_gauss_join(p3, zr, zi, zcount, s));
     (void)hipFreeAsync(ws, s);
     return st;
@@ -1690,16 +1781,14 @@ af_status af_karatsuba_ragged_i64(int64_t npairs, const int64_t *offsets, const 
     if (!x || !y || !z) return AF_INVALID_ARG;
     if (offsets[0] != 0) return AF_INVALID_ARG;
     std::vector<int64_t> meta(3 * npairs);
-    int maxP = 0;
     for (int64_t p = 0; p < npairs; ++p) {
         const int64_t L = offsets[p + 1] - offsets[p];
         if (L < 1) return AF_INVALID_ARG;  // equal, non-empty operand lengths per pair
         meta[3 * p] = offsets[p];
         meta[3 * p + 1] = L;
         meta[3 * p + 2] = 2 * offsets[p] - p;
-        maxP = std::max(maxP, karatsuba_P(L));
     }
-    return karatsuba_run(meta, npairs, maxP, x, y, z, 0, (hipStream_t)stream);
+    return karatsuba_run(meta, npairs, x, y, z, 0, (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -1772,7 +1861,15 @@ struct af_batch_s {
     const void *last[4] = {nullptr, nullptr, nullptr, nullptr};
     bool uploaded = false;
     std::string text;
+    // Calls share the descriptors and the scratch intermediates: the mutex
+    // serialises their host side, and `done` (recorded after each graph launch)
+    // orders every launch after the previous one, whatever its stream, and keeps
+    // the descriptors from being rewritten while a launch may still read them.
+    std::mutex mu;
+    hipEvent_t done = nullptr;
+    bool recorded = false;
     ~af_batch_s() {
+        if (done) (void)hipEventDestroy(done);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (auto &kv : plans) af_plan_destroy(kv.second);
@@ -1945,6 +2042,18 @@ af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign, i
     }
     B->text = "batch" + std::to_string(count) + " launches=" + std::to_string(B->groups.size());
     for (auto &kv : counts) B->text += " " + kv.first + "=" + std::to_string(kv.second);
+    // one line per grouped launch: its kernel key, row count and first row
+    for (size_t k = 0; k < B->groups.size(); ++k) {
+        const auto &g = B->groups[k];
+        char line[160];
+        if (g.ext)
+            std::snprintf(line, sizeof line, "\nlaunch %zu phase %d ext L=%d B=%d conv=%d sign=%d rows=%zu first=%lld", k,
+                          g.phase, g.L, g.B, g.conv ? 1 : 0, g.sign, g.rows.size(), (long long)g.rows[0]);
+        else
+            std::snprintf(line, sizeof line, "\nlaunch %zu phase %d generic threads=%d full=%d rows=%zu first=%lld", k,
+                          g.phase, g.nthreads, g.full ? 1 : 0, g.rows.size(), (long long)g.rows[0]);
+        B->text += line;
+    }
     *batch = B.release();
     return AF_SUCCESS;
 }
@@ -1968,14 +2077,24 @@ af_status af_batch_execute(af_batch B, const void *in_re, const void *in_im, voi
     if ((in_im == nullptr) != (out_im == nullptr)) return AF_INVALID_ARG;
     hipStream_t s = (hipStream_t)stream;
     const void *cur[4] = {in_re, in_im, out_re, out_im};
+    std::lock_guard<std::mutex> lk(B->mu);
+    if (!B->done) HIPCHK(hipEventCreateWithFlags(&B->done, hipEventDisableTiming));
     if (!B->uploaded || std::memcmp(cur, B->last, sizeof cur) != 0) {
+        // the previous launch may still read the descriptors about to change
+        if (B->recorded) HIPCHK(hipEventSynchronize(B->done));
         std::vector<AfPass> ps;
         for (const auto &g : B->groups) {
             int64_t j = g.desc_off;
             for (int64_t i : g.rows) {
                 ps.clear();
                 AFCHK(batch_passes(B, i, in_re, in_im, out_re, out_im, ps));
-                B->h_descs[j++] = ps[g.phase];
+                const AfPass &d = ps[g.phase];
+                // the captured launch fixed this descriptor's tiles and shape
+                const int64_t k = j - g.desc_off;
+                if (d.ntiles != B->h_prefix[g.prefix_off + k + 1] - B->h_prefix[g.prefix_off + k] ||
+                    (g.ext && (d.L != g.L || d.B != g.B)) || (!g.ext && (d.nthreads > g.nthreads || d.lds_bytes > g.lds)))
+                    return AF_INTERNAL;
+                B->h_descs[j++] = d;
             }
         }
         if (!B->h_descs.empty())
@@ -1986,7 +2105,10 @@ af_status af_batch_execute(af_batch B, const void *in_re, const void *in_im, voi
         B->uploaded = true;
     }
     if (B->groups.empty()) return AF_SUCCESS;
+    if (B->recorded) HIPCHK(hipStreamWaitEvent(s, B->done, 0));
     HIPCHK(hipGraphLaunch(B->exec, s));
+    HIPCHK(hipEventRecord(B->done, s));
+    B->recorded = true;
     return AF_SUCCESS;
 }
 

@@ -319,6 +319,77 @@ static unsigned elementwise_grid(int64_t count) {
     return (unsigned)(g < 8192 ? g : 8192);
 }
 
+// Top levels of pairs longer than one workgroup's LDS holds, in global memory.
+// A split task takes one pair (x, y) of length L to the three pairs of length
+// h = ceil(L/2) of one Karatsuba level (karatsubaReorder, Convolution/Adhoc.hs:66-79):
+// low halves, high halves zero-padded to h, and their sums, stored as three
+// consecutive h-blocks at cx and cy.  A join task takes the three products of
+// such a split (consecutive (2h-1)-blocks at zc) to
+// z = zl + u^h (zm - zl - zr) + u^(2h) zr (karatsubaGo, Adhoc.hs:81-98),
+// 2L - 1 coefficients, or their cyclic wrap to L (Adhoc.hs:104-111).  All in
+// the ring of 64-bit words, so the bits equal any other exact evaluation.
+__global__ void __launch_bounds__(256)
+k_kar_split(const KarSplitTask *__restrict__ tasks, int64_t ntasks) {
+    for (int64_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const KarSplitTask k = tasks[t];
+        const int64_t h = (k.L + 1) / 2;
+        const u64 *px = (const u64 *)k.x, *py = (const u64 *)k.y;
+        u64 *cx = (u64 *)k.cx, *cy = (u64 *)k.cy;
+        for (int64_t i = threadIdx.x; i < h; i += blockDim.x) {
+            const u64 xl = px[i], yl = py[i];
+            const u64 xh = h + i < k.L ? px[h + i] : 0ull, yh = h + i < k.L ? py[h + i] : 0ull;
+            cx[i] = xl;
+            cx[h + i] = xh;
+            cx[2 * h + i] = xl + xh;
+            cy[i] = yl;
+            cy[h + i] = yh;
+            cy[2 * h + i] = yl + yh;
+        }
+    }
+}
+
+__device__ __forceinline__ u64 kar_join_at(const u64 *zc, int64_t h, int64_t k) {
+    const int64_t w = 2 * h - 1;
+    const u64 *zl = zc, *zr = zc + w, *zm = zc + 2 * w;
+    u64 v = 0;
+    if (k < w) v += zl[k];
+    const int64_t m = k - h;
+    if (m >= 0 && m < w) v += zm[m] - zl[m] - zr[m];
+    const int64_t r = k - 2 * h;
+    if (r >= 0 && r < w) v += zr[r];
+    return v;
+}
+
+__global__ void __launch_bounds__(256)
+k_kar_join(const KarJoinTask *__restrict__ tasks, int64_t ntasks) {
+    for (int64_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const KarJoinTask k = tasks[t];
+        const int64_t h = (k.L + 1) / 2, L = k.L;
+        const u64 *zc = (const u64 *)k.zc;
+        u64 *z = (u64 *)k.z;
+        if (k.cyclic) {
+            for (int64_t i = threadIdx.x; i < L; i += blockDim.x)
+                z[i] = kar_join_at(zc, h, i) + (i + L < 2 * L - 1 ? kar_join_at(zc, h, i + L) : 0ull);
+        } else {
+            for (int64_t i = threadIdx.x; i < 2 * L - 1; i += blockDim.x) z[i] = kar_join_at(zc, h, i);
+        }
+    }
+}
+
+hipError_t launch_kar_split(const KarSplitTask *dev_tasks, int64_t ntasks, hipStream_t s) {
+    if (ntasks <= 0) return hipSuccess;
+    const unsigned g = (unsigned)(ntasks < 65536 ? ntasks : 65536);
+    hipLaunchKernelGGL(k_kar_split, dim3(g), dim3(256), 0, s, dev_tasks, ntasks);
+    return hipGetLastError();
+}
+
+hipError_t launch_kar_join(const KarJoinTask *dev_tasks, int64_t ntasks, hipStream_t s) {
+    if (ntasks <= 0) return hipSuccess;
+    const unsigned g = (unsigned)(ntasks < 65536 ? ntasks : 65536);
+    hipLaunchKernelGGL(k_kar_join, dim3(g), dim3(256), 0, s, dev_tasks, ntasks);
+    return hipGetLastError();
+}
+
 hipError_t launch_gauss_split(const int64_t *xr, const int64_t *xi, const int64_t *yr, const int64_t *yi, int64_t *x3,
                               int64_t *y3, int64_t count, hipStream_t s) {
     if (count <= 0) return hipSuccess;

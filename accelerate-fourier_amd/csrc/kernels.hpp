@@ -81,5 +81,25 @@ hipError_t launch_karatsuba(const int64_t *x, const int64_t *y, int64_t *z, cons
 hipError_t launch_gauss_split(const int64_t *xr, const int64_t *xi, const int64_t *yr, const int64_t *yi, int64_t *x3,
                               int64_t *y3, int64_t count, hipStream_t s);
 hipError_t launch_gauss_join(const int64_t *p3, int64_t *zr, int64_t *zi, int64_t count, hipStream_t s);
+// Global-memory top levels for pairs whose padded length exceeds the LDS kernel's:
+// a split task reads x, y (length L) and writes the three child operand pairs
+// (low, high, low + high; h = ceil(L/2) words each, consecutive) to cx, cy; a
+// join task combines the three child products (2h - 1 words each, consecutive
+// at zc) into z (2L - 1 words, or L when cyclic).
+struct KarSplitTask {
+    const int64_t *x, *y;
+    int64_t L;
+    int64_t *cx, *cy;
+};
+struct KarJoinTask {
+    const int64_t *zc;
+    int64_t L;
+    int64_t *z;
+    int64_t cyclic;
+};
+hipError_t launch_kar_split(const KarSplitTask *dev_tasks, int64_t ntasks, hipStream_t s);
+hipError_t launch_kar_join(const KarJoinTask *dev_tasks, int64_t ntasks, hipStream_t s);
+// the largest padded pair length the LDS kernel takes
+constexpr int kKaratsubaMaxP = 4608;
 
 }  // namespace af

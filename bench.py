@@ -124,6 +124,10 @@ def tile_lines(report, i, default):
     return int(m.group(1)) if m else default
 
 
+def launch_name(report, i):
+    return f"k_fast<float, 256, {tile_lines(report, i, 16)}, -1> " + PASS_NAMES.get(i, f"pass {i}")
+
+
 def time_it(fn, steps, warmup):
     """Average ms per call, HIP events on the current stream around `steps` calls."""
     import torch
@@ -149,6 +153,29 @@ def profiled_traffic(pass_index):
         with open(path) as f:
             for line in f:
                 if line.startswith(f"pass {pass_index} traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def karatsuba_leaf_macs(L):
+    """Multiply-adds of the LDS Karatsuba kernel's leaf products for one pair of
+    length L: padded to P = 2^d s (s <= 64, d smallest), 3^d leaves of s x s."""
+    d, s = 0, L
+    while s > 64:
+        d += 1
+        s = (L + (1 << d) - 1) >> d
+    return (3 ** d) * s * s
+
+
+def profiled_c2b_traffic():
+    """HBM bytes of one 2^24 transform from the committed counter summary, or None."""
+    path = os.path.join(REPO, "profiles", "c2b_2p24_f32_hbm_counters.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("transform traffic_bytes:"):
                     return int(line.split(":")[1])
     except OSError:
         pass
@@ -265,7 +292,7 @@ def uneven_batch(torch, np, steps):
     out = {"c3_uneven_rows": int(len(lens)), "c3_uneven_total_length": tot, "c3_uneven_f32_ms": ms,
            "c3_uneven_GFLOPs": flops / (ms * 1e-3) / 1e9,
            "c3_uneven_hbm_frac": 16.0 * tot / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-           "c3_uneven_create_s": create_s, "c3_uneven_plan": b32.describe(), "c3_uneven_error_vs_fp64": {}}
+           "c3_uneven_create_s": create_s, "c3_uneven_plan": b32.describe().split("\n")[0], "c3_uneven_error_vs_fp64": {}}
     for c, name in enumerate(synthetic.UNEVEN_CLASSES):
         r = rel[cls == c]
         out["c3_uneven_error_vs_fp64"][name] = {"rows": int(len(r)), "max_rel_l2": float(r.max()),
@@ -337,12 +364,13 @@ def main():
                           "copy_bytes_each_way": 32 * 2 ** 20}))
         return
     ms = time_it(f32.run, steps, warmup)
-    # average launch duration of each pass: HIP events around `steps` launches of
-    # that pass alone, on the stream the kernels run on
+    # average duration of each launch of the transform: HIP events around `steps`
+    # launches of that launch alone, on the stream the kernels run on
     pass_ms = [time_it(lambda i=i: f32.run_pass(i), steps, warmup) for i in range(f32.npasses)]
-    dom = int(np.argmax(pass_ms))
-    # every pass of the four-step transform reads and writes the whole array once
-    achieved = alg_bytes / (pass_ms[dom] * 1e-3) / 1e9
+    # the roofline is the whole transform's: its algorithmic bytes (one read and
+    # one write of the array) over the time of all its launches
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    traffic = [profiled_traffic(i) for i in range(f32.npasses)]
     out = {
         "metric": METRIC,
         "value": flops / (ms * 1e-3) / 1e9,
@@ -358,14 +386,15 @@ def main():
         "data": "synthetic: N(0,1) re/im from splitmix64 seed 1, resident in HBM",
         "config": {"workload": "batched C2C FFT, Complex Float, 64 x 65536, forward, planar layout",
                    "batch": batch, "length": n, "plan": f"{f32.npasses} passes"},
-        "hbm_GBps_end_to_end": alg_bytes / (ms * 1e-3) / 1e9,
-        "hbm_frac_end_to_end": alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None if None in traffic else int(sum(traffic)),
                      "algorithmic_bytes": alg_bytes,
-                     "kernel": f"k_fast<float, 256, {tile_lines(tune_report, dom, 16)}, -1> "
-                               + PASS_NAMES.get(dom, f"pass {dom}"),
-                     "pass_ms": pass_ms},
+                     "kernel": f"the whole 64 x 65536 transform: {f32.npasses} launch(es) of "
+                               + ", ".join(launch_name(tune_report, i) for i in range(f32.npasses)),
+                     "per_launch": [{"launch": launch_name(tune_report, i), "ms": pass_ms[i],
+                                     "GBps_if_alone": alg_bytes / (pass_ms[i] * 1e-3) / 1e9,
+                                     "traffic": traffic[i]} for i in range(f32.npasses)]},
     }
     if tune_report:
         out["config"]["autotune"] = [l for l in tune_report.strip().split("\n")]
@@ -411,6 +440,10 @@ def main():
         extras["c2b_2p24_f32_ms"] = ms24
         extras["c2b_2p24_f32_GFLOPs"] = 5.0 * n24 * 24 / (ms24 * 1e-3) / 1e9
         extras["c2b_2p24_hbm_frac"] = b24 / (ms24 * 1e-3) / 1e9 / HBM_PEAK_GBS
+        extras["c2b_roofline"] = {"bound": "hbm", "achieved": b24 / (ms24 * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": b24 / (ms24 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                  "traffic": profiled_c2b_traffic(), "algorithmic_bytes": int(b24),
+                                  "kernel": f"the whole 2^24 transform: {g.npasses} launches"}
         y24 = g.result()
         extras["c2b_2p24_rel_l2_vs_numpy_fp64"] = rel_l2(y24[0], np.fft.fft(x24[0]))
         del g
@@ -465,6 +498,14 @@ def main():
         extras["c5_karatsuba_ms"] = msk
         extras["c5_pairs_per_s"] = 1024 / (msk * 1e-3)
         extras["c5_schoolbook_equiv_GMACs"] = float(np.sum(lens.astype(np.float64) ** 2)) / (msk * 1e-3) / 1e9
+        # compute roofline: the kernel's own leaf multiply-adds (3^d leaf products of
+        # s x s coefficients per pair, P = 2^d s, exact in FP64 for these magnitudes)
+        # against the FP64 vector peak (one FMA = 2 flops)
+        leaf = float(np.sum([karatsuba_leaf_macs(int(L)) for L in lens]))
+        extras["c5_roofline"] = {"bound": "valu-fp64", "achieved": 2 * leaf / (msk * 1e-3) / 1e12,
+                                 "peak": 78.6, "unit": "TFLOP/s",
+                                 "frac": 2 * leaf / (msk * 1e-3) / 1e12 / 78.6, "leaf_macs": leaf,
+                                 "kernel": "k_karatsuba (one workgroup per pair)"}
         extras.update(karatsuba_fft_f32(torch, np, lens, kdata, steps, warmup))
         del kar, kdata
         extras.update(uneven_batch(torch, np, steps))

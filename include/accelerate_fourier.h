@@ -311,6 +311,14 @@ af_status af_karatsuba_complex_i64(int64_t batch, int64_t len, int32_t cyclic,
 af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign,
                           int32_t precision, af_batch *batch);
 af_status af_batch_destroy(af_batch batch);
+/*
+ * One call runs every row.  A batch owns one set of launch descriptors and
+ * intermediates, so calls on one batch are serialised: concurrent calls from
+ * several threads are allowed, and a call on another stream is ordered after
+ * the previous call's work (an event it waits on); a call with new buffers
+ * waits on the host until the previous call's work is done before it rewrites
+ * the descriptors.  Distinct batches run concurrently.
+ */
 af_status af_batch_execute(af_batch batch, const void *in_re, const void *in_im,
                            void *out_re, void *out_im, void *stream);
 af_status af_batch_describe(af_batch batch, char *buf, size_t buflen);

@@ -443,6 +443,31 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
             }
         };
     }
+    if (name.find("af::k_kar_split") == 0 || name.find("af::k_kar_join") == 0) {
+        // split tasks (x, y, L, cx, cy): x, y hold L words, cx, cy 3 ceil(L/2);
+        // join tasks (zc, L, z, cyclic): zc holds 3 (2h - 1), z 2L - 1 (L if cyclic)
+        const bool split = name.find("split") != std::string::npos;
+        const int64_t *t = *(const int64_t *const *)args[0];
+        const int64_t nt = *(const int64_t *)args[1];
+        return [=] {
+            if (!in_alloc(t, 0, nt * (split ? 40 : 32), "task table", name)) return;
+            for (int64_t i = 0; i < nt; ++i) {
+                const int64_t *k = t + i * (split ? 5 : 4);
+                const int64_t L = k[split ? 2 : 1], h = (L + 1) / 2;
+                if (L < 2) problem(name + ": a split needs a length of at least 2");
+                if (split) {
+                    in_alloc((const void *)k[0], 0, L * 8, "x", name);
+                    in_alloc((const void *)k[1], 0, L * 8, "y", name);
+                    in_alloc((const void *)k[3], 0, 3 * h * 8, "cx", name);
+                    in_alloc((const void *)k[4], 0, 3 * h * 8, "cy", name);
+                } else {
+                    in_alloc((const void *)k[0], 0, 3 * (2 * h - 1) * 8, "zc", name);
+                    in_alloc((const void *)k[2], 0, (k[3] ? L : 2 * L - 1) * 8, "z", name);
+                }
+            }
+            if (block.x != 256) problem(name + ": block differs from 256");
+        };
+    }
     if (name.find("af::k_gen_twiddles<") == 0) {
         void *out = *(void **)args[0];
         const int64_t count = *(const int64_t *)args[2];
@@ -624,6 +649,7 @@ hipError_t hipEventCreate(hipEvent_t *e) {
 hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
 hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned int) { return hipSuccess; }
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) {
     *ms = 1.0f;
     return hipSuccess;

@@ -205,17 +205,19 @@ def test_uneven_batch_4096_rows(dry, prec):
 def test_karatsuba(dry):
     before = dry.dry_problem_count()
     b = Bufs(dry)
-    for L in (1, 2, 63, 64, 65, 127, 128, 129, 500, 512, 513, 1000, 2048, 2049, 4095, 4096, 4097, 4608):
+    # past 4608 (the LDS kernel's longest padded pair) the top levels split in global memory
+    for L in (1, 2, 63, 64, 65, 127, 128, 129, 500, 512, 513, 1000, 2048, 2049, 4095, 4096, 4097, 4608, 4609,
+              5000, 9217, 16384, 100000):
         for batch in (1, 3):
             x, y = b(batch * L * 8), b(batch * L * 8)
             assert dry.af_karatsuba_i64(batch, L, x, y, b(batch * (2 * L - 1) * 8), None) == 0
             assert dry.af_karatsuba_cyclic_i64(batch, L, x, y, b(batch * L * 8), None) == 0
-            if L in (1, 65, 4096, 4608):
+            if L in (1, 65, 4096, 4608, 5000):
                 xi, yi = b(batch * L * 8), b(batch * L * 8)
                 for cyc, nz in ((0, 2 * L - 1), (1, L)):
                     assert dry.af_karatsuba_complex_i64(batch, L, cyc, x, xi, y, yi, b(batch * nz * 8),
                                                         b(batch * nz * 8), None) == 0
-    lens = np.array([256, 4096, 1, 77, 3000, 4096, 2049], dtype=np.int64)
+    lens = np.array([256, 4096, 1, 77, 3000, 4096, 2049, 5000, 20000, 4609], dtype=np.int64)
     offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
     arr = (ctypes.c_int64 * len(offs))(*[int(v) for v in offs])
     tot = int(offs[-1])

@@ -222,6 +222,52 @@ def test_karatsuba_ragged_mixed_magnitudes():
         assert np.array_equal(got, ref), (p, L, mags[p])
 
 
+@pytest.mark.parametrize("L", [4609, 5000, 16384, 100000])
+def test_karatsuba_past_lds_limit(L):
+    """Convolution.Adhoc.karatsuba has no length limit (Adhoc.hs:33-64): pairs
+    whose padded length exceeds one workgroup's LDS split their top levels in
+    global memory.  Linear and cyclic, bit for bit against the oracle's
+    restatement of the reference's Karatsuba, with coefficients small enough for
+    the exact-FP64 leaves and large enough to wrap around."""
+    import oracle
+    for m in (2 ** 15, 2 ** 62):
+        batch = 2 if L < 20000 else 1
+        x = synthetic.integers(900 + L + m % 7, batch * L, -m, m).reshape(batch, L)
+        y = synthetic.integers(950 + L + m % 7, batch * L, -m, m).reshape(batch, L)
+        xd, yd = to_dev(x, torch.int64), to_dev(y, torch.int64)
+        ref = oracle.karatsuba(x, y)
+        z = host(convolution.karatsuba(xd, yd))
+        assert np.array_equal(z, ref), (L, m)
+        # Convolution.cyclic: the tail wrapped onto the head (Adhoc.hs:104-111)
+        refc = ref[:, :L].copy()
+        refc[:, :L - 1] += ref[:, L:]
+        zc = host(convolution.cyclic(convolution.karatsuba)(xd, yd))
+        assert np.array_equal(zc, refc), (L, m)
+
+
+def test_karatsuba_ragged_and_complex_past_lds_limit():
+    """Ragged pairs on both sides of the LDS limit in one call, and the Gauss
+    form of Convolution.complex over pairs past it, against the oracle."""
+    import oracle
+    lens = [5000, 300, 9000, 4608, 4609, 1, 12345]
+    xs = [synthetic.integers(1200 + i, L, -2 ** 20, 2 ** 20) for i, L in enumerate(lens)]
+    ys = [synthetic.integers(1300 + i, L, -2 ** 20, 2 ** 20) for i, L in enumerate(lens)]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    z = host(convolution.karatsuba_ragged(off, to_dev(np.concatenate(xs), torch.int64),
+                                          to_dev(np.concatenate(ys), torch.int64)))
+    for p, L in enumerate(lens):
+        ref = oracle.karatsuba(xs[p][None, :], ys[p][None, :])[0]
+        assert np.array_equal(z[2 * off[p] - p: 2 * off[p] - p + 2 * L - 1], ref), (p, L)
+    L = 6000
+    xr, xi, yr, yi = (synthetic.integers(1400 + i, L, -2 ** 40, 2 ** 40).reshape(1, L) for i in range(4))
+    dev = [to_dev(v, torch.int64) for v in (xr, xi, yr, yi)]
+    for cyc in (False, True):
+        conv = convolution.complex(convolution.cyclic(convolution.karatsuba) if cyc else convolution.karatsuba)
+        zr, zi = conv((dev[0], dev[1]), (dev[2], dev[3]))
+        rr, ri = oracle.complex_karatsuba_i64(xr, xi, yr, yi, cyclic=cyc)
+        assert np.array_equal(host(zr), rr) and np.array_equal(host(zi), ri), cyc
+
+
 def test_uneven_batch():
     import oracle
     lengths = [64, 1, 0, 60, 97, 257, 1000, 4099, 20000, 65536, 17, 65521, 100003, 5]
@@ -578,39 +624,143 @@ def test_karatsuba_full_config_bit_exact():
     assert np.array_equal(z, oracle.karatsuba_ragged(off, x, y))
 
 
+def _lsr(z, k):
+    """Logical right shift of int64 words (torch shifts arithmetically)."""
+    return (z >> k) & ((1 << (64 - k)) - 1)
+
+
+def _i64(c):
+    return c - (1 << 64) if c >= 1 << 63 else c
+
+
+def dev_uniform(seed, count, start=0, chunk=1 << 26):
+    """synthetic.uniform(seed, count, -1, 1, start) computed on the GPU in FP64:
+    the same counter-based splitmix64 stream in int64 words (products wrap)."""
+    out = torch.empty(count, dtype=torch.float64, device=DEV)
+    g, c1, c2 = _i64(0x9E3779B97F4A7C15), _i64(0xBF58476D1CE4E5B9), _i64(0x94D049BB133111EB)
+    for lo in range(0, count, chunk):
+        hi = min(count, lo + chunk)
+        z = torch.arange(start + lo + 1, start + hi + 1, dtype=torch.int64, device=DEV) * g + seed
+        z = (z ^ _lsr(z, 30)) * c1
+        z = (z ^ _lsr(z, 27)) * c2
+        z = z ^ _lsr(z, 31)
+        out[lo:hi] = _lsr(z, 11).to(torch.float64) * (2.0 / 9007199254740992.0) - 1.0
+    return out
+
+
+def _uneven_inputs():
+    """The 4096-row batch's lengths and its input, U[-1, 1) from splitmix64
+    seed 2 (real parts, then imaginary parts), checked against the host stream."""
+    lens, cls = synthetic.uneven_lengths(2)
+    tot = int(lens.sum())
+    probe = dev_uniform(2, 4096, start=tot - 1000).cpu().numpy()
+    assert np.array_equal(probe, synthetic.uniform(2, 4096, start=tot - 1000))
+    x64 = torch.complex(dev_uniform(2, tot), dev_uniform(2, tot, start=tot))
+    return lens, cls, x64
+
+
+def _launch_keys(desc):
+    """(key text, first row) of every grouped launch in a batch description."""
+    keys = []
+    for line in desc.split("\n")[1:]:
+        head, first = line.rsplit(" first=", 1)
+        keys.append((head.split(" ", 2)[2].rsplit(" rows=", 1)[0], int(first)))
+    return keys
+
+
 def test_uneven_batch_full_config():
-    """The full uneven batch (4096 rows, four length classes, seed 2), FP32:
-    every row within 1e-6 relative L2 of the FP64 batch on the same inputs, and
-    two rows of each class (the shortest and the longest) against the oracle in
-    FP64 at the oracle's own bound.  Inputs U[-1, 1) drawn on the device by a
-    seeded torch generator (2.9e8 values)."""
+    """The full uneven batch (4096 rows, four length classes, splitmix64 seed 2),
+    FP32, in both layouts.  FP32 rows against numpy's FP64 FFT of the same
+    (FP32-rounded) inputs -- an implementation independent of this library --
+    at 1e-6 relative L2: 32 rows per class spread over its lengths, plus the
+    first row of every grouped launch of the plan (each extended-kernel key
+    (L, B, conv, sign) and each generic thread class of every phase).  Every
+    row of the FP32 batch within 1e-6 of the FP64 batch, and two rows per class
+    of the FP64 batch against the oracle."""
     import oracle
     from accelerate_fourier.batch import UnevenBatch
-    lens, cls = synthetic.uneven_lengths(2)
+    lens, cls, x64 = _uneven_inputs()
     off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
-    tot = int(off[-1])
-    g = torch.Generator(device=DEV)
-    g.manual_seed(2)
-    v = torch.rand(2, tot, dtype=torch.float64, device=DEV, generator=g) * 2 - 1
-    x64 = torch.complex(v[0], v[1])
-    del v
+    b32 = UnevenBatch(lens, forward, "f32")
+    keys = _launch_keys(b32.describe())
+    assert len(keys) >= 10
+    rows = {r for _, r in keys}
+    for c in range(len(synthetic.UNEVEN_CLASSES)):
+        members = np.nonzero(cls == c)[0]
+        members = members[np.argsort(lens[members], kind="stable")]
+        rows.update(int(members[i]) for i in np.linspace(0, len(members) - 1, 32).round().astype(int))
+    x32 = x64.to(torch.complex64)
+    y_int = b32(x32)
+    y_pla = b32((x32.real.contiguous(), x32.imag.contiguous()))
+    torch.cuda.synchronize()
+    worst = {}
+    for r in sorted(rows):
+        lo, hi = int(off[r]), int(off[r + 1])
+        xr = host(x32[lo:hi]).astype(np.complex128)
+        ref = np.fft.fft(xr)
+        for name, got in (("interleaved", host(y_int[lo:hi])),
+                          ("planar", host(y_pla[0][lo:hi]) + 1j * host(y_pla[1][lo:hi]))):
+            e = rel_l2(got, ref)
+            assert e <= 1e-6, (name, r, int(lens[r]), e)
+            c = int(cls[r])
+            worst[c] = max(worst.get(c, 0.0), e)
+    print(f"uneven batch: {len(rows)} rows vs numpy FP64 in both layouts, launch keys covered: "
+          + "; ".join(k for k, _ in keys) + "; worst rel-L2 per class " + str(worst))
+    del y_pla
     y64 = UnevenBatch(lens, forward, "f64")(x64)
-    y32 = UnevenBatch(lens, forward, "f32")(x64.to(torch.complex64))
     sizes = [int(n) for n in lens]
-    err = y32.to(torch.complex128) - y64
+    err = y_int.to(torch.complex128) - y64
     d = torch.stack([torch.linalg.vector_norm(r) for r in torch.split(err, sizes)])
     nrm = torch.stack([torch.linalg.vector_norm(r) for r in torch.split(y64, sizes)])
     rel = (d / nrm).cpu().numpy()
     del err
     assert rel.max() <= 1e-6, (int(rel.argmax()), float(rel.max()))
     for c in range(len(synthetic.UNEVEN_CLASSES)):
-        rows = np.nonzero(cls == c)[0]
-        for r in (rows[np.argmin(lens[rows])], rows[np.argmax(lens[rows])]):
+        members = np.nonzero(cls == c)[0]
+        for r in (members[np.argmin(lens[members])], members[np.argmax(lens[members])]):
             n = int(lens[r])
             ref = oracle.transform(-1, host(x64[off[r]:off[r + 1]]))
             got = host(y64[off[r]:off[r + 1]])
             assert rel_l2(got, ref) <= tol_f64(n), (c, int(r), n)
-    del x64, y64, y32
+    del x64, y64, y_int, x32
+
+
+def test_uneven_batch_two_streams_and_threads():
+    """One batch called from two streams back to back (same buffers, then new
+    output buffers) and from two host threads: every call's result equals the
+    single-stream result (calls on one batch are ordered by the batch)."""
+    import threading
+    from accelerate_fourier.batch import UnevenBatch
+    lengths = [100003, 65536, 4099, 20000, 97, 600011]
+    x = to_dev(synthetic.complex_uniform(31, (sum(lengths),)), torch.complex64)
+    b = UnevenBatch(lengths, forward, "f32")
+    ref = host(b(x))
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    outs = [torch.empty_like(x) for _ in range(4)]
+    with torch.cuda.stream(s1):
+        b(x, out=outs[0])
+    with torch.cuda.stream(s2):
+        b(x, out=outs[1])
+        b(x, out=outs[0])
+    with torch.cuda.stream(s1):
+        b(x, out=outs[2])
+    torch.cuda.synchronize()
+    for o in outs[:3]:
+        assert np.array_equal(host(o), ref)
+
+    def work(o):
+        st = torch.cuda.Stream()
+        with torch.cuda.stream(st):
+            for _ in range(3):
+                b(x, out=o)
+        st.synchronize()
+    th = [threading.Thread(target=work, args=(outs[i],)) for i in (1, 3)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    torch.cuda.synchronize()
+    assert np.array_equal(host(outs[1]), ref) and np.array_equal(host(outs[3]), ref)
 
 
 def test_concurrent_plans_and_executes():
