@@ -55,6 +55,7 @@ _SIGNATURES = {
     "af_plan_describe": (ctypes.c_int, [ctypes.POINTER(PlanDesc), ctypes.c_char_p, ctypes.c_size_t]),
     "af_plan_workspace_size": (ctypes.c_int, [_v, _i64, ctypes.POINTER(ctypes.c_size_t)]),
     "af_execute": (ctypes.c_int, [_v, _i64, _v, _v, _v, _v, _v, _v]),
+    "af_execute_noalloc": (ctypes.c_int, [_v, _i64, _v, _v, _v, _v, _v]),
     "af_plan_num_passes": (ctypes.c_int, [_v, _i64, ctypes.POINTER(_i32)]),
     "af_execute_pass": (ctypes.c_int, [_v, _i32, _i64, _v, _v, _v, _v, _v, _v]),
     "af_two_to_spectrum": (ctypes.c_int, [_v, _i64, _v, _v, _v, _v, _v, _v, _v]),

@@ -50,6 +50,37 @@ af_status hip_try(hipError_t e, const char *call, int line) { return e == hipSuc
 std::once_flag g_init_flag;
 af_status g_init_status = AF_SUCCESS;
 
+}  // namespace
+
+// Scratch of af_execute_noalloc: one fixed buffer per GPU, in the code
+// object's zero-initialised data (no file space), held until the process
+// ends.  Host code never passes `scratch` itself to a kernel: its host-side
+// name is only a handle; the buffer's device address comes from
+// hipGetSymbolAddress, once per device (noalloc_scratch).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the
+// first call for that device (device_init) and kept.
+af_status noalloc_scratch(int dev, void **ptr) {
+    if (dev < 0 || dev >= kMaxDevices) return AF_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return AF_INTERNAL;
+        g_noalloc_ptr[dev] = p;
+    }
+    *ptr = g_noalloc_ptr[dev];
+    return AF_SUCCESS;
+}
+
 af_status device_init() {
     std::call_once(g_init_flag, [] {
         int n = 0;
@@ -66,7 +97,9 @@ af_status device_init() {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return AF_UNSUPPORTED;
-    return AF_SUCCESS;
+    // set-up of the allocation-free entry point's buffer on this device
+    void *p = nullptr;
+    return noalloc_scratch(dev, &p);
 }
 
 // Device buffer owned by a plan.
@@ -1053,6 +1086,28 @@ af_status af_execute(af_plan plan, int64_t batch, const void *in_re, const void 
     if (batch > 0 && plan->desc.length > 0 && bad_ptrs(in_re, out_re)) return AF_INVALID_ARG;
     if (batch * std::max<int64_t>(plan->desc.length, 1) > ((int64_t)1 << 40)) return AF_INVALID_ARG;
     return run_c2c(plan, plan->desc.sign, batch, in_re, in_im, out_re, out_im, workspace, (hipStream_t)stream);
+}
+
+af_status af_execute_noalloc(af_plan plan, int64_t batch, const void *in_re, const void *in_im, void *out_re,
+                             void *out_im, void *stream) {
+    if (!plan || plan->desc.kind != AF_C2C || batch < 0) return AF_INVALID_ARG;
+    if (batch > 0 && plan->desc.length > 0 && bad_ptrs(in_re, out_re)) return AF_INVALID_ARG;
+    if (batch * std::max<int64_t>(plan->desc.length, 1) > ((int64_t)1 << 40)) return AF_INVALID_ARG;
+    const size_t need = plan->scratch_per_row() * (size_t)batch;
+    if (need > kNoAllocScratchBytes) {
+        std::snprintf(g_last_error, sizeof g_last_error,
+                      "af_execute_noalloc: %zu bytes of scratch needed for %lld rows, the fixed buffer holds %zu",
+                      need, (long long)batch, kNoAllocScratchBytes);
+        return AF_UNSUPPORTED;
+    }
+    AFCHK(device_init());
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    void *ws = nullptr;
+    AFCHK(noalloc_scratch(dev, &ws));
+    // every pass writes the scratch it later reads, so it needs no clearing
+    return run_c2c(plan, plan->desc.sign, batch, in_re, in_im, out_re, out_im, need ? ws : nullptr,
+                   (hipStream_t)stream);
 }
 
 af_status af_plan_num_passes(af_plan plan, int64_t batch, int32_t *npasses) {

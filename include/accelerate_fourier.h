@@ -114,6 +114,21 @@ af_status af_execute(af_plan plan, int64_t batch,
                      void *workspace, void *stream);
 
 /*
+ * af_execute without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * scratch a plan needs (af_plan_workspace_size) comes from one fixed buffer
+ * of 64 MiB per GPU, set up when the library first runs on that GPU and held
+ * until the process ends; a call whose scratch exceeds it returns
+ * AF_UNSUPPORTED (af_last_error names the sizes) and runs nothing.  Calls
+ * that overlap in time on one GPU share the buffer and race: keep them on
+ * one stream, or order them.  The caller passes inputs and outputs as for
+ * af_execute; the call makes no hipMalloc/hipFree.
+ */
+af_status af_execute_noalloc(af_plan plan, int64_t batch, const void *in_re, const void *in_im,
+                             void *out_re, void *out_im, void *stream);
+
+/*
  * The launches of one af_execute call, one at a time, for profiling and for
  * callers that interleave other work between them.  af_plan_num_passes gives
  * the count for `batch` rows; pass i of af_execute_pass runs launch i of the

@@ -555,6 +555,14 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
 
 }  // namespace
 
+namespace {
+// device variables: their registered size, by host handle
+std::map<const void *, size_t> &dev_vars() {
+    static auto *m = new std::map<const void *, size_t>;
+    return *m;
+}
+}  // namespace
+
 extern "C" {
 
 // ---- registration ----
@@ -572,7 +580,9 @@ void __hipRegisterFunction(void **, const void *hostFunction, char *deviceFuncti
     g_kernels[hostFunction] = n;
     std::free(dm);
 }
-void __hipRegisterVar(void **, void *, char *, const char *, int, size_t, int, int) {}
+void __hipRegisterVar(void **, void *var, char *, const char *, int, size_t size, int, int) {
+    dev_vars()[var] = size;
+}
 
 hipError_t __hipPushCallConfiguration(dim3 gridDim, dim3 blockDim, size_t sharedMem, hipStream_t stream) {
     g_cfg.push_back({gridDim, blockDim, sharedMem, stream});
@@ -618,6 +628,20 @@ hipError_t hipMalloc(void **p, size_t n) {
     return hipSuccess;
 }
 hipError_t hipMallocAsync(void **p, size_t n, hipStream_t) { return hipMalloc(p, n); }
+// a device variable's "device" address: one allocation of its registered size
+// per variable (the host handle itself is never a device address)
+hipError_t hipGetSymbolAddress(void **p, const void *sym) {
+    static std::map<const void *, void *> addr;
+    auto it = dev_vars().find(sym);
+    if (it == dev_vars().end()) return hipErrorInvalidSymbol;
+    void *&a = addr[sym];
+    if (!a) {
+        const hipError_t e = hipMalloc(&a, it->second);
+        if (e != hipSuccess) return e;
+    }
+    *p = a;
+    return hipSuccess;
+}
 hipError_t hipFree(void *p) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!p) return hipSuccess;

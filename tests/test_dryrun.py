@@ -47,6 +47,8 @@ def dry():
     lib.af_plan_create.argtypes = [ctypes.POINTER(PlanDesc), ctypes.POINTER(V)]
     lib.af_plan_destroy.argtypes = [V]
     lib.af_execute.argtypes = [V, ctypes.c_int64, V, V, V, V, V, V]
+    lib.af_execute_noalloc.argtypes = [V, ctypes.c_int64, V, V, V, V, V]
+    lib.af_plan_workspace_size.argtypes = [V, ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]
     lib.af_vjp_execute.argtypes = [V, ctypes.c_int64, V, V, V, V, V, V]
     lib.af_to_spectrum.argtypes = [V, ctypes.c_int64, V, V, V, V, V]
     lib.af_from_spectrum.argtypes = [V, ctypes.c_int64, V, V, V, V, V]
@@ -346,3 +348,32 @@ def test_multidimensional(dry):
             for p in (p0, p1, p2):
                 dry.af_plan_destroy(p)
     assert problems(dry, before) == []
+
+
+def test_execute_noalloc(dry):
+    """af_execute_noalloc: plans with scratch (four-step, Bluestein, Rader,
+    three-pass) run their launches on the fixed per-device buffer (every
+    footprint inside it); a call whose scratch exceeds the buffer returns
+    AF_UNSUPPORTED and launches nothing."""
+    before = dry.dry_problem_count()
+    b = Bufs(dry)
+    for n, prec, batch in ((65536, 1, 64), (65536, 0, 8), (100003, 1, 4), (4099, 0, 3), (65521, 1, 2),
+                           (1 << 24, 1, 1), (1000, 0, 5)):
+        d = PlanDesc(n, -1, C2C, prec, 0, 0)
+        p = V()
+        assert dry.af_plan_create(ctypes.byref(d), ctypes.byref(p)) == 0
+        ws = ctypes.c_size_t()
+        assert dry.af_plan_workspace_size(p, batch, ctypes.byref(ws)) == 0
+        e = 16 if prec == 0 else 8
+        x, y = b(batch * n * e), b(batch * n * e)
+        launches = dry.dry_launches()
+        st = dry.af_execute_noalloc(p, batch, x, None, y, None, None)
+        if ws.value <= 64 << 20:
+            assert st == 0, (n, prec, batch)
+            assert dry.dry_launches() > launches
+        else:
+            assert st == 2 and dry.dry_launches() == launches, (n, prec, batch)
+        dry.af_plan_destroy(p)
+    b.free()
+    assert problems(dry, before) == []
+
