@@ -54,6 +54,12 @@ def parse():
     ap.add_argument("--profile-conv", action="store_true",
                     help="only the 512 x 8192 FP32 real convolution and its adjoint (warmup + steps "
                          "calls), for rocprofv3 kernel traces")
+    ap.add_argument("--profile-c2b", action="store_true",
+                    help="only one 2^24 FP32 transform (warmup + steps runs) and a 128 MiB device copy of "
+                         "known size, for rocprofv3 kernel traces and counters")
+    ap.add_argument("--profile-c1", action="store_true",
+                    help="only the FP64 256 x 1024 transform, autotuned (warmup + steps runs), for rocprofv3 "
+                         "kernel traces")
     ap.add_argument("--profile-karatsuba", action="store_true",
                     help="only the 1024-pair ragged Int64 Karatsuba (warmup + steps calls), for "
                          "rocprofv3 kernel traces")
@@ -343,6 +349,26 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"profile": "done", "calls": warmup + steps,
                           "workload": "conv" if args.profile_conv else "karatsuba"}))
+        return
+    if args.profile_c2b or args.profile_c1:
+        if args.profile_c2b:
+            g = C2C(1 << 24, 1, _abi.F32)
+            g.set_io(synthetic.complex_normal(1, (1, 1 << 24)))
+        else:
+            g = C2C(1024, 256, _abi.F64)
+            g.set_io(synthetic.complex_uniform(0, (256, 1024)))
+            if not args.no_tune:
+                g.tune()
+        for _ in range(warmup + steps):
+            g.run()
+        if args.profile_c2b:
+            src = torch.empty(128 * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
+            dst = torch.empty_like(src)
+            for _ in range(steps):
+                dst.copy_(src)  # reference kernel: reads and writes exactly 128 MiB
+        torch.cuda.synchronize()
+        print(json.dumps({"profile": "done", "workload": "c2b 2^24 f32" if args.profile_c2b else "c1 256x1024 f64",
+                          "transform_launches": g.npasses * (warmup + steps)}))
         return
     n, batch = 65536, 64
     flops = 5.0 * n * math.log2(n) * batch
