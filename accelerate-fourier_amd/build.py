@@ -20,7 +20,7 @@ BUILD = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libaccelerate_fourier_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["fft_kernels.hip", "fast_pass.hip", "fast_ext.hip", "karatsuba.hip", "abi.cpp", "planner.cpp"]
+SOURCES = ["fft_kernels.hip", "fast_pass.hip", "fast_ext.hip", "decomp.hip", "karatsuba.hip", "abi.cpp", "planner.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -561,12 +561,21 @@ struct af_plan_s {
     af_plan sub = nullptr;                          // convolveCyclic of non-smooth lengths, or the half-length
                                                     // transform of unfused real kinds: C2C
     bool real_fused = false;
+    // decomposition plans (ALG_DECOMP): Good-Thomas and Cooley-Tukey run
+    // dec_a (length a) then dec_b (length b); Rader runs dec_a (p - 1, sign
+    // -1) and dec_b (p - 1, sign +1) around the spectrum product.  dec_in /
+    // dec_out: the input and output index maps on the device.
+    af_plan dec_a = nullptr, dec_b = nullptr;
+    int32_t *dec_in = nullptr, *dec_out = nullptr;
+    void *dec_spec = nullptr;
     // autotuned tile lines and radix orders of the C2C passes, per (batch, planar)
     std::map<std::pair<int64_t, int>, std::vector<PassTune>> tuned;
     std::mutex tune_mu;
     ~af_plan_s() {
         if (dual) af_plan_destroy(dual);
         if (sub) af_plan_destroy(sub);
+        if (dec_a) af_plan_destroy(dec_a);
+        if (dec_b) af_plan_destroy(dec_b);
     }
     size_t scratch_per_row() const {
         const size_t e = (size_t)elem_bytes(desc.precision);
@@ -576,6 +585,7 @@ struct af_plan_s {
             n = (size_t)core.sh.N * (core.sh.kind == CORE_FOUR ? 2 : 1);
             return e * n;
         }
+        if (shape.algo == ALG_DECOMP) return e * 2 * (size_t)shape.n;
         if (core.sh.kind == CORE_FOUR) n = (size_t)core.sh.N + (shape.algo == ALG_RADER ? 2 : 0);
         if (core.sh.kind == CORE_THREE) n = (size_t)core.sh.N;
         return e * n;
@@ -775,6 +785,76 @@ static af_status plan_create_impl(const af_plan_desc *desc, af_plan *plan, bool 
 
 af_status af_plan_create(const af_plan_desc *desc, af_plan *plan) { return plan_create_impl(desc, plan, true); }
 
+// Sub-plans and index maps of a decomposition plan (Planned.hs:255-293 with no
+// Bluestein): Good-Thomas maps as af_good_thomas_maps (the reference's
+// permuteSkewGrid / permuteSkewGridInv), Rader's g^-q gather and g^r scatter
+// and its kernel spectrum F-(b) / (p - 1), b_m = exp(sign 2 pi i g^m / p), from
+// an FP64 decomposition of p - 1.
+static af_status build_decomp(af_plan P, hipStream_t s) {
+    const PlanShape &sh = P->shape;
+    auto sub = [&](int64_t len, int sign, af_plan *out) {
+        af_plan_desc d = P->desc;
+        d.length = len;
+        d.sign = sign;
+        d.algorithm = AF_ALG_DECOMPOSE;
+        return plan_create_impl(&d, out, false);
+    };
+    const int64_t n = sh.n;
+    if (n >= ((int64_t)1 << 31)) return AF_UNSUPPORTED;
+    if (sh.dec == DEC_GOOD_THOMAS || sh.dec == DEC_COOLEY_TUKEY) {
+        AFCHK(sub(sh.da, P->desc.sign, &P->dec_a));
+        AFCHK(sub(sh.db, P->desc.sign, &P->dec_b));
+        if (sh.dec == DEC_COOLEY_TUKEY) return AF_SUCCESS;
+        std::vector<int64_t> im(n), om(n);
+        AFCHK(af_good_thomas_maps(sh.da, sh.db, im.data(), om.data()));
+        std::vector<int32_t> im32(im.begin(), im.end()), om32(om.begin(), om.end());
+        AFCHK(P->own.alloc((void **)&P->dec_in, n * 4));
+        AFCHK(P->own.alloc((void **)&P->dec_out, n * 4));
+        HIPCHK(hipMemcpyAsync(P->dec_in, im32.data(), n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(P->dec_out, om32.data(), n * 4, hipMemcpyHostToDevice, s));
+        return HIPST(hipStreamSynchronize(s));
+    }
+    // Rader: p = n, convolution length L = p - 1
+    const int64_t p = n, L = p - 1;
+    AFCHK(sub(L, -1, &P->dec_a));
+    AFCHK(sub(L, 1, &P->dec_b));
+    const int64_t g = multiplicative_generator(p);
+    std::vector<int32_t> powg(L), inv_pow(L);
+    int64_t x = 1;
+    for (int64_t i = 0; i < L; ++i) {
+        powg[i] = (int32_t)x;
+        x = (int64_t)(((unsigned __int128)x * (uint64_t)g) % (uint64_t)p);
+    }
+    for (int64_t i = 0; i < L; ++i) inv_pow[i] = powg[(L - i) % L];  // g^-i
+    AFCHK(P->own.alloc((void **)&P->dec_in, L * 4));
+    AFCHK(P->own.alloc((void **)&P->dec_out, L * 4));
+    HIPCHK(hipMemcpyAsync(P->dec_in, inv_pow.data(), L * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(P->dec_out, powg.data(), L * 4, hipMemcpyHostToDevice, s));
+    af_plan f64 = nullptr;
+    {
+        af_plan_desc d = P->desc;
+        d.length = L;
+        d.sign = -1;
+        d.precision = AF_F64;
+        d.algorithm = AF_ALG_DECOMPOSE;
+        d.flags = 0;
+        AFCHK(plan_create_impl(&d, &f64, false));
+    }
+    void *kb = nullptr, *kf = nullptr;
+    af_status st = HIPST(hipMalloc(&kb, L * 16));
+    if (st == AF_SUCCESS) st = HIPST(hipMalloc(&kf, L * 16));
+    if (st == AF_SUCCESS) st = HIPST(gen_rader_kernel_d(kb, P->dec_out, p, L, P->desc.sign, s));
+    if (st == AF_SUCCESS) st = af_execute(f64, 1, kb, nullptr, kf, nullptr, nullptr, s);
+    if (st == AF_SUCCESS) st = P->own.alloc(&P->dec_spec, L * elem_bytes(P->desc.precision));
+    if (st == AF_SUCCESS) st = HIPST(convert_to_plain(P->prec, P->dec_spec, kf, L, 1.0 / (double)L, 0, 0, s));
+    { const af_status h__ = HIPST(hipStreamSynchronize(s)); if (st == AF_SUCCESS) st = h__; }
+    if (kb) (void)hipFree(kb);
+    if (kf) (void)hipFree(kf);
+    af_plan_destroy(f64);
+    return st;
+}
+
+
 static af_status plan_create_impl(const af_plan_desc *desc, af_plan *plan, bool with_dual) {
     if (!plan) return AF_INVALID_ARG;
     *plan = nullptr;
@@ -817,13 +897,16 @@ static af_status plan_create_impl(const af_plan_desc *desc, af_plan *plan, bool 
         one.kind = CORE_SINGLE;
         one.N = 1;
         P->core.sh = one;
+    } else if (sh.algo == ALG_DECOMP) {
+        st = build_decomp(P.get(), s);
     } else if (sh.algo != ALG_EMPTY) {
         st = build_core(P->prec, sh.core, P->core, P->own, s);
     }
     if (st == AF_SUCCESS && desc->kind == AF_C2C) {
         if (sh.algo == ALG_RADER) st = build_rader(P.get(), s);
         else if (sh.algo == ALG_BLUESTEIN) st = build_bluestein(P.get(), s);
-        if (st == AF_SUCCESS && with_dual && (sh.algo == ALG_RADER || sh.algo == ALG_BLUESTEIN)) {
+        if (st == AF_SUCCESS && with_dual &&
+            (sh.algo == ALG_RADER || sh.algo == ALG_BLUESTEIN || sh.algo == ALG_DECOMP)) {
             // the adjoint needs the tables of the opposite sign (cacheDuplex, Planned.hs:335-341)
             af_plan_desc d2 = *desc;
             d2.sign = -desc->sign;
@@ -915,6 +998,8 @@ af_status build_c2c(af_plan P, int sign, int64_t batch, const Endpoint &in, cons
 
 af_status build_c2c_passes(af_plan P, int sign, int64_t batch, const Endpoint &in, const Endpoint &out,
                            void *scratch, std::vector<AfPass> &ps) {
+    // decomposition plans are composed on the host, not a sequence of passes
+    if (P->shape.algo == ALG_DECOMP) return AF_UNSUPPORTED;
     if (sign != P->desc.sign && (P->shape.algo == ALG_RADER || P->shape.algo == ALG_BLUESTEIN)) {
         // Rader kernels and chirps depend on the sign: the opposite sign is the dual plan's
         if (!P->dual) return AF_INTERNAL;
@@ -978,9 +1063,61 @@ af_status build_c2c_passes(af_plan P, int sign, int64_t batch, const Endpoint &i
 }
 
 af_status run_c2c(af_plan P, int sign, int64_t batch, const void *in_re, const void *in_im, void *out_re,
+                  void *out_im, void *ws, hipStream_t s);
+
+// A decomposition plan (ALG_DECOMP) of its own sign over `batch` rows: the
+// steps of Good-Thomas, Cooley-Tukey or Rader (decomp.hip) around the
+// sub-plans, intermediates in two interleaved buffers of batch x n.
+af_status run_decomp(af_plan P, int64_t batch, const void *in_re, const void *in_im, void *out_re, void *out_im,
+                     void *ws, hipStream_t s) {
+    if (!P || !P->dec_a || !P->dec_b) return AF_INTERNAL;
+    const int64_t n = P->shape.n, a = P->shape.da, b = P->shape.db;
+    const size_t eb = (size_t)elem_bytes(P->desc.precision);
+    const int prec = P->prec;
+    Scratch sc;
+    AFCHK(sc.get(2 * (size_t)batch * (size_t)n * eb, ws, s));
+    void *t1 = sc.p, *t2 = (char *)sc.p + (size_t)batch * (size_t)n * eb;
+    af_plan A = P->dec_a, B = P->dec_b;
+    switch (P->shape.dec) {
+    case DEC_GOOD_THOMAS:
+        // t1[r][j2][j1] = x[r][(a j2 + b j1) mod n]; rows over j1; transpose; rows over j2;
+        // out[r][q] = t2[r][(q mod a) b + (q mod b)]
+        HIPCHK(launch_dc_gather(prec, in_re, in_im, n, t1, nullptr, n, P->dec_in, batch, s));
+        AFCHK(run_c2c(A, A->desc.sign, batch * b, t1, nullptr, t2, nullptr, nullptr, s));
+        HIPCHK(launch_transpose(prec, t2, nullptr, t1, nullptr, b, a, batch, s));
+        AFCHK(run_c2c(B, B->desc.sign, batch * a, t1, nullptr, t2, nullptr, nullptr, s));
+        HIPCHK(launch_dc_gather(prec, t2, nullptr, n, out_re, out_im, n, P->dec_out, batch, s));
+        return AF_SUCCESS;
+    case DEC_COOLEY_TUKEY:
+        // x[r][b j1 + j2] as a x b; transpose; rows over j1; W_n^(j2 k1); transpose;
+        // rows over j2; X[k1 + a k2] is the transpose of the a x b result
+        HIPCHK(launch_transpose(prec, in_re, in_im, t1, nullptr, a, b, batch, s));
+        AFCHK(run_c2c(A, A->desc.sign, batch * b, t1, nullptr, t2, nullptr, nullptr, s));
+        HIPCHK(launch_dc_twiddle(prec, t2, a, b, n, P->desc.sign, batch, s));
+        HIPCHK(launch_transpose(prec, t2, nullptr, t1, nullptr, b, a, batch, s));
+        AFCHK(run_c2c(B, B->desc.sign, batch * a, t1, nullptr, t2, nullptr, nullptr, s));
+        HIPCHK(launch_transpose(prec, t2, nullptr, out_re, out_im, a, b, batch, s));
+        return AF_SUCCESS;
+    case DEC_RADER: {
+        // a[q] = x[g^-q]; c = F+(F-(a) * F-(b) / (p - 1)); X[0] = sum x, X[g^r] = x[0] + c[r]
+        const int64_t L = n - 1;
+        HIPCHK(launch_dc_gather(prec, in_re, in_im, n, t1, nullptr, L, P->dec_in, batch, s));
+        AFCHK(run_c2c(A, -1, batch, t1, nullptr, t2, nullptr, nullptr, s));
+        HIPCHK(launch_dc_spec_mul(prec, t2, P->dec_spec, L, batch, s));
+        AFCHK(run_c2c(B, 1, batch, t2, nullptr, t1, nullptr, nullptr, s));
+        HIPCHK(launch_dc_rader_out(prec, in_re, in_im, t1, P->dec_out, n, batch, out_re, out_im, s));
+        return AF_SUCCESS;
+    }
+    }
+    return AF_INTERNAL;
+}
+
+af_status run_c2c(af_plan P, int sign, int64_t batch, const void *in_re, const void *in_im, void *out_re,
                   void *out_im, void *ws, hipStream_t s) {
     const int64_t n = P->shape.n;
     if (n == 0 || batch == 0) return AF_SUCCESS;
+    if (P->shape.algo == ALG_DECOMP)
+        return run_decomp(sign == P->desc.sign ? P : P->dual, batch, in_re, in_im, out_re, out_im, ws, s);
     Scratch sc;
     AFCHK(sc.get(P->scratch_per_row() * (size_t)batch, ws, s));
     std::vector<AfPass> ps;
@@ -1093,6 +1230,11 @@ af_status af_execute_noalloc(af_plan plan, int64_t batch, const void *in_re, con
     if (!plan || plan->desc.kind != AF_C2C || batch < 0) return AF_INVALID_ARG;
     if (batch > 0 && plan->desc.length > 0 && bad_ptrs(in_re, out_re)) return AF_INVALID_ARG;
     if (batch * std::max<int64_t>(plan->desc.length, 1) > ((int64_t)1 << 40)) return AF_INVALID_ARG;
+    if (plan->shape.algo == ALG_DECOMP) {
+        std::snprintf(g_last_error, sizeof g_last_error,
+                      "af_execute_noalloc: decomposition plans run sub-plans that allocate their own scratch");
+        return AF_UNSUPPORTED;
+    }
     const size_t need = plan->scratch_per_row() * (size_t)batch;
     if (need > kNoAllocScratchBytes) {
         std::snprintf(g_last_error, sizeof g_last_error,

@@ -75,6 +75,17 @@ void init_fast_ext_attributes();
 hipError_t launch_fast_rconv(int prec, const AfPass &d, const void *in0, const void *in1, const void *in2, void *out0,
                              void *out1, bool vjp, hipStream_t s, bool &done);
 
+// Decomposition plans (decomp.hip): dst[r m + i] = src[r n + map[i]];
+// W_n^(sign j2 k1) on rows of b x a interleaved elements; x[r L + i] *= spec[i];
+// Rader's output X[0] = sum x, X[pw[i]] = x[0] + c[i] (one row per workgroup).
+hipError_t launch_dc_gather(int prec, const void *src_re, const void *src_im, int64_t n, void *dst_re, void *dst_im,
+                            int64_t m, const int32_t *map, int64_t rows, hipStream_t s);
+hipError_t launch_dc_twiddle(int prec, void *x, int64_t a, int64_t b, int64_t n, int sign, int64_t rows,
+                             hipStream_t s);
+hipError_t launch_dc_spec_mul(int prec, void *x, const void *spec, int64_t L, int64_t rows, hipStream_t s);
+hipError_t launch_dc_rader_out(int prec, const void *x_re, const void *x_im, const void *c, const int32_t *pw,
+                               int64_t p, int64_t rows, void *out_re, void *out_im, hipStream_t s);
+
 // Karatsuba (karatsuba.hip)
 hipError_t launch_karatsuba(const int64_t *x, const int64_t *y, int64_t *z, const int64_t *dev_meta, int64_t npairs,
                             int max_P, int cyclic, hipStream_t s);

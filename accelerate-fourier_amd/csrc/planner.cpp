@@ -211,8 +211,34 @@ bool plan_shape(int prec, int64_t n, int algorithm, PlanShape &out, std::string 
             }
         }
         if (algorithm == AF_ALG_DECOMPOSE) {
-            why = "length needs nested Rader transforms, which this library only serves by Bluestein";
-            return false;
+            // transformDecompose never pads: compose sub-plans on the host
+            out.algo = ALG_DECOMP;
+            if (is_prime(n)) {
+                out.dec = DEC_RADER;
+                out.da = n - 1;
+                return true;
+            }
+            // the largest prime factor q and its multiplicity: Good-Thomas over
+            // (n / q^e, q^e) when that is a coprime split, else Cooley-Tukey (q, n / q)
+            int64_t m = n, q = 1;
+            for (int64_t f = 2; f * f <= m; ++f)
+                while (m % f == 0) {
+                    q = f;
+                    m /= f;
+                }
+            if (m > 1) q = std::max(q, m);
+            int64_t qe = 1;
+            while (n % (qe * q) == 0) qe *= q;
+            if (qe < n) {
+                out.dec = DEC_GOOD_THOMAS;
+                out.da = n / qe;
+                out.db = qe;
+            } else {
+                out.dec = DEC_COOLEY_TUKEY;
+                out.da = q;
+                out.db = n / q;
+            }
+            return true;
         }
     }
     int64_t M;
@@ -277,6 +303,17 @@ std::string describe_shape(const PlanShape &s) {
     case ALG_STOCKHAM: return core_str(s.core);
     case ALG_RADER: return "rader" + std::to_string(s.n) + "(" + core_str(s.core) + ")";
     case ALG_BLUESTEIN: return "bluestein" + std::to_string(s.n) + "(" + core_str(s.core) + ")";
+    case ALG_DECOMP: {
+        // the sub-plans, described the same way (decompositions of the factors)
+        auto sub = [&](int64_t m) {
+            PlanShape t;
+            std::string w;
+            return plan_shape(AF_F64, m, AF_ALG_DECOMPOSE, t, w) ? describe_shape(t) : std::string("?");
+        };
+        if (s.dec == DEC_RADER) return "rader" + std::to_string(s.n) + "(" + sub(s.da) + ")";
+        return std::string(s.dec == DEC_GOOD_THOMAS ? "goodthomas" : "cooleytukey") + std::to_string(s.n) + "(" +
+               sub(s.da) + "," + sub(s.db) + ")";
+    }
     }
     return "?";
 }

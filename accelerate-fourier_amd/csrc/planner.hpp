@@ -60,13 +60,21 @@ enum PlanAlgo {
     ALG_COPY = 1,      // n = 1: identity
     ALG_STOCKHAM = 2,  // smooth n
     ALG_RADER = 3,     // prime n, Rader convolution of length n-1
-    ALG_BLUESTEIN = 4  // chirp-z with padded length M
+    ALG_BLUESTEIN = 4, // chirp-z with padded length M
+    ALG_DECOMP = 5     // transformDecompose of a length with a prime factor no device plan serves
 };
+// The split of a decomposition plan (Planned.hs:255-293 without Bluestein),
+// composed on the host from sub-plans: Good-Thomas for coprime a b (the
+// reference's skew-grid maps, no twiddles), Cooley-Tukey for a prime power
+// n = a b, Rader for a prime whose p - 1 = a needs a decomposition itself.
+enum DecompKind { DEC_GOOD_THOMAS = 1, DEC_COOLEY_TUKEY = 2, DEC_RADER = 3 };
 struct PlanShape {
     int algo = ALG_EMPTY;
     int64_t n = 0;
     int64_t M = 0;     // Bluestein padded length
     CoreShape core;    // the FFT that runs on the device
+    int dec = 0;       // ALG_DECOMP: DecompKind, with its factors a (and b)
+    int64_t da = 0, db = 0;
 };
 // algorithm: af_algorithm (auto, decompose, chirp2, chirp235)
 bool plan_shape(int prec, int64_t n, int algorithm, PlanShape &out, std::string &why, bool allow_three = false);

@@ -57,7 +57,7 @@ typedef enum {
 /* Algorithm selection, mirroring the reference's entry points. */
 typedef enum {
     AF_ALG_AUTO = 0,       /* Planned.transform: decompose, Bluestein when Rader nests (Planned.hs:215-220) */
-    AF_ALG_DECOMPOSE = 1,  /* Planned.transformDecompose: never Bluestein       (Planned.hs:114-119) */
+    AF_ALG_DECOMPOSE = 1,  /* Planned.transformDecompose: never Bluestein; Rader nests (Planned.hs:114-119) */
     AF_ALG_CHIRP2 = 2,     /* Planned.transformChirp2: Bluestein, power-of-two pad (Planned.hs:704-708) */
     AF_ALG_CHIRP235 = 3    /* Planned.transformChirp235: Bluestein, 5-smooth pad   (Planned.hs:716-720) */
 } af_algorithm;

@@ -11,7 +11,7 @@ mkdir -p "$OUT"
 HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
 FLAGS=(--cuda-host-only -O1 -g -std=c++17 -fPIC -I "$REPO/include" -I "$CSRC" -Wno-unused-command-line-argument)
 objs=()
-for f in fft_kernels.hip fast_pass.hip fast_ext.hip karatsuba.hip abi.cpp planner.cpp; do
+for f in fft_kernels.hip fast_pass.hip fast_ext.hip decomp.hip karatsuba.hip abi.cpp planner.cpp; do
     o="$OUT/${f%.*}.o"
     "$HIPCC" "${FLAGS[@]}" -x hip -c "$CSRC/$f" -o "$o" 2> "$OUT/${f%.*}.log" || { cat "$OUT/${f%.*}.log"; exit 1; }
     objs+=("$o")

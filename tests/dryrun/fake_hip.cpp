@@ -443,6 +443,67 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
             }
         };
     }
+    if (name.find("af::k_dc_") == 0) {
+        // decomposition plans: gathers through index maps, twiddles, spectrum
+        // products and Rader's output, each over `rows` rows
+        const int tsz = name.find("<double>") != std::string::npos ? 8 : 4;
+        if (name.find("af::k_dc_gather<") == 0) {
+            const void *sr = *(const void *const *)args[0], *si = *(const void *const *)args[1];
+            const int64_t n = *(const int64_t *)args[2];
+            void *dr = *(void **)args[3], *di = *(void **)args[4];
+            const int64_t m = *(const int64_t *)args[5];
+            const int32_t *map = *(const int32_t *const *)args[6];
+            const int64_t rows = *(const int64_t *)args[7];
+            return [=] {
+                check_c(dr, di, 0, rows * m, tsz, "dst", name);
+                check_c(sr, si, 0, rows * n, tsz, "src", name);
+                if (!in_alloc(map, 0, m * 4, "map", name)) return;
+                for (int64_t i = 0; i < m; ++i)
+                    if (map[i] < 0 || map[i] >= n) {
+                        problem(name + ": map entry outside the source row");
+                        break;
+                    }
+            };
+        }
+        if (name.find("af::k_dc_twiddle<") == 0) {
+            const void *x = *(const void *const *)args[0];
+            const int64_t a = *(const int64_t *)args[1], b = *(const int64_t *)args[2], n = *(const int64_t *)args[3];
+            const int64_t rows = *(const int64_t *)args[5];
+            return [=] {
+                if (a * b != n) problem(name + ": a b differs from n");
+                in_alloc(x, 0, rows * n * 2 * tsz, "x", name);
+            };
+        }
+        if (name.find("af::k_dc_spec_mul<") == 0) {
+            const void *x = *(const void *const *)args[0], *spec = *(const void *const *)args[1];
+            const int64_t L = *(const int64_t *)args[2], rows = *(const int64_t *)args[3];
+            return [=] {
+                in_alloc(x, 0, rows * L * 2 * tsz, "x", name);
+                in_alloc(spec, 0, L * 2 * tsz, "spec", name);
+            };
+        }
+        if (name.find("af::k_dc_rader_out<") == 0) {
+            const void *xr = *(const void *const *)args[0], *xi = *(const void *const *)args[1];
+            const void *c = *(const void *const *)args[2];
+            const int32_t *pw = *(const int32_t *const *)args[3];
+            const int64_t p = *(const int64_t *)args[4];
+            void *orr = *(void **)args[5], *oi = *(void **)args[6];
+            const int64_t rows = grid.x;
+            return [=] {
+                check_c(xr, xi, 0, rows * p, tsz, "x", name);
+                check_c(orr, oi, 0, rows * p, tsz, "out", name);
+                in_alloc(c, 0, rows * (p - 1) * 2 * tsz, "c", name);
+                if (block.x != 256) problem(name + ": the reduction assumes 256 threads");
+                if (!in_alloc(pw, 0, (p - 1) * 4, "pw", name)) return;
+                for (int64_t i = 0; i < p - 1; ++i)
+                    if (pw[i] < 1 || pw[i] >= p) {
+                        problem(name + ": scatter index outside 1 .. p - 1");
+                        break;
+                    }
+            };
+        }
+        return [=] { problem(name + ": no footprint model"); };
+    }
     if (name.find("af::k_kar_split") == 0 || name.find("af::k_kar_join") == 0) {
         // split tasks (x, y, L, cx, cy): x, y hold L words, cx, cy 3 ceil(L/2);
         // join tasks (zc, L, z, cyclic): zc holds 3 (2h - 1), z 2L - 1 (L if cyclic)

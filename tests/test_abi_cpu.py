@@ -134,10 +134,18 @@ def test_host_planner_descriptions():
     assert planned.describe(0) == "c2c.f32:empty"
     assert planned.describe(1) == "c2c.f32:copy1"
     assert planned.describe(8192, kind=_abi.REAL_CONV) == "realconv.f32:stockham4096[16,16,16]"
-    # a prime whose p-1 has a large factor: Rader would nest, so Bluestein, and
-    # transformDecompose refuses it
+    # a prime whose p-1 has a large factor: Rader would nest, so Bluestein; and
+    # transformDecompose nests Rader through host-composed decompositions
+    # (10006 = 2 x 5003: Good-Thomas; 5003 prime: Rader over 5002 = 82 x 61 ...)
     assert planned.describe(10007).startswith("c2c.f32:bluestein10007(")
-    assert "unsupported" in planned.describe(10007, algorithm=_abi.ALG_DECOMPOSE)
+    assert planned.describe(10007, algorithm=_abi.ALG_DECOMPOSE) == (
+        "c2c.f32:rader10007(goodthomas10006(stockham2[2],rader5003(goodthomas5002(goodthomas82(stockham2[2],"
+        "rader41(stockham40[8,5])),rader61(stockham60[4,5,3])))))")
+    assert planned.describe(47, algorithm=_abi.ALG_DECOMPOSE) == (
+        "c2c.f32:rader47(goodthomas46(stockham2[2],rader23(stockham22[2,11])))")
+    assert planned.describe(289, algorithm=_abi.ALG_DECOMPOSE) == (
+        "c2c.f32:cooleytukey289(rader17(stockham16[16]),rader17(stockham16[16]))")
+    assert "bluestein" not in planned.describe(1000003, algorithm=_abi.ALG_DECOMPOSE)
 
 
 def test_bluestein_pads_follow_the_reference(orc):
@@ -197,7 +205,7 @@ def test_planners_and_caches_host(orc):
     algorithm without a device."""
     assert planned.plan(1024) == planned.describe(1024)
     assert planned.plan_with_map_update(10007).startswith("c2c.f32:bluestein10007(")
-    assert "unsupported" in planned.plan_decompose_with_map_update(10007)
+    assert planned.plan_decompose_with_map_update(10007).startswith("c2c.f32:rader10007(goodthomas10006(")
     assert planned.plan_decompose_with_map_update(65521).startswith("c2c.f32:rader65521(")
     for pad, alg in ((planned.ceiling_power_of_two, orc.CHIRP2), (planned.ceiling_5_smooth, orc.CHIRP235)):
         for n in (3, 47, 1000):

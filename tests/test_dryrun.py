@@ -377,3 +377,16 @@ def test_execute_noalloc(dry):
     b.free()
     assert problems(dry, before) == []
 
+
+def test_decompose_plans(dry):
+    """transformDecompose of lengths with prime factors no single device plan
+    serves (nested Rader, Good-Thomas and Cooley-Tukey compositions): every
+    launch of the composition, sub-plans included, in both precisions, both
+    layouts and both signs, with batches of rows."""
+    before = dry.dry_problem_count()
+    for n in (34, 47, 94, 289, 4418, 10007):
+        for prec in (0, 1):
+            for sign, batch, planar in ((-1, 1, False), (1, 3, True)):
+                assert run_c2c(dry, n, sign, prec, batch, planar, alg=1) == 0, (n, prec, sign)
+    assert problems(dry, before) == []
+

@@ -565,11 +565,35 @@ def test_cache_duplex_and_planners(n):
             for pl in (planned.plan_with_map_update, planned.plan_decompose_with_map_update,
                        planned.plan_chirp_with_map_update(planned.ceiling_power_of_two),
                        planned.plan_chirp_with_map_update(planned.ceiling_5_smooth)):
-                if "unsupported" in pl(n):
-                    continue
                 tp = planned.transform_with_planner(pl, Sign(sign), n)
                 got = host(tp(to_dev(x, ct)))
                 assert rel_l2(got, ref) <= (tol + 4e-15 * n if prec == "f64" else 1e-6), pl
+
+
+@pytest.mark.parametrize("n", [34, 47, 94, 289, 1009, 4418, 10007, 100003])
+def test_transform_decompose_nested(n):
+    """Planned.transformDecompose (Planned.hs:114-119) never pads: lengths whose
+    prime factors no single device plan serves run as host-composed
+    decompositions -- Rader whose p - 1 convolution is itself decomposed
+    (47 = Rader(46 = GT(2, Rader(23)))), Good-Thomas over the reference's
+    skew-grid maps (34, 94, 4418), Cooley-Tukey for prime powers (289 = 17^2).
+    Against the oracle's own decomposition plan, both signs, both precisions,
+    both layouts."""
+    import oracle
+    assert "bluestein" not in planned.describe(n, -1, _abi.C2C, _abi.F64, _abi.ALG_DECOMPOSE)
+    x = synthetic.complex_uniform(300 + n, (3, n))
+    for sign in (-1, 1):
+        ref = oracle.transform(sign, x, algorithm=oracle.DECOMPOSE)
+        t = planned.transform_decompose(Sign(sign), n)
+        got = host(t(to_dev(x, torch.complex128)))
+        assert rel_l2(got, ref) <= 1e-12, (n, sign, "f64")
+        y = t((to_dev(x.real, torch.float32), to_dev(x.imag, torch.float32)))
+        got32 = host(y[0]) + 1j * host(y[1])
+        assert rel_l2(got32, ref) <= 1e-6, (n, sign, "f32")
+    # the adjoint runs the plan of the opposite sign (its own decomposition)
+    g = synthetic.complex_uniform(400 + n, (2, n))
+    vj = host(planned.transform_decompose(forward, n).vjp(to_dev(g, torch.complex128)))
+    assert rel_l2(vj, oracle.transform(1, g, algorithm=oracle.DECOMPOSE)) <= 1e-12
 
 
 @pytest.mark.parametrize("L,m", [(1, 2 ** 15), (2, 2 ** 15), (65, 2 ** 15), (777, 2 ** 62), (4096, 2 ** 15)])
