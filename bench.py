@@ -55,8 +55,8 @@ def parse():
                     help="only the 512 x 8192 FP32 real convolution and its adjoint (warmup + steps "
                          "calls), for rocprofv3 kernel traces")
     ap.add_argument("--profile-c2b", action="store_true",
-                    help="only one 2^24 FP32 transform (warmup + steps runs) and a 128 MiB device copy of "
-                         "known size, for rocprofv3 kernel traces and counters")
+                    help="only one 2^24 FP32 transform, autotuned unless --no-tune (warmup + steps runs), "
+                         "and a 128 MiB device copy of known size, for rocprofv3 kernel traces and counters")
     ap.add_argument("--profile-c1", action="store_true",
                     help="only the FP64 256 x 1024 transform, autotuned (warmup + steps runs), for rocprofv3 "
                          "kernel traces")
@@ -357,8 +357,7 @@ def main():
         else:
             g = C2C(1024, 256, _abi.F64)
             g.set_io(synthetic.complex_uniform(0, (256, 1024)))
-            if not args.no_tune:
-                g.tune()
+        report = "" if args.no_tune else g.tune()
         for _ in range(warmup + steps):
             g.run()
         if args.profile_c2b:
@@ -368,7 +367,8 @@ def main():
                 dst.copy_(src)  # reference kernel: reads and writes exactly 128 MiB
         torch.cuda.synchronize()
         print(json.dumps({"profile": "done", "workload": "c2b 2^24 f32" if args.profile_c2b else "c1 256x1024 f64",
-                          "transform_launches": g.npasses * (warmup + steps)}))
+                          "transform_launches": g.npasses * (warmup + steps),
+                          "autotune": report.strip().split("\n") if report else None}))
         return
     n, batch = 65536, 64
     flops = 5.0 * n * math.log2(n) * batch
@@ -461,6 +461,8 @@ def main():
         x24 = synthetic.complex_normal(1, (1, n24))
         g = C2C(n24, 1, _abi.F32)
         g.set_io(x24)
+        if not args.no_tune:
+            extras["c2b_autotune"] = g.tune().strip().split("\n")
         ms24 = time_it(g.run, max(steps // 2, 3), 2)
         b24 = 16.0 * n24
         extras["c2b_2p24_f32_ms"] = ms24
