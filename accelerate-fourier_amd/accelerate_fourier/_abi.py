@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_PKG_ROOT, "libaccelerate_fourier_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_PKG_ROOT), "include", "accelerate_fourier.h")
 
 # af_status
-SUCCESS, INVALID_ARG, UNSUPPORTED, NO_DEVICE, HIP_ERROR, OUT_OF_MEMORY, INTERNAL, TIMEOUT = range(8)
+SUCCESS, INVALID_ARG, UNSUPPORTED, NO_DEVICE, HIP_ERROR, OUT_OF_MEMORY, INTERNAL = range(7)
 # af_precision
 F64, F32 = 0, 1
 # af_kind

@@ -755,7 +755,6 @@ const char *af_status_string(af_status s) {
     case AF_HIP_ERROR: return "HIP runtime error";
     case AF_OUT_OF_MEMORY: return "out of device memory";
     case AF_INTERNAL: return "internal error";
-    case AF_TIMEOUT: return "device wait timed out";
     }
     return "unknown status";
 }

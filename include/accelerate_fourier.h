@@ -38,8 +38,7 @@ typedef enum {
     AF_NO_DEVICE = 3,     /* no gfx950 device is visible */
     AF_HIP_ERROR = 4,     /* a HIP runtime call failed */
     AF_OUT_OF_MEMORY = 5,
-    AF_INTERNAL = 6,
-    AF_TIMEOUT = 7        /* a bounded device-side wait gave up (persistent batch) */
+    AF_INTERNAL = 6
 } af_status;
 
 typedef enum { AF_F64 = 0, AF_F32 = 1 } af_precision;

@@ -210,6 +210,27 @@ def test_real_spectrum(n):
     assert rel(O.from_spectrum(O.to_spectrum(x)) / n, x) < 1e-13                # Fourier.hs:478-486
 
 
+@pytest.mark.parametrize("n", [2, 4, 6, 10, 64, 100, 1024])
+def test_real_packing_f32(n):
+    """The FP32 instantiations of toSpectrum / fromSpectrum (Real.hs:49-107) and
+    untangleSpectra / entangleSpectra (Real.hs:168-236) against the same closed
+    forms as the FP64 ones, at FP32 tolerances: numpy's FP64 spectrum, n x for
+    the inverse, (z + conj z_-k) / 2 and (z - conj z_-k)(-i/2) on arbitrary
+    input, and a + i b for the entangled pair."""
+    rng = np.random.default_rng(100 + n)
+    x = rng.standard_normal((3, n)).astype(np.float32)
+    S = np.fft.fft(x.astype(np.float64))
+    assert rel(O.to_spectrum(x, precision="f32"), S) < 2e-6
+    assert rel(O.from_spectrum(S, precision="f32"), n * x) < 2e-6
+    z = (rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))).astype(np.complex64)
+    ua, ub = O.untangle_spectra(z, precision="f32")
+    zr = np.conj(z[:, (-np.arange(n)) % n]).astype(np.complex128)
+    assert rel(ua, (z + zr) / 2) < 1e-6
+    assert rel(ub, (z - zr) * (-0.5j)) < 1e-6
+    a, b = S[:2], np.fft.fft(rng.standard_normal((2, n)))
+    assert rel(O.entangle_spectra(a, b, precision="f32"), a + 1j * b) < 1e-6
+
+
 def test_convolve_cyclic_numpy():
     for n in (1, 2, 7, 16, 47, 100):
         x, y = rng_c(n, (2, n)), rng_c(n + 9, (2, n))
