@@ -3,8 +3,9 @@
     python accelerate-fourier_amd/build.py
 
 compiles csrc/*.hip and csrc/*.cpp with hipcc for gfx950 (objects in
-csrc/_build/) and links accelerate-fourier_amd/libaccelerate_fourier_amd.so.
-Every call recompiles.
+csrc/_build/), links accelerate-fourier_amd/libaccelerate_fourier_amd.so and
+then builds tests/c_binding/replay (the C replay of the reference-side
+binding) against it.  Every call recompiles.
 """
 from __future__ import annotations
 
@@ -47,6 +48,27 @@ def _compile(src: str) -> str:
     return obj
 
 
+REPLAY_SRC = os.path.join(REPO, "tests", "c_binding", "replay.c")
+REPLAY_BIN = os.path.join(REPO, "tests", "c_binding", "replay")
+
+
+def build_replay(verbose: bool = True) -> str:
+    """The C program that replays the reference-side binding's call sequence
+    against the library (a GPU test runs it): plain C on the HIP runtime's C
+    API, built with gcc, linked to the library by relative rpath."""
+    cmd = ["gcc", "-std=c11", "-O2", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include",
+           "-I", os.path.join(REPO, "include"), REPLAY_SRC, "-o", REPLAY_BIN + ".tmp", "-L", HERE,
+           "-laccelerate_fourier_amd", "-L", "/opt/rocm/lib", "-lamdhip64", "-lm",
+           "-Wl,-rpath,$ORIGIN/../../accelerate-fourier_amd", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"gcc failed on replay.c:\n{r.stdout}\n{r.stderr}")
+    os.replace(REPLAY_BIN + ".tmp", REPLAY_BIN)
+    if verbose:
+        print(f"built {REPLAY_BIN}")
+    return REPLAY_BIN
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
@@ -59,6 +81,7 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_replay(verbose)
     return LIB
 
 
