@@ -2,8 +2,8 @@
 
 ``UnevenBatch(lengths, sign, precision)`` applies ``Planned.transform sign
 lengths[i]`` (Planned.hs:103-106) to each of ``len(lengths)`` rows packed back
-to back in one flat complex array.  All rows run as grouped launches of the
-generic pass kernel captured in one hipGraph.
+to back in one flat complex array.  All rows run as grouped launches captured
+in one hipGraph, rows with intermediates in cache-sized chunks.
 """
 from __future__ import annotations
 
@@ -38,10 +38,25 @@ class UnevenBatch:
             self.handle = None
 
     def describe(self) -> str:
-        """Plan counts per algorithm on the first line, then one line per
-        grouped launch: its kernel key, row count and first row."""
-        buf = ctypes.create_string_buffer(1 << 16)
+        """Plan counts per algorithm, launches, chunks and scratch on the first
+        line, then one line per grouped launch: its chunk, phase, kernel key,
+        row count and first row."""
+        buf = ctypes.create_string_buffer(1 << 20)
         _abi.check(_abi.lib().af_batch_describe(self.handle, buf, len(buf)), "af_batch_describe")
+        return buf.value.decode()
+
+    def set_chunk_bytes(self, chunk_bytes: int) -> None:
+        """af_batch_set_chunk_bytes: rows with intermediates in chunks of at
+        most chunk_bytes of them (0: no chunking)."""
+        if int(chunk_bytes) < 0:
+            raise ValueError("chunk_bytes must be >= 0")
+        _abi.check(_abi.lib().af_batch_set_chunk_bytes(self.handle, int(chunk_bytes)), "af_batch_set_chunk_bytes")
+
+    def tune(self) -> str:
+        """af_batch_tune: times chunk sizes of the batch's graph on the current
+        stream and keeps the fastest; returns the report."""
+        buf = ctypes.create_string_buffer(8192)
+        _abi.check(_abi.lib().af_batch_tune(self.handle, _arrays.stream_handle(), buf, len(buf)), "af_batch_tune")
         return buf.value.decode()
 
     def __call__(self, x, out=None):

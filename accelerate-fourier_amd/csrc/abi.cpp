@@ -2033,11 +2033,20 @@ struct af_batch_s {
     int prec = PREC_F32;
     std::vector<int64_t> lengths, offsets;
     std::map<int64_t, af_plan> plans;
+    // every row's passes built with placeholder pointers: the launch shapes
+    std::vector<std::vector<AfPass>> shapes;
+    std::vector<size_t> scratch_need;  // bytes of intermediates per row (256-aligned)
     std::vector<size_t> scratch_off;
-    Owner own;
+    // Rows with intermediates run in chunks of at most chunk_bytes of them
+    // (0: one chunk): all passes of one chunk, then the next chunk, every chunk
+    // on the same scratch region, so that a chunk's intermediates can stay in
+    // the memory-side cache between its passes.
+    int64_t chunk_bytes = 0;
+    int nchunks = 0;
+    DevBuf scratch_buf, descs_buf, prefix_buf;
     void *scratch = nullptr;
     struct Group {
-        int phase = 0, nthreads = 0, lds = 0;
+        int phase = 0, nthreads = 0, lds = 0, chunk = 0;
         bool full = false;  // some descriptor sets a flag of the full pass kernel
         // power-of-two passes of compile-time shape (L, B, conv, sign) run on the
         // extended fast kernel instead of the generic one
@@ -2047,7 +2056,6 @@ struct af_batch_s {
         int64_t desc_off = 0, prefix_off = 0, total_tiles = 0;
     };
     std::vector<Group> groups;
-    std::vector<int> npasses;
     AfPass *d_descs = nullptr;
     int64_t *d_prefix = nullptr;
     std::vector<AfPass> h_descs;
@@ -2056,7 +2064,7 @@ struct af_batch_s {
     hipGraphExec_t exec = nullptr;
     const void *last[4] = {nullptr, nullptr, nullptr, nullptr};
     bool uploaded = false;
-    std::string text;
+    std::string text, summary;
     // Calls share the descriptors and the scratch intermediates: the mutex
     // serialises their host side, and `done` (recorded after each graph launch)
     // orders every launch after the previous one, whatever its stream, and keeps
@@ -2064,10 +2072,15 @@ struct af_batch_s {
     std::mutex mu;
     hipEvent_t done = nullptr;
     bool recorded = false;
-    ~af_batch_s() {
-        if (done) (void)hipEventDestroy(done);
+    void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~af_batch_s() {
+        if (done) (void)hipEventDestroy(done);
+        drop_graph();
         for (auto &kv : plans) af_plan_destroy(kv.second);
     }
 };
@@ -2087,16 +2100,213 @@ Endpoint offset_ep(const void *re, const void *im, int64_t off, int64_t n, size_
     return e;
 }
 
+// The passes of row i on the given buffers; its intermediates at
+// scratch_base + scratch_off[i].
 af_status batch_passes(af_batch B, int64_t i, const void *in_re, const void *in_im, void *out_re, void *out_im,
-                       std::vector<AfPass> &ps) {
+                       char *scratch_base, std::vector<AfPass> &ps) {
     const int64_t n = B->lengths[i];
     if (n == 0) return AF_SUCCESS;
     af_plan P = B->plans.at(n);
     const size_t rb = B->precision == AF_F64 ? 8 : 4;
     return build_c2c(P, B->sign, 1, offset_ep(in_re, in_im, B->offsets[i], n, rb),
-                     offset_ep(out_re, out_im, B->offsets[i], n, rb), (char *)B->scratch + B->scratch_off[i], ps,
-                     false);
+                     offset_ep(out_re, out_im, B->offsets[i], n, rb), scratch_base + B->scratch_off[i], ps, false);
 }
+
+// (Re)builds the launch groups, the scratch region and the captured graph for
+// a chunk budget.  Rows without intermediates form one chunk, launched first.
+// Rows with intermediates are ordered by pass count, intermediate size and
+// length (rows of one shape share grouped launches) and cut into chunks of at
+// most `chunk_bytes` of intermediates (a row larger than the budget is a chunk
+// of its own); every chunk's intermediates start at offset 0 of one region.
+// Within a chunk, the k-th passes of its rows are grouped by kernel, as the
+// whole batch is when chunk_bytes is 0.
+af_status batch_build(af_batch B, int64_t chunk_bytes) {
+    B->drop_graph();
+    B->groups.clear();
+    B->h_descs.clear();
+    B->h_prefix.clear();
+    B->uploaded = false;
+    B->chunk_bytes = chunk_bytes;
+    const int64_t count = B->count;
+    std::vector<int64_t> light, heavy;
+    for (int64_t i = 0; i < count; ++i) {
+        if (B->shapes[i].empty()) continue;
+        (B->scratch_need[i] ? heavy : light).push_back(i);
+    }
+    std::stable_sort(heavy.begin(), heavy.end(), [&](int64_t a, int64_t b) {
+        const size_t pa = B->shapes[a].size(), pb = B->shapes[b].size();
+        if (pa != pb) return pa < pb;
+        if (B->scratch_need[a] != B->scratch_need[b]) return B->scratch_need[a] < B->scratch_need[b];
+        return B->lengths[a] < B->lengths[b];
+    });
+    std::vector<std::vector<int64_t>> chunks;
+    if (!light.empty()) chunks.push_back(light);
+    size_t region = 0, used = 0;
+    for (int64_t i : heavy) {
+        const size_t need = B->scratch_need[i];
+        const bool fresh = chunks.empty() || chunks.back().empty() || B->scratch_need[chunks.back()[0]] == 0 ||
+                           (chunk_bytes > 0 && used + need > (size_t)chunk_bytes);
+        if (fresh) {
+            chunks.emplace_back();
+            used = 0;
+        }
+        B->scratch_off[i] = used;
+        used += need;
+        region = std::max(region, used);
+        chunks.back().push_back(i);
+    }
+    for (int64_t i : light) B->scratch_off[i] = 0;
+    B->nchunks = (int)chunks.size();
+    // scratch: one region for every chunk
+    if (B->scratch_buf.p) {
+        (void)hipFree(B->scratch_buf.p);
+        B->scratch_buf.p = nullptr;
+    }
+    HIPCHK(hipMalloc(&B->scratch_buf.p, std::max<size_t>(region, 256)));
+    B->scratch = B->scratch_buf.p;
+    const int classes[] = {64, 128, 256, 512, 1024};
+    int64_t ndesc = 0, nprefix = 0;
+    for (size_t c = 0; c < chunks.size(); ++c) {
+        int maxp = 0;
+        for (int64_t i : chunks[c]) maxp = std::max(maxp, (int)B->shapes[i].size());
+        for (int ph = 0; ph < maxp; ++ph) {
+            // groups of this phase: generic thread classes (key 0, class) and
+            // extended fast shapes (key 1, L, B, conv, sign)
+            std::map<std::array<int, 5>, af_batch_s::Group> gm;
+            for (int64_t i : chunks[c]) {
+                if ((int)B->shapes[i].size() <= ph) continue;
+                const AfPass &d = B->shapes[i][ph];
+                std::array<int, 5> key;
+                if (fast_ext_eligible(B->prec, d)) {
+                    key = {1, d.L, d.B, (d.flags & AFP_CONV) ? 1 : 0, d.sign};
+                } else {
+                    const int cls = d.nthreads <= 64 ? 0 : d.nthreads <= 128 ? 1 : d.nthreads <= 256 ? 2
+                                    : d.nthreads <= 512 ? 3 : 4;
+                    key = {0, cls, 0, 0, 0};
+                }
+                af_batch_s::Group &g = gm[key];
+                if (g.rows.empty()) {
+                    g.phase = ph;
+                    g.chunk = (int)c;
+                    if (key[0] == 1) {
+                        g.ext = true;
+                        g.L = d.L;
+                        g.B = d.B;
+                        g.conv = key[3] != 0;
+                        g.sign = d.sign;
+                    } else {
+                        g.nthreads = classes[key[1]];
+                    }
+                }
+                g.rows.push_back(i);
+                g.lds = std::max(g.lds, d.lds_bytes);
+                if (d.flags & kRarePassFlags) g.full = true;
+            }
+            for (auto &kv : gm) {
+                af_batch_s::Group g = kv.second;
+                g.desc_off = ndesc;
+                g.prefix_off = nprefix;
+                int64_t t = 0;
+                for (int64_t i : g.rows) {
+                    B->h_prefix.push_back(t);
+                    t += B->shapes[i][ph].ntiles;
+                }
+                B->h_prefix.push_back(t);
+                g.total_tiles = t;
+                ndesc += (int64_t)g.rows.size();
+                nprefix += (int64_t)g.rows.size() + 1;
+                B->groups.push_back(g);
+            }
+        }
+    }
+    B->h_descs.resize(ndesc);
+    for (DevBuf *b : {&B->descs_buf, &B->prefix_buf})
+        if (b->p) {
+            (void)hipFree(b->p);
+            b->p = nullptr;
+        }
+    HIPCHK(hipMalloc(&B->descs_buf.p, std::max<int64_t>(ndesc, 1) * sizeof(AfPass)));
+    HIPCHK(hipMalloc(&B->prefix_buf.p, std::max<int64_t>(nprefix, 1) * 8));
+    B->d_descs = (AfPass *)B->descs_buf.p;
+    B->d_prefix = (int64_t *)B->prefix_buf.p;
+    if (nprefix) HIPCHK(hipMemcpy(B->d_prefix, B->h_prefix.data(), nprefix * 8, hipMemcpyHostToDevice));
+    // capture the grouped launches once
+    if (!B->groups.empty()) {
+        hipStream_t s = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        af_status st = AF_SUCCESS;
+        { const af_status h__ = HIPST(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)); if (st == AF_SUCCESS) st = h__; }
+        for (const auto &g : B->groups) {
+            if (st != AF_SUCCESS) break;
+            if (g.ext)
+                st = HIPST(launch_fast_ext_grouped(B->prec, g.L, g.B, g.conv, g.sign, B->d_descs + g.desc_off,
+                                                   B->d_prefix + g.prefix_off, (int)g.rows.size(), g.total_tiles, s));
+            else
+                st = HIPST(launch_pass_grouped(B->prec, B->d_descs + g.desc_off, B->d_prefix + g.prefix_off,
+                                               (int)g.rows.size(), g.total_tiles, g.nthreads, g.lds, g.full, s));
+        }
+        hipGraph_t graph = nullptr;
+        { const af_status h__ = HIPST(hipStreamEndCapture(s, &graph)); if (st == AF_SUCCESS) st = h__; }
+        (void)hipStreamDestroy(s);
+        if (st != AF_SUCCESS) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        B->graph = graph;
+        HIPCHK(hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0));
+    }
+    // description: algorithm counts, chunks, one line per grouped launch
+    B->text = B->summary + " launches=" + std::to_string(B->groups.size()) + " chunks=" +
+              std::to_string(B->nchunks) + " chunk_bytes=" + std::to_string((long long)chunk_bytes) +
+              " scratch_bytes=" + std::to_string((unsigned long long)region);
+    for (size_t k = 0; k < B->groups.size(); ++k) {
+        const auto &g = B->groups[k];
+        char line[192];
+        if (g.ext)
+            std::snprintf(line, sizeof line, "\nlaunch %zu chunk %d phase %d ext L=%d B=%d conv=%d sign=%d rows=%zu first=%lld",
+                          k, g.chunk, g.phase, g.L, g.B, g.conv ? 1 : 0, g.sign, g.rows.size(), (long long)g.rows[0]);
+        else
+            std::snprintf(line, sizeof line, "\nlaunch %zu chunk %d phase %d generic threads=%d full=%d rows=%zu first=%lld",
+                          k, g.chunk, g.phase, g.nthreads, g.full ? 1 : 0, g.rows.size(), (long long)g.rows[0]);
+        B->text += line;
+    }
+    return AF_SUCCESS;
+}
+
+// The descriptors of every grouped launch for the given buffers, uploaded on s
+// (the caller holds B->mu and has waited for the previous launch).
+af_status batch_upload(af_batch B, const void *in_re, const void *in_im, void *out_re, void *out_im, hipStream_t s) {
+    std::vector<AfPass> ps;
+    for (const auto &g : B->groups) {
+        int64_t j = g.desc_off;
+        for (int64_t i : g.rows) {
+            ps.clear();
+            AFCHK(batch_passes(B, i, in_re, in_im, out_re, out_im, (char *)B->scratch, ps));
+            const AfPass &d = ps[g.phase];
+            // the captured launch fixed this descriptor's tiles and shape
+            const int64_t k = j - g.desc_off;
+            if (d.ntiles != B->h_prefix[g.prefix_off + k + 1] - B->h_prefix[g.prefix_off + k] ||
+                (g.ext && (d.L != g.L || d.B != g.B)) || (!g.ext && (d.nthreads > g.nthreads || d.lds_bytes > g.lds)))
+                return AF_INTERNAL;
+            B->h_descs[j++] = d;
+        }
+    }
+    if (!B->h_descs.empty())
+        HIPCHK(hipMemcpyAsync(B->d_descs, B->h_descs.data(), B->h_descs.size() * sizeof(AfPass), hipMemcpyHostToDevice,
+                              s));
+    HIPCHK(hipStreamSynchronize(s));
+    return AF_SUCCESS;
+}
+
+// Default chunk budget: none (one chunk).  On MI355X the 4096-row batch of four
+// length classes ran in 10.6 ms unchunked against 11.0 ms with chunks of 128
+// or 384 MiB of intermediates (half and one and a half times the 256 MB
+// memory-side cache) and 13.3 ms with 32 MiB chunks: the passes did not gain
+// from re-reading cache-resident intermediates as much as the extra launches
+// and their partly filled last rounds cost.  af_batch_tune measures it again
+// for each batch; chunking also bounds the scratch memory (5.7 GB unchunked,
+// the chunk budget chunked).
+constexpr int64_t kBatchChunkBytes = 0;
 
 }  // namespace
 
@@ -2119,11 +2329,10 @@ af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign, i
     B->lengths.assign(lengths, lengths + count);
     B->offsets.resize(count + 1, 0);
     for (int64_t i = 0; i < count; ++i) B->offsets[i + 1] = B->offsets[i] + lengths[i];
-    size_t scr = 0;
-    B->scratch_off.resize(count);
+    B->scratch_need.assign(count, 0);
+    B->scratch_off.assign(count, 0);
     for (int64_t i = 0; i < count; ++i) {
         const int64_t n = lengths[i];
-        B->scratch_off[i] = scr;
         if (n == 0) continue;
         if (!B->plans.count(n)) {
             af_plan_desc d{};
@@ -2136,122 +2345,101 @@ af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign, i
             AFCHK(plan_create_impl(&d, &p, false));
             B->plans[n] = p;
         }
-        scr += (B->plans[n]->scratch_per_row() + 255) / 256 * 256;
+        B->scratch_need[i] = (B->plans[n]->scratch_per_row() + 255) / 256 * 256;
     }
-    AFCHK(B->own.alloc(&B->scratch, scr));
     // pass structure with placeholder pointers
-    std::vector<std::vector<AfPass>> per(count);
-    int maxp = 0;
-    B->npasses.resize(count);
+    B->shapes.resize(count);
     const char *fake = (const char *)0x1000;
-    for (int64_t i = 0; i < count; ++i) {
-        AFCHK(batch_passes(B.get(), i, fake, nullptr, (void *)fake, nullptr, per[i]));
-        B->npasses[i] = (int)per[i].size();
-        maxp = std::max(maxp, B->npasses[i]);
-    }
-    const int classes[] = {64, 128, 256, 512, 1024};
-    int64_t ndesc = 0, nprefix = 0;
-    for (int ph = 0; ph < maxp; ++ph) {
-        // groups of this phase: generic thread classes (key 0, class) and
-        // extended fast shapes (key 1, L, B, conv, sign)
-        std::map<std::array<int, 5>, af_batch_s::Group> gm;
-        for (int64_t i = 0; i < count; ++i) {
-            if (B->npasses[i] <= ph) continue;
-            const AfPass &d = per[i][ph];
-            std::array<int, 5> key;
-            if (fast_ext_eligible(B->prec, d)) {
-                key = {1, d.L, d.B, (d.flags & AFP_CONV) ? 1 : 0, d.sign};
-            } else {
-                const int cls = d.nthreads <= 64 ? 0 : d.nthreads <= 128 ? 1 : d.nthreads <= 256 ? 2
-                                : d.nthreads <= 512 ? 3 : 4;
-                key = {0, cls, 0, 0, 0};
-            }
-            af_batch_s::Group &g = gm[key];
-            if (g.rows.empty()) {
-                g.phase = ph;
-                if (key[0] == 1) {
-                    g.ext = true;
-                    g.L = d.L;
-                    g.B = d.B;
-                    g.conv = key[3] != 0;
-                    g.sign = d.sign;
-                } else {
-                    g.nthreads = classes[key[1]];
-                }
-            }
-            g.rows.push_back(i);
-            g.lds = std::max(g.lds, d.lds_bytes);
-            if (d.flags & kRarePassFlags) g.full = true;
-        }
-        for (auto &kv : gm) {
-            af_batch_s::Group g = kv.second;
-            g.desc_off = ndesc;
-            g.prefix_off = nprefix;
-            int64_t t = 0;
-            for (int64_t i : g.rows) {
-                B->h_prefix.push_back(t);
-                t += per[i][ph].ntiles;
-            }
-            B->h_prefix.push_back(t);
-            g.total_tiles = t;
-            ndesc += (int64_t)g.rows.size();
-            nprefix += (int64_t)g.rows.size() + 1;
-            B->groups.push_back(g);
-        }
-    }
-    B->h_descs.resize(ndesc);
-    AFCHK(B->own.alloc((void **)&B->d_descs, std::max<int64_t>(ndesc, 1) * sizeof(AfPass)));
-    AFCHK(B->own.alloc((void **)&B->d_prefix, std::max<int64_t>(nprefix, 1) * 8));
-    if (nprefix) HIPCHK(hipMemcpy(B->d_prefix, B->h_prefix.data(), nprefix * 8, hipMemcpyHostToDevice));
-    // capture the grouped launches once
-    hipStream_t s = nullptr;
-    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    af_status st = AF_SUCCESS;
-    { const af_status h__ = HIPST(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)); if (st == AF_SUCCESS) st = h__; }
-    for (const auto &g : B->groups) {
-        if (st != AF_SUCCESS) break;
-        if (g.ext)
-            st = HIPST(launch_fast_ext_grouped(B->prec, g.L, g.B, g.conv, g.sign, B->d_descs + g.desc_off,
-                                               B->d_prefix + g.prefix_off, (int)g.rows.size(), g.total_tiles, s));
-        else
-            st = HIPST(launch_pass_grouped(B->prec, B->d_descs + g.desc_off, B->d_prefix + g.prefix_off,
-                                           (int)g.rows.size(), g.total_tiles, g.nthreads, g.lds, g.full, s));
-    }
-    hipGraph_t graph = nullptr;
-    { const af_status h__ = HIPST(hipStreamEndCapture(s, &graph)); if (st == AF_SUCCESS) st = h__; }
-    (void)hipStreamDestroy(s);
-    if (st != AF_SUCCESS) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return st;
-    }
-    B->graph = graph;
-    HIPCHK(hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0));
-    // description: algorithm counts
+    for (int64_t i = 0; i < count; ++i)
+        AFCHK(batch_passes(B.get(), i, fake, nullptr, (void *)fake, nullptr, (char *)0x100000, B->shapes[i]));
+    // algorithm counts for the description
     std::map<std::string, int64_t> counts;
     for (int64_t i = 0; i < count; ++i) {
         if (!lengths[i]) continue;
         const int a = B->plans[lengths[i]]->shape.algo;
         const int k = B->plans[lengths[i]]->core.sh.kind;
-        std::string key = a == ALG_RADER ? "rader" : a == ALG_BLUESTEIN ? "bluestein" : a == ALG_COPY ? "copy" : "stockham";
+        std::string key = a == ALG_RADER ? "rader" : a == ALG_BLUESTEIN ? "bluestein" : a == ALG_COPY ? "copy"
+                          : a == ALG_DECOMP ? "decompose" : "stockham";
         key += k == CORE_FOUR ? "/fourstep" : k == CORE_THREE ? "/threepass" : "/single";
         counts[key]++;
     }
-    B->text = "batch" + std::to_string(count) + " launches=" + std::to_string(B->groups.size());
-    for (auto &kv : counts) B->text += " " + kv.first + "=" + std::to_string(kv.second);
-    // one line per grouped launch: its kernel key, row count and first row
-    for (size_t k = 0; k < B->groups.size(); ++k) {
-        const auto &g = B->groups[k];
-        char line[160];
-        if (g.ext)
-            std::snprintf(line, sizeof line, "\nlaunch %zu phase %d ext L=%d B=%d conv=%d sign=%d rows=%zu first=%lld", k,
-                          g.phase, g.L, g.B, g.conv ? 1 : 0, g.sign, g.rows.size(), (long long)g.rows[0]);
-        else
-            std::snprintf(line, sizeof line, "\nlaunch %zu phase %d generic threads=%d full=%d rows=%zu first=%lld", k,
-                          g.phase, g.nthreads, g.full ? 1 : 0, g.rows.size(), (long long)g.rows[0]);
-        B->text += line;
-    }
+    B->summary = "batch" + std::to_string(count);
+    for (auto &kv : counts) B->summary += " " + kv.first + "=" + std::to_string(kv.second);
+    AFCHK(batch_build(B.get(), kBatchChunkBytes));
     *batch = B.release();
     return AF_SUCCESS;
+}
+
+af_status af_batch_set_chunk_bytes(af_batch B, int64_t chunk_bytes) {
+    if (!B || chunk_bytes < 0) return AF_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(B->mu);
+    if (B->recorded) HIPCHK(hipEventSynchronize(B->done));
+    B->recorded = false;
+    return batch_build(B, chunk_bytes);
+}
+
+af_status af_batch_tune(af_batch B, void *stream, char *report, size_t report_len) {
+    if (!B) return AF_INVALID_ARG;
+    if (report && report_len) report[0] = 0;
+    hipStream_t s = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(B->mu);
+    if (B->recorded) HIPCHK(hipEventSynchronize(B->done));
+    const int64_t total = B->offsets.back();
+    if (total == 0) return AF_SUCCESS;
+    // zero-filled interleaved buffers: the timing does not depend on the values
+    const size_t bytes = (size_t)total * (B->precision == AF_F64 ? 16 : 8);
+    DevBuf in, out;
+    HIPCHK(hipMalloc(&in.p, bytes));
+    HIPCHK(hipMalloc(&out.p, bytes));
+    HIPCHK(hipMemsetAsync(in.p, 0, bytes, s));
+    HIPCHK(hipMemsetAsync(out.p, 0, bytes, s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int64_t start = B->chunk_bytes;
+    const int64_t candidates[] = {0, (int64_t)32 << 20, (int64_t)64 << 20, (int64_t)128 << 20, (int64_t)192 << 20,
+                                  (int64_t)384 << 20};
+    std::string text;
+    int64_t best = start;
+    double tbest = 1e30;
+    af_status st = AF_SUCCESS;
+    // each budget: one warm-up, then the best of three launches; the incumbent
+    // must be beaten by 3%
+    for (int64_t cb : candidates) {
+        st = batch_build(B, cb);
+        if (st == AF_SUCCESS) st = batch_upload(B, in.p, nullptr, out.p, nullptr, s);
+        double t = 1e30;
+        for (int r = 0; r < 4 && st == AF_SUCCESS; ++r) {
+            st = HIPST(hipEventRecord(e0, s));
+            if (st == AF_SUCCESS) st = HIPST(hipGraphLaunch(B->exec, s));
+            if (st == AF_SUCCESS) st = HIPST(hipEventRecord(e1, s));
+            if (st == AF_SUCCESS) st = HIPST(hipEventSynchronize(e1));
+            float ms = 0.f;
+            if (st == AF_SUCCESS) st = HIPST(hipEventElapsedTime(&ms, e0, e1));
+            if (r > 0) t = std::min(t, (double)ms);
+        }
+        if (st != AF_SUCCESS) break;
+        char line[160];
+        std::snprintf(line, sizeof line, "chunk_bytes=%lld chunks=%d launches=%zu %.3f ms\n", (long long)cb, B->nchunks,
+                      B->groups.size(), t);
+        text += line;
+        if (t < tbest * 0.97) {
+            tbest = t;
+            best = cb;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    { const af_status h__ = HIPST(hipStreamSynchronize(s)); if (st == AF_SUCCESS) st = h__; }
+    // the chosen budget (or, after a failure, the one the batch had)
+    const af_status st2 = batch_build(B, st == AF_SUCCESS ? best : start);
+    B->recorded = false;
+    if (st == AF_SUCCESS) st = st2;
+    if (st == AF_SUCCESS && report && report_len) {
+        text += "chosen chunk_bytes=" + std::to_string((long long)best) + "\n";
+        std::snprintf(report, report_len, "%s", text.c_str());
+    }
+    return st;
 }
 
 af_status af_batch_destroy(af_batch batch) {
@@ -2278,25 +2466,7 @@ af_status af_batch_execute(af_batch B, const void *in_re, const void *in_im, voi
     if (!B->uploaded || std::memcmp(cur, B->last, sizeof cur) != 0) {
         // the previous launch may still read the descriptors about to change
         if (B->recorded) HIPCHK(hipEventSynchronize(B->done));
-        std::vector<AfPass> ps;
-        for (const auto &g : B->groups) {
-            int64_t j = g.desc_off;
-            for (int64_t i : g.rows) {
-                ps.clear();
-                AFCHK(batch_passes(B, i, in_re, in_im, out_re, out_im, ps));
-                const AfPass &d = ps[g.phase];
-                // the captured launch fixed this descriptor's tiles and shape
-                const int64_t k = j - g.desc_off;
-                if (d.ntiles != B->h_prefix[g.prefix_off + k + 1] - B->h_prefix[g.prefix_off + k] ||
-                    (g.ext && (d.L != g.L || d.B != g.B)) || (!g.ext && (d.nthreads > g.nthreads || d.lds_bytes > g.lds)))
-                    return AF_INTERNAL;
-                B->h_descs[j++] = d;
-            }
-        }
-        if (!B->h_descs.empty())
-            HIPCHK(hipMemcpyAsync(B->d_descs, B->h_descs.data(), B->h_descs.size() * sizeof(AfPass),
-                                  hipMemcpyHostToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
+        AFCHK(batch_upload(B, in_re, in_im, out_re, out_im, s));
         std::memcpy(B->last, cur, sizeof cur);
         B->uploaded = true;
     }

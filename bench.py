@@ -49,8 +49,8 @@ def parse():
                     help="only the headline transform (steps launches) and a 32 MiB device copy of "
                          "known size, for rocprofv3 kernel traces and counters")
     ap.add_argument("--profile-uneven", action="store_true",
-                    help="only the 4096-row uneven batch (FP32 forward, warmup + steps runs), for "
-                         "rocprofv3 kernel traces")
+                    help="only the 4096-row uneven batch (FP32 forward, chunk size autotuned unless "
+                         "--no-tune, warmup + steps runs), for rocprofv3 kernel traces")
     ap.add_argument("--profile-conv", action="store_true",
                     help="only the 512 x 8192 FP32 real convolution and its adjoint (warmup + steps "
                          "calls), for rocprofv3 kernel traces")
@@ -266,10 +266,11 @@ def karatsuba_fft_f32(torch, np, lens, data, steps, warmup):
             "c5_fft_f32_rounded_exact_frac": hit / tot}
 
 
-def uneven_batch(torch, np, steps):
+def uneven_batch(torch, np, steps, tune=True):
     """The 4096-row uneven batch (four length classes, sum of lengths 2.86e8), FP32
-    forward, inputs U[-1, 1) from splitmix64 seed 2 resident in HBM; error of every
-    row against the FP64 batch on the same inputs, summarised per class."""
+    forward, inputs U[-1, 1) from splitmix64 seed 2 resident in HBM, its graph's
+    chunk size autotuned (af_batch_tune) unless tune is False; error of every row
+    against the FP64 batch on the same inputs, summarised per class."""
     from accelerate_fourier import batch, forward, synthetic
     lens, cls = synthetic.uneven_lengths(2)
     tot = int(lens.sum())
@@ -277,6 +278,8 @@ def uneven_batch(torch, np, steps):
     b32 = batch.UnevenBatch(lens, forward, "f32")
     b64 = batch.UnevenBatch(lens, forward, "f64")
     create_s = time.perf_counter() - t0
+    default_plan = b32.describe().split("\n")[0]
+    report = b32.tune() if tune else ""
     v = synthetic.uniform(2, 2 * tot)
     x64 = torch.complex(torch.from_numpy(v[:tot]), torch.from_numpy(v[tot:])).cuda()
     del v
@@ -298,7 +301,9 @@ def uneven_batch(torch, np, steps):
     out = {"c3_uneven_rows": int(len(lens)), "c3_uneven_total_length": tot, "c3_uneven_f32_ms": ms,
            "c3_uneven_GFLOPs": flops / (ms * 1e-3) / 1e9,
            "c3_uneven_hbm_frac": 16.0 * tot / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-           "c3_uneven_create_s": create_s, "c3_uneven_plan": b32.describe().split("\n")[0], "c3_uneven_error_vs_fp64": {}}
+           "c3_uneven_create_s": create_s, "c3_uneven_plan": b32.describe().split("\n")[0],
+           "c3_uneven_default_plan": default_plan,
+           "c3_autotune": report.strip().split("\n") if report else None, "c3_uneven_error_vs_fp64": {}}
     for c, name in enumerate(synthetic.UNEVEN_CLASSES):
         r = rel[cls == c]
         out["c3_uneven_error_vs_fp64"][name] = {"rows": int(len(r)), "max_rel_l2": float(r.max()),
@@ -332,6 +337,7 @@ def main():
         lens, _ = syn.uneven_lengths(2)
         tot = int(lens.sum())
         b32 = ubatch.UnevenBatch(lens, forward, "f32")
+        report = "" if args.no_tune else b32.tune()
         v = syn.uniform(2, 2 * tot)
         x32 = torch.complex(torch.from_numpy(v[:tot].astype(np.float32)),
                             torch.from_numpy(v[tot:].astype(np.float32))).cuda()
@@ -340,7 +346,8 @@ def main():
         for _ in range(warmup + steps):
             b32(x32, out=y)
         torch.cuda.synchronize()
-        print(json.dumps({"profile": "done", "uneven_runs": 1 + warmup + steps, "plan": b32.describe()}))
+        print(json.dumps({"profile": "done", "uneven_runs": 1 + warmup + steps, "plan": b32.describe(),
+                          "autotune": report.strip().split("\n") if report else None}))
         return
     if args.profile_conv or args.profile_karatsuba:
         fn = conv_workload(torch)[0] if args.profile_conv else karatsuba_workload(torch, np)[0]
@@ -536,7 +543,7 @@ def main():
                                  "kernel": "k_karatsuba (one workgroup per pair)"}
         extras.update(karatsuba_fft_f32(torch, np, lens, kdata, steps, warmup))
         del kar, kdata
-        extras.update(uneven_batch(torch, np, steps))
+        extras.update(uneven_batch(torch, np, steps, tune=not args.no_tune))
         out["extras"] = extras
 
     if not args.skip_cpu:

@@ -320,11 +320,28 @@ af_status af_karatsuba_complex_i64(int64_t batch, int64_t len, int32_t cyclic,
  * A batch of `count` complex transforms of lengths lengths[i] >= 0, all with
  * one sign and precision, packed back to back (row i starts at the prefix sum
  * of the lengths).  Equivalent to Planned.transform sign lengths[i] on each
- * row.  The work of all rows runs as one captured hipGraph.
+ * row.  The work of all rows runs as one captured hipGraph, the k-th passes
+ * of all rows grouped by kernel.  af_batch_tune may instead cut the rows whose
+ * plans need intermediates (four-step, Rader and Bluestein cores) into chunks
+ * of a bounded intermediate size, all passes of a chunk before the next, so
+ * that intermediates can stay in the memory-side cache between passes and the
+ * scratch memory is the chunk size instead of the sum over all rows.
  */
 af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign,
                           int32_t precision, af_batch *batch);
 af_status af_batch_destroy(af_batch batch);
+/*
+ * Autotuner of a batch (no reference counterpart): times the batch's graph
+ * for several chunk sizes (no chunking, 32 MiB ... 384 MiB of intermediates)
+ * on zero-filled interleaved buffers it allocates itself, keeps the fastest
+ * (it must beat the incumbent by 3%) and recaptures the graph with it.
+ * Synchronizes `stream`.  `report` (may be NULL) receives one line per
+ * candidate and the choice.
+ */
+af_status af_batch_tune(af_batch batch, void *stream, char *report, size_t report_len);
+/* Recaptures the batch's graph with chunks of at most chunk_bytes of
+ * intermediates (0: no chunking; a row needing more is a chunk of its own). */
+af_status af_batch_set_chunk_bytes(af_batch batch, int64_t chunk_bytes);
 /*
  * One call runs every row.  A batch owns one set of launch descriptors and
  * intermediates, so calls on one batch are serialised: concurrent calls from

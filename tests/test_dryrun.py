@@ -189,7 +189,11 @@ def _uneven_lengths():
 @pytest.mark.parametrize("prec", [1, 0])
 def test_uneven_batch_4096_rows(dry, prec):
     """The 4096-row uneven batch: every plan (with its plan-time FP64 transforms)
-    and the captured grouped launches."""
+    and the captured grouped launches (one chunk of rows without intermediates,
+    one with); in FP32 also,
+    on every 7th row, every chunk size the batch autotuner builds and launches
+    (the stand-in runtime's events read 1 ms for every interval, so the first
+    candidate, no chunking, is kept and launched again)."""
     before = dry.dry_problem_count()
     lens = _uneven_lengths()
     arr = (ctypes.c_int64 * len(lens))(*[int(x) for x in lens])
@@ -198,8 +202,28 @@ def test_uneven_batch_4096_rows(dry, prec):
     e = 16 if prec == 0 else 8
     b = Bufs(dry)
     tot = int(np.sum(lens))
+    dry.af_batch_describe.argtypes = [V, ctypes.c_char_p, ctypes.c_size_t]
+    desc = ctypes.create_string_buffer(1 << 20)
+    assert dry.af_batch_describe(h, desc, len(desc)) == 0
+    head = desc.value.decode().split("\n")[0]
+    assert "chunk_bytes=0" in head and int(head.split(" chunks=")[1].split()[0]) == 2, head
     assert dry.af_batch_execute(h, b(tot * e), None, b(tot * e), None, None) == 0
     dry.af_batch_destroy(h)
+    if prec == 1:
+        lens = lens[::7]
+        arr = (ctypes.c_int64 * len(lens))(*[int(x) for x in lens])
+        tot = int(np.sum(lens))
+        assert dry.af_batch_create(len(lens), arr, -1, prec, ctypes.byref(h)) == 0
+        dry.af_batch_tune.argtypes = [V, V, ctypes.c_char_p, ctypes.c_size_t]
+        rep = ctypes.create_string_buffer(8192)
+        assert dry.af_batch_tune(h, None, rep, len(rep)) == 0
+        text = rep.value.decode()
+        assert text.count("chunk_bytes=") == 7 and "chosen chunk_bytes=0" in text, text
+        assert dry.af_batch_describe(h, desc, len(desc)) == 0
+        assert " chunks=" in desc.value.decode().split("\n")[0]
+        h2 = b(tot * e // 2)
+        assert dry.af_batch_execute(h, b(tot * e // 2), h2, b(tot * e // 2), b(tot * e // 2), None) == 0
+        dry.af_batch_destroy(h)
     b.free()
     assert problems(dry, before) == []
 

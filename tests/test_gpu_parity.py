@@ -684,11 +684,13 @@ def _uneven_inputs():
 
 
 def _launch_keys(desc):
-    """(key text, first row) of every grouped launch in a batch description."""
+    """(key text, first row) of every grouped launch in a batch description
+    ("launch k chunk c phase p <kernel key> rows=r first=f"); the key text is
+    the phase and the kernel key."""
     keys = []
     for line in desc.split("\n")[1:]:
         head, first = line.rsplit(" first=", 1)
-        keys.append((head.split(" ", 2)[2].rsplit(" rows=", 1)[0], int(first)))
+        keys.append((head.split(" ", 4)[4].rsplit(" rows=", 1)[0], int(first)))
     return keys
 
 
@@ -729,7 +731,7 @@ def test_uneven_batch_full_config():
             c = int(cls[r])
             worst[c] = max(worst.get(c, 0.0), e)
     print(f"uneven batch: {len(rows)} rows vs numpy FP64 in both layouts, launch keys covered: "
-          + "; ".join(k for k, _ in keys) + "; worst rel-L2 per class " + str(worst))
+          + "; ".join(sorted({k for k, _ in keys})) + "; worst rel-L2 per class " + str(worst))
     del y_pla
     y64 = UnevenBatch(lens, forward, "f64")(x64)
     sizes = [int(n) for n in lens]
@@ -747,6 +749,49 @@ def test_uneven_batch_full_config():
             got = host(y64[off[r]:off[r + 1]])
             assert rel_l2(got, ref) <= tol_f64(n), (c, int(r), n)
     del x64, y64, y_int, x32
+
+
+def test_uneven_batch_chunks_and_tuner():
+    """The 4096-row FP32 batch with its rows that need intermediates cut into
+    chunks of 32 MiB and 160 MiB (every chunk's passes on one shared scratch
+    region, rows reordered by shape): the same results as the unchunked graph
+    on the seed-2 inputs, up to the rounding of kernels that a chunk's grouping
+    may select differently (1e-6 relative per row, and bit-identical rows for
+    the power-of-two groups, whose kernel key does not depend on the grouping);
+    then af_batch_tune's report names every candidate and its choice, and the
+    tuned graph gives the unchunked results again."""
+    from accelerate_fourier.batch import UnevenBatch
+    lens, cls, x64 = _uneven_inputs()
+    x32 = x64.to(torch.complex64)
+    del x64
+    sizes = [int(n) for n in lens]
+    b = UnevenBatch(lens, forward, "f32")
+    ref = b(x32)
+    torch.cuda.synchronize()
+    for cb in (32 << 20, 160 << 20):
+        b.set_chunk_bytes(cb)
+        head = b.describe().split("\n")[0]
+        assert f"chunk_bytes={cb}" in head, head
+        nchunks = int(head.split(" chunks=")[1].split()[0])
+        assert nchunks > 4, head
+        got = b(x32)
+        torch.cuda.synchronize()
+        d = torch.stack([torch.linalg.vector_norm(r) for r in torch.split(got - ref, sizes)])
+        nrm = torch.stack([torch.linalg.vector_norm(r) for r in torch.split(ref, sizes)])
+        rel = (d / nrm).cpu().numpy()
+        assert rel.max() <= 1e-6, (cb, float(rel.max()), int(rel.argmax()))
+        print(f"chunk_bytes={cb}: {nchunks} chunks, max rel-L2 vs unchunked {rel.max():.2e}, "
+              f"identical rows {int((rel == 0).sum())} of {len(rel)}")
+        del got
+    rep = b.tune()
+    assert rep.count("chunk_bytes=") == 7 and "chosen chunk_bytes=" in rep, rep
+    got = b(x32)
+    torch.cuda.synchronize()
+    d = torch.stack([torch.linalg.vector_norm(r) for r in torch.split(got - ref, sizes)])
+    nrm = torch.stack([torch.linalg.vector_norm(r) for r in torch.split(ref, sizes)])
+    assert float((d / nrm).max()) <= 1e-6
+    with pytest.raises(ValueError):
+        b.set_chunk_bytes(-1)
 
 
 def test_uneven_batch_two_streams_and_threads():
