@@ -88,7 +88,7 @@ _SIGNATURES = {
     "af_batch_execute": (ctypes.c_int, [_v, _v, _v, _v, _v, _v]),
     "af_batch_describe": (ctypes.c_int, [_v, ctypes.c_char_p, ctypes.c_size_t]),
     "af_batch_tune": (ctypes.c_int, [_v, _v, ctypes.c_char_p, ctypes.c_size_t]),
-    "af_batch_set_chunk_bytes": (ctypes.c_int, [_v, _i64]),
+    "af_batch_set_schedule": (ctypes.c_int, [_v, _i64, _i32]),
     "af_rader_permutations": (ctypes.c_int, [_i64, _p64, _p64, _p64]),
     "af_good_thomas_maps": (ctypes.c_int, [_i64, _i64, _p64, _p64]),
     "af_multiplicative_generator": (_i64, [_i64]),

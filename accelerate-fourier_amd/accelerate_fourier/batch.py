@@ -45,12 +45,16 @@ class UnevenBatch:
         _abi.check(_abi.lib().af_batch_describe(self.handle, buf, len(buf)), "af_batch_describe")
         return buf.value.decode()
 
-    def set_chunk_bytes(self, chunk_bytes: int) -> None:
-        """af_batch_set_chunk_bytes: rows with intermediates in chunks of at
-        most chunk_bytes of them (0: no chunking)."""
+    def set_schedule(self, chunk_bytes: int = 0, lanes: int = 1) -> None:
+        """af_batch_set_schedule: rows with intermediates in chunks of at most
+        chunk_bytes of them (0: no chunking); lanes = 2 runs the rows without
+        intermediates on a parallel branch of the graph."""
         if int(chunk_bytes) < 0:
             raise ValueError("chunk_bytes must be >= 0")
-        _abi.check(_abi.lib().af_batch_set_chunk_bytes(self.handle, int(chunk_bytes)), "af_batch_set_chunk_bytes")
+        if int(lanes) not in (1, 2):
+            raise ValueError("lanes must be 1 or 2")
+        _abi.check(_abi.lib().af_batch_set_schedule(self.handle, int(chunk_bytes), int(lanes)),
+                   "af_batch_set_schedule")
 
     def tune(self) -> str:
         """af_batch_tune: times chunk sizes of the batch's graph on the current

@@ -2043,6 +2043,11 @@ struct af_batch_s {
     // the memory-side cache between its passes.
     int64_t chunk_bytes = 0;
     int nchunks = 0;
+    // lanes = 2: the rows without intermediates (chunk 0 when there are any)
+    // run on a branch of the graph beside the others, so that their small
+    // launches overlap the large ones; 1: every launch in one sequence
+    int lanes = 1;
+    bool light_chunk = false;
     DevBuf scratch_buf, descs_buf, prefix_buf;
     void *scratch = nullptr;
     struct Group {
@@ -2120,8 +2125,9 @@ af_status batch_passes(af_batch B, int64_t i, const void *in_re, const void *in_
 // of its own); every chunk's intermediates start at offset 0 of one region.
 // Within a chunk, the k-th passes of its rows are grouped by kernel, as the
 // whole batch is when chunk_bytes is 0.
-af_status batch_build(af_batch B, int64_t chunk_bytes) {
+af_status batch_build(af_batch B, int64_t chunk_bytes, int lanes) {
     B->drop_graph();
+    B->lanes = lanes;
     B->groups.clear();
     B->h_descs.clear();
     B->h_prefix.clear();
@@ -2141,6 +2147,7 @@ af_status batch_build(af_batch B, int64_t chunk_bytes) {
     });
     std::vector<std::vector<int64_t>> chunks;
     if (!light.empty()) chunks.push_back(light);
+    B->light_chunk = !light.empty();
     size_t region = 0, used = 0;
     for (int64_t i : heavy) {
         const size_t need = B->scratch_need[i];
@@ -2230,23 +2237,44 @@ af_status batch_build(af_batch B, int64_t chunk_bytes) {
     B->d_descs = (AfPass *)B->descs_buf.p;
     B->d_prefix = (int64_t *)B->prefix_buf.p;
     if (nprefix) HIPCHK(hipMemcpy(B->d_prefix, B->h_prefix.data(), nprefix * 8, hipMemcpyHostToDevice));
-    // capture the grouped launches once
+    // capture the grouped launches once; with two lanes the light chunk's
+    // launches go to a second stream forked from and joined back to the first
+    // (the capture turns the fork and join into graph dependencies)
     if (!B->groups.empty()) {
-        hipStream_t s = nullptr;
+        const bool two = lanes == 2 && B->light_chunk && B->nchunks > 1;
+        hipStream_t s = nullptr, s2 = nullptr;
+        hipEvent_t fork = nullptr, join = nullptr;
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         af_status st = AF_SUCCESS;
-        { const af_status h__ = HIPST(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)); if (st == AF_SUCCESS) st = h__; }
+        if (two) {
+            st = HIPST(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+            if (st == AF_SUCCESS) st = HIPST(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+            if (st == AF_SUCCESS) st = HIPST(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+        }
+        if (st == AF_SUCCESS) st = HIPST(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const bool capturing = st == AF_SUCCESS;
+        if (two && st == AF_SUCCESS) st = HIPST(hipEventRecord(fork, s));
+        if (two && st == AF_SUCCESS) st = HIPST(hipStreamWaitEvent(s2, fork, 0));
         for (const auto &g : B->groups) {
             if (st != AF_SUCCESS) break;
+            hipStream_t gs = two && g.chunk == 0 ? s2 : s;
             if (g.ext)
                 st = HIPST(launch_fast_ext_grouped(B->prec, g.L, g.B, g.conv, g.sign, B->d_descs + g.desc_off,
-                                                   B->d_prefix + g.prefix_off, (int)g.rows.size(), g.total_tiles, s));
+                                                   B->d_prefix + g.prefix_off, (int)g.rows.size(), g.total_tiles, gs));
             else
                 st = HIPST(launch_pass_grouped(B->prec, B->d_descs + g.desc_off, B->d_prefix + g.prefix_off,
-                                               (int)g.rows.size(), g.total_tiles, g.nthreads, g.lds, g.full, s));
+                                               (int)g.rows.size(), g.total_tiles, g.nthreads, g.lds, g.full, gs));
         }
+        if (two && st == AF_SUCCESS) st = HIPST(hipEventRecord(join, s2));
+        if (two && st == AF_SUCCESS) st = HIPST(hipStreamWaitEvent(s, join, 0));
         hipGraph_t graph = nullptr;
-        { const af_status h__ = HIPST(hipStreamEndCapture(s, &graph)); if (st == AF_SUCCESS) st = h__; }
+        if (capturing) {
+            const af_status h__ = HIPST(hipStreamEndCapture(s, &graph));
+            if (st == AF_SUCCESS) st = h__;
+        }
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (s2) (void)hipStreamDestroy(s2);
         (void)hipStreamDestroy(s);
         if (st != AF_SUCCESS) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -2257,7 +2285,8 @@ af_status batch_build(af_batch B, int64_t chunk_bytes) {
     }
     // description: algorithm counts, chunks, one line per grouped launch
     B->text = B->summary + " launches=" + std::to_string(B->groups.size()) + " chunks=" +
-              std::to_string(B->nchunks) + " chunk_bytes=" + std::to_string((long long)chunk_bytes) +
+              std::to_string(B->nchunks) + " lanes=" + std::to_string(B->lanes) +
+              " chunk_bytes=" + std::to_string((long long)chunk_bytes) +
               " scratch_bytes=" + std::to_string((unsigned long long)region);
     for (size_t k = 0; k < B->groups.size(); ++k) {
         const auto &g = B->groups[k];
@@ -2307,6 +2336,7 @@ af_status batch_upload(af_batch B, const void *in_re, const void *in_im, void *o
 // for each batch; chunking also bounds the scratch memory (5.7 GB unchunked,
 // the chunk budget chunked).
 constexpr int64_t kBatchChunkBytes = 0;
+constexpr int kBatchLanes = 1;
 
 }  // namespace
 
@@ -2365,17 +2395,17 @@ af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign, i
     }
     B->summary = "batch" + std::to_string(count);
     for (auto &kv : counts) B->summary += " " + kv.first + "=" + std::to_string(kv.second);
-    AFCHK(batch_build(B.get(), kBatchChunkBytes));
+    AFCHK(batch_build(B.get(), kBatchChunkBytes, kBatchLanes));
     *batch = B.release();
     return AF_SUCCESS;
 }
 
-af_status af_batch_set_chunk_bytes(af_batch B, int64_t chunk_bytes) {
-    if (!B || chunk_bytes < 0) return AF_INVALID_ARG;
+af_status af_batch_set_schedule(af_batch B, int64_t chunk_bytes, int32_t lanes) {
+    if (!B || chunk_bytes < 0 || (lanes != 1 && lanes != 2)) return AF_INVALID_ARG;
     std::lock_guard<std::mutex> lk(B->mu);
     if (B->recorded) HIPCHK(hipEventSynchronize(B->done));
     B->recorded = false;
-    return batch_build(B, chunk_bytes);
+    return batch_build(B, chunk_bytes, lanes);
 }
 
 af_status af_batch_tune(af_batch B, void *stream, char *report, size_t report_len) {
@@ -2397,16 +2427,24 @@ af_status af_batch_tune(af_batch B, void *stream, char *report, size_t report_le
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     const int64_t start = B->chunk_bytes;
-    const int64_t candidates[] = {0, (int64_t)32 << 20, (int64_t)64 << 20, (int64_t)128 << 20, (int64_t)192 << 20,
-                                  (int64_t)384 << 20};
+    const int start_lanes = B->lanes;
+    const std::pair<int64_t, int> candidates[] = {{0, 1},
+                                                  {0, 2},
+                                                  {(int64_t)32 << 20, 2},
+                                                  {(int64_t)64 << 20, 2},
+                                                  {(int64_t)128 << 20, 2},
+                                                  {(int64_t)192 << 20, 2},
+                                                  {(int64_t)384 << 20, 2}};
     std::string text;
     int64_t best = start;
+    int best_lanes = start_lanes;
     double tbest = 1e30;
     af_status st = AF_SUCCESS;
-    // each budget: one warm-up, then the best of three launches; the incumbent
-    // must be beaten by 3%
-    for (int64_t cb : candidates) {
-        st = batch_build(B, cb);
+    // each candidate: one warm-up, then the best of three launches; the
+    // incumbent must be beaten by 3%
+    for (const auto &cand : candidates) {
+        const int64_t cb = cand.first;
+        st = batch_build(B, cb, cand.second);
         if (st == AF_SUCCESS) st = batch_upload(B, in.p, nullptr, out.p, nullptr, s);
         double t = 1e30;
         for (int r = 0; r < 4 && st == AF_SUCCESS; ++r) {
@@ -2420,23 +2458,25 @@ af_status af_batch_tune(af_batch B, void *stream, char *report, size_t report_le
         }
         if (st != AF_SUCCESS) break;
         char line[160];
-        std::snprintf(line, sizeof line, "chunk_bytes=%lld chunks=%d launches=%zu %.3f ms\n", (long long)cb, B->nchunks,
-                      B->groups.size(), t);
+        std::snprintf(line, sizeof line, "chunk_bytes=%lld lanes=%d chunks=%d launches=%zu %.3f ms\n", (long long)cb,
+                      cand.second, B->nchunks, B->groups.size(), t);
         text += line;
         if (t < tbest * 0.97) {
             tbest = t;
             best = cb;
+            best_lanes = cand.second;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     { const af_status h__ = HIPST(hipStreamSynchronize(s)); if (st == AF_SUCCESS) st = h__; }
     // the chosen budget (or, after a failure, the one the batch had)
-    const af_status st2 = batch_build(B, st == AF_SUCCESS ? best : start);
+    const af_status st2 =
+        st == AF_SUCCESS ? batch_build(B, best, best_lanes) : batch_build(B, start, start_lanes);
     B->recorded = false;
     if (st == AF_SUCCESS) st = st2;
     if (st == AF_SUCCESS && report && report_len) {
-        text += "chosen chunk_bytes=" + std::to_string((long long)best) + "\n";
+        text += "chosen chunk_bytes=" + std::to_string((long long)best) + " lanes=" + std::to_string(best_lanes) + "\n";
         std::snprintf(report, report_len, "%s", text.c_str());
     }
     return st;

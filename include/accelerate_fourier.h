@@ -331,17 +331,23 @@ af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign,
                           int32_t precision, af_batch *batch);
 af_status af_batch_destroy(af_batch batch);
 /*
+ * Schedule of a batch's graph (no reference counterpart): rows with
+ * intermediates in chunks of at most chunk_bytes of them (0: no chunking; a
+ * row needing more is a chunk of its own), and lanes = 2 to run the rows
+ * without intermediates on a branch of the graph beside the others (their
+ * small launches then overlap the large ones) or 1 for one sequence.
+ * Recaptures the graph.  The default is (0, 1).
+ */
+af_status af_batch_set_schedule(af_batch batch, int64_t chunk_bytes, int32_t lanes);
+/*
  * Autotuner of a batch (no reference counterpart): times the batch's graph
- * for several chunk sizes (no chunking, 32 MiB ... 384 MiB of intermediates)
- * on zero-filled interleaved buffers it allocates itself, keeps the fastest
- * (it must beat the incumbent by 3%) and recaptures the graph with it.
- * Synchronizes `stream`.  `report` (may be NULL) receives one line per
- * candidate and the choice.
+ * for several schedules (one lane unchunked; two lanes unchunked and with
+ * chunks of 32 MiB ... 384 MiB of intermediates) on zero-filled interleaved
+ * buffers it allocates itself, keeps the fastest (it must beat the incumbent
+ * by 3%) and recaptures the graph with it.  Synchronizes `stream`.  `report`
+ * (may be NULL) receives one line per candidate and the choice.
  */
 af_status af_batch_tune(af_batch batch, void *stream, char *report, size_t report_len);
-/* Recaptures the batch's graph with chunks of at most chunk_bytes of
- * intermediates (0: no chunking; a row needing more is a chunk of its own). */
-af_status af_batch_set_chunk_bytes(af_batch batch, int64_t chunk_bytes);
 /*
  * One call runs every row.  A batch owns one set of launch descriptors and
  * intermediates, so calls on one batch are serialised: concurrent calls from

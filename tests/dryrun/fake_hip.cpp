@@ -66,7 +66,19 @@ thread_local std::vector<Config> g_cfg;
 struct FakeStream {
     bool capturing = false;
     std::vector<std::function<void()>> captured;
+    // stream capture across streams: a stream that waits on an event recorded
+    // on a capturing stream joins that capture (its launches go to the same
+    // graph) until the capture ends; `pending` marks work not yet joined back
+    // to the origin by an event the origin waits on
+    struct FakeStream *joined = nullptr;
+    bool pending = false;
+    std::vector<struct FakeStream *> forks;
 };
+struct FakeEvent {
+    FakeStream *root = nullptr;  // capture the recording stream belonged to
+    FakeStream *from = nullptr;
+};
+FakeStream *capture_root(FakeStream *s) { return !s ? nullptr : s->capturing ? s : s->joined; }
 struct FakeGraph {
     std::vector<std::function<void()>> nodes;
 };
@@ -668,8 +680,9 @@ hipError_t hipLaunchKernel(const void *f, dim3 grid, dim3 block, void **args, si
     if (grid.x == 0 || block.x == 0) problem(name + ": empty launch");
     auto fn = check_launch(name, grid, block, shmem, args);
     FakeStream *fs = (FakeStream *)stream;
-    if (fs && fs->capturing) {
-        fs->captured.push_back(fn);
+    if (FakeStream *root = capture_root(fs)) {
+        root->captured.push_back(fn);
+        if (fs != root) fs->pending = true;
         return hipSuccess;
     }
     ++g_checked;
@@ -726,20 +739,40 @@ hipError_t hipMemset(void *dst, int v, size_t n) { return hipMemsetAsync(dst, v,
 
 // ---- events: every interval reads as 1 ms (the autotuner then keeps the planned shapes) ----
 hipError_t hipEventCreate(hipEvent_t *e) {
-    static char token[64];
-    static int next = 0;
-    *e = (hipEvent_t)&token[next++ % 64];
+    *e = (hipEvent_t) new FakeEvent;
     return hipSuccess;
 }
 hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
-hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) {
+    if (!e) return hipErrorInvalidValue;
+    FakeEvent *fe = (FakeEvent *)e;
+    fe->root = capture_root((FakeStream *)s);
+    fe->from = (FakeStream *)s;
+    return hipSuccess;
+}
 hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
-hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned int) { return hipSuccess; }
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned int) {
+    if (!e) return hipErrorInvalidValue;
+    FakeEvent *fe = (FakeEvent *)e;
+    FakeStream *fs = (FakeStream *)s;
+    if (!fe->root || !fs || fe->from == fs) return hipSuccess;
+    if (fs == fe->root) {
+        // the origin waits on a forked stream: its work so far is joined back
+        if (fe->from) fe->from->pending = false;
+    } else if (capture_root(fs) != fe->root) {
+        fs->joined = fe->root;
+        fe->root->forks.push_back(fs);
+    }
+    return hipSuccess;
+}
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) {
     *ms = 1.0f;
     return hipSuccess;
 }
-hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventDestroy(hipEvent_t e) {
+    delete (FakeEvent *)e;
+    return hipSuccess;
+}
 hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
 // pinned host memory: plain host memory here (never a kernel operand)
 hipError_t hipHostMalloc(void **p, size_t n, unsigned int) {
@@ -802,6 +835,12 @@ hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
 }
 hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t *g) {
     FakeStream *fs = (FakeStream *)s;
+    for (FakeStream *f : fs->forks) {
+        if (f->pending) problem("stream capture ended with a forked stream's work not joined back");
+        f->joined = nullptr;
+        f->pending = false;
+    }
+    fs->forks.clear();
     FakeGraph *fg = new FakeGraph;
     fg->nodes.swap(fs->captured);
     fs->capturing = false;

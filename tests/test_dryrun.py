@@ -191,9 +191,10 @@ def test_uneven_batch_4096_rows(dry, prec):
     """The 4096-row uneven batch: every plan (with its plan-time FP64 transforms)
     and the captured grouped launches (one chunk of rows without intermediates,
     one with); in FP32 also,
-    on every 7th row, every chunk size the batch autotuner builds and launches
+    on every 7th row, every schedule the batch autotuner builds and launches:
+    chunk sizes, and the rows without intermediates on a second captured stream
     (the stand-in runtime's events read 1 ms for every interval, so the first
-    candidate, no chunking, is kept and launched again)."""
+    candidate, no chunking on one lane, is kept and launched again)."""
     before = dry.dry_problem_count()
     lens = _uneven_lengths()
     arr = (ctypes.c_int64 * len(lens))(*[int(x) for x in lens])
@@ -218,7 +219,7 @@ def test_uneven_batch_4096_rows(dry, prec):
         rep = ctypes.create_string_buffer(8192)
         assert dry.af_batch_tune(h, None, rep, len(rep)) == 0
         text = rep.value.decode()
-        assert text.count("chunk_bytes=") == 7 and "chosen chunk_bytes=0" in text, text
+        assert text.count("chunk_bytes=") == 8 and "chosen chunk_bytes=0 lanes=1" in text, text
         assert dry.af_batch_describe(h, desc, len(desc)) == 0
         assert " chunks=" in desc.value.decode().split("\n")[0]
         h2 = b(tot * e // 2)

@@ -754,7 +754,9 @@ def test_uneven_batch_full_config():
 def test_uneven_batch_chunks_and_tuner():
     """The 4096-row FP32 batch with its rows that need intermediates cut into
     chunks of 32 MiB and 160 MiB (every chunk's passes on one shared scratch
-    region, rows reordered by shape): the same results as the unchunked graph
+    region, rows reordered by shape), and the rows without intermediates on a
+    second branch of the graph (160 MiB and unchunked): the same results as the
+    default graph
     on the seed-2 inputs, up to the rounding of kernels that a chunk's grouping
     may select differently (1e-6 relative per row, and bit-identical rows for
     the power-of-two groups, whose kernel key does not depend on the grouping);
@@ -768,30 +770,33 @@ def test_uneven_batch_chunks_and_tuner():
     b = UnevenBatch(lens, forward, "f32")
     ref = b(x32)
     torch.cuda.synchronize()
-    for cb in (32 << 20, 160 << 20):
-        b.set_chunk_bytes(cb)
+    for cb, lanes in ((32 << 20, 1), (160 << 20, 2), (0, 2)):
+        b.set_schedule(cb, lanes)
         head = b.describe().split("\n")[0]
-        assert f"chunk_bytes={cb}" in head, head
+        assert f"chunk_bytes={cb}" in head and f"lanes={lanes}" in head, head
         nchunks = int(head.split(" chunks=")[1].split()[0])
-        assert nchunks > 4, head
+        assert nchunks > 4 or cb == 0, head
         got = b(x32)
         torch.cuda.synchronize()
         d = torch.stack([torch.linalg.vector_norm(r) for r in torch.split(got - ref, sizes)])
         nrm = torch.stack([torch.linalg.vector_norm(r) for r in torch.split(ref, sizes)])
         rel = (d / nrm).cpu().numpy()
         assert rel.max() <= 1e-6, (cb, float(rel.max()), int(rel.argmax()))
-        print(f"chunk_bytes={cb}: {nchunks} chunks, max rel-L2 vs unchunked {rel.max():.2e}, "
+        print(f"chunk_bytes={cb} lanes={lanes}: {nchunks} chunks, max rel-L2 vs unchunked {rel.max():.2e}, "
               f"identical rows {int((rel == 0).sum())} of {len(rel)}")
         del got
     rep = b.tune()
-    assert rep.count("chunk_bytes=") == 7 and "chosen chunk_bytes=" in rep, rep
+    assert rep.count("chunk_bytes=") == 8 and "chosen chunk_bytes=" in rep, rep
+    print(rep)
     got = b(x32)
     torch.cuda.synchronize()
     d = torch.stack([torch.linalg.vector_norm(r) for r in torch.split(got - ref, sizes)])
     nrm = torch.stack([torch.linalg.vector_norm(r) for r in torch.split(ref, sizes)])
     assert float((d / nrm).max()) <= 1e-6
     with pytest.raises(ValueError):
-        b.set_chunk_bytes(-1)
+        b.set_schedule(-1)
+    with pytest.raises(ValueError):
+        b.set_schedule(0, 3)
 
 
 def test_uneven_batch_two_streams_and_threads():
