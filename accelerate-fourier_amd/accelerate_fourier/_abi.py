@@ -21,6 +21,7 @@ C2C, R2C_FULL, C2R_FULL, CONV_CYCLIC, REAL_CONV = range(5)
 # af_algorithm
 ALG_AUTO, ALG_DECOMPOSE, ALG_CHIRP2, ALG_CHIRP235 = range(4)
 FLAG_PLAIN_TWIDDLES = 0x1
+FLAG_COOLEY_TUKEY = 0x2
 
 
 class PlanDesc(ctypes.Structure):

@@ -53,9 +53,9 @@ def get_plan(length, sign, kind, precision, algorithm=_abi.ALG_AUTO, flags=0) ->
         return p
 
 
-def describe(length, sign=-1, kind=_abi.C2C, precision=_abi.F32, algorithm=_abi.ALG_AUTO) -> str:
+def describe(length, sign=-1, kind=_abi.C2C, precision=_abi.F32, algorithm=_abi.ALG_AUTO, flags=0) -> str:
     """Host-only text of the device plan for a length."""
-    d = _abi.PlanDesc(length, sign, kind, precision, algorithm, 0)
+    d = _abi.PlanDesc(length, sign, kind, precision, algorithm, flags)
     buf = ctypes.create_string_buffer(4096)
     st = _abi.lib().af_plan_describe(ctypes.byref(d), buf, len(buf))
     if st not in (_abi.SUCCESS, _abi.UNSUPPORTED):
@@ -66,7 +66,8 @@ def describe(length, sign=-1, kind=_abi.C2C, precision=_abi.F32, algorithm=_abi.
 class Transform:
     """``Planned.transform sign n`` as a callable over GPU arrays."""
 
-    def __init__(self, sign: Sign, n: int, algorithm: int = _abi.ALG_AUTO, plain_twiddles: bool = False):
+    def __init__(self, sign: Sign, n: int, algorithm: int = _abi.ALG_AUTO, plain_twiddles: bool = False,
+                 cooley_tukey: bool = False):
         if not isinstance(sign, Sign):
             sign = Sign(int(sign))
         if n < 0:
@@ -74,7 +75,9 @@ class Transform:
         self.sign = sign
         self.n = int(n)
         self.algorithm = algorithm
-        self.flags = _abi.FLAG_PLAIN_TWIDDLES if plain_twiddles else 0
+        # cooley_tukey: single-pass lengths with several prime factors keep the
+        # twiddled Cooley-Tukey order instead of the default prime-factor order
+        self.flags = (_abi.FLAG_PLAIN_TWIDDLES if plain_twiddles else 0) | (_abi.FLAG_COOLEY_TUKEY if cooley_tukey else 0)
 
     def plan(self, precision) -> Plan:
         return get_plan(self.n, self.sign.value, _abi.C2C, precision, self.algorithm, self.flags)

@@ -14,8 +14,10 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <numeric>
 #include <string>
 #include <vector>
+#include <numeric>
 
 #include "accelerate_fourier.h"
 #include "kernels.hpp"
@@ -140,7 +142,96 @@ struct Core {
     // FP64 copies (the same tables for FP64 plans)
     void *tw_d = nullptr, *tw1_d = nullptr, *tw2_d = nullptr, *ft_coarse_d = nullptr, *ft_fine_d = nullptr;
     void *tw3_d = nullptr, *ft2_coarse_d = nullptr, *ft2_fine_d = nullptr;
+    // single pass in prime-factor order (AfPass::gt_in, gt_out, grp_mask)
+    int32_t *gt_in = nullptr, *gt_out = nullptr;
+    int32_t gt_mask = 0;
 };
+
+// The prime-factor order of a single-pass line of length L whose radices n_1
+// ... n_G (stage order) are pairwise coprime: stage g consumes the input digit
+// of weight S_g = n_{g+1} ... n_G and produces the output digit of weight
+// n_1 ... n_{g-1} (the Stockham stages leave the output in natural order of
+// that mixed radix), with no twiddles between the factors.  Input element e
+// enters at position sum_g S_g ((e mod n_g) (L/n_g)^-1 mod n_g): the inverse of
+// the skew-grid map sum_g (L/n_g) j_g mod L (permuteSkewGrid,
+// Planned.hs:597-606, for any number of factors); output element e is read
+// from position sum_g (e mod n_g) n_1 ... n_{g-1}, the Chinese-remainder map
+// (permuteSkewGridInv, 608-617).  False unless there are at least three
+// radices and no two of them share a prime: a repeated prime needs twiddles
+// inside its factor, and such lengths keep the Cooley-Tukey order.
+// Three radices at least: with two, the maps cost more than the one twiddled
+// stage they remove (measured on MI355X: 15 points 85.6 against 80.6 us per
+// 32 MiB; from three on prime-factor order is 2-11% faster).
+bool prime_factor_radices(const std::vector<int> &radices) {
+    if (radices.size() < 3) return false;
+    for (size_t a = 0; a < radices.size(); ++a)
+        for (size_t b = a + 1; b < radices.size(); ++b)
+            if (std::gcd(radices[a], radices[b]) != 1) return false;
+    return true;
+}
+
+bool prime_factor_maps(int64_t L, const std::vector<int> &radices, std::vector<int32_t> &gin,
+                       std::vector<int32_t> &gout, int32_t &mask) {
+    if (!prime_factor_radices(radices)) {
+        mask = 0;
+        return false;
+    }
+    auto prime_of = [](int r) {
+        for (int q : {2, 3, 5, 7, 11, 13})
+            if (r % q == 0) return q;
+        return r;
+    };
+    std::vector<int64_t> n;
+    mask = 0;
+    int last = 0;
+    for (size_t s = 0; s < radices.size(); ++s) {
+        const int q = prime_of(radices[s]);
+        if (s == 0 || q != last) {
+            if (s > 0) mask |= 1 << s;
+            n.push_back(radices[s]);
+        } else {
+            n.back() *= radices[s];
+        }
+        last = q;
+    }
+    if (n.size() < 2 || n.size() != radices.size()) {
+        mask = 0;
+        return false;
+    }
+    const size_t G = n.size();
+    std::vector<int64_t> sin(G), sout(G), inv(G);
+    int64_t acc = 1;
+    for (size_t g = G; g-- > 0;) {
+        sin[g] = acc;
+        acc *= n[g];
+    }
+    acc = 1;
+    for (size_t g = 0; g < G; ++g) {
+        sout[g] = acc;
+        acc *= n[g];
+        const int64_t c = (L / n[g]) % n[g];
+        int64_t x = 1;
+        while (x < n[g] && (c * x) % n[g] != 1) ++x;  // n[g] <= 16: a short search
+        if (x >= n[g] && n[g] > 1) {
+            mask = 0;
+            return false;
+        }
+        inv[g] = x;
+    }
+    gin.assign(L, 0);
+    gout.assign(L, 0);
+    for (int64_t e = 0; e < L; ++e) {
+        int64_t qi = 0, qo = 0;
+        for (size_t g = 0; g < G; ++g) {
+            const int64_t r = e % n[g];
+            qi += sin[g] * ((r * inv[g]) % n[g]);
+            qo += sout[g] * r;
+        }
+        gin[e] = (int32_t)qi;
+        gout[e] = (int32_t)qo;
+    }
+    return true;
+}
 
 // coarse x fine split of the twiddles W_N^a: 4^shift >= N
 int ft_shift_for(int64_t N) {
@@ -149,11 +240,19 @@ int ft_shift_for(int64_t N) {
     return shift;
 }
 
-af_status build_core(int prec, const CoreShape &sh, Core &c, Owner &own, hipStream_t s) {
+af_status build_core(int prec, const CoreShape &sh, Core &c, Owner &own, hipStream_t s, bool prime_factor = false) {
     c.sh = sh;
     if (sh.kind == CORE_SINGLE) {
         AFCHK(own.alloc(&c.tw, (size_t)sh.N * tw_bytes(prec)));
         HIPCHK(gen_twiddles(prec, c.tw, sh.N, sh.N, 1, s));
+        std::vector<int32_t> gin, gout;
+        if (prime_factor && prime_factor_maps(sh.N, sh.r, gin, gout, c.gt_mask)) {
+            AFCHK(own.alloc((void **)&c.gt_in, (size_t)sh.N * 4));
+            AFCHK(own.alloc((void **)&c.gt_out, (size_t)sh.N * 4));
+            HIPCHK(hipMemcpyAsync(c.gt_in, gin.data(), (size_t)sh.N * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(c.gt_out, gout.data(), (size_t)sh.N * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(hipStreamSynchronize(s));  // the host maps go out of scope
+        }
     } else if (sh.kind == CORE_FOUR || sh.kind == CORE_THREE) {
         AFCHK(own.alloc(&c.tw1, (size_t)sh.N1 * tw_bytes(prec)));
         AFCHK(own.alloc(&c.tw2, (size_t)sh.N2 * tw_bytes(prec)));
@@ -391,6 +490,9 @@ void c2c_passes(int prec, const Core &c, int64_t nrows, int sign, bool plain, co
         set_in(d, i2, 1, in.bs);
         set_out(d, o2, 1, out.bs);
         d.in_bs = d.out_bs = 0;
+        d.gt_in = c.gt_in;
+        d.gt_out = c.gt_out;
+        d.grp_mask = c.gt_mask;
         ps.push_back(d);
         return;
     }
@@ -514,6 +616,10 @@ struct PassTune {
 // the stage twiddle table W_L^m does not depend on it).
 void reorder_radices(AfPass &d, int order) {
     if (order == 0 || d.nst < 2) return;
+    // the prime-factor maps hold for the planned order only: a reordered pass
+    // runs in Cooley-Tukey order
+    d.gt_in = d.gt_out = nullptr;
+    d.grp_mask = 0;
     std::vector<int> r(d.radix, d.radix + d.nst);
     if (order == 1) std::reverse(r.begin(), r.end());
     else std::stable_partition(r.begin(), r.end(), [](int x) { return x % 2 == 1; });
@@ -600,7 +706,7 @@ af_status validate_desc(const af_plan_desc *d) {
     if (d->sign != 1 && d->sign != -1) return AF_INVALID_ARG;
     if (d->precision != AF_F64 && d->precision != AF_F32) return AF_INVALID_ARG;
     if (d->algorithm < AF_ALG_AUTO || d->algorithm > AF_ALG_CHIRP235) return AF_INVALID_ARG;
-    if (d->flags & ~AF_FLAG_PLAIN_TWIDDLES) return AF_INVALID_ARG;
+    if (d->flags & ~(AF_FLAG_PLAIN_TWIDDLES | AF_FLAG_COOLEY_TUKEY)) return AF_INVALID_ARG;
     switch (d->kind) {
     case AF_C2C: break;
     case AF_R2C_FULL:
@@ -776,6 +882,11 @@ af_status af_plan_describe(const af_plan_desc *desc, char *buf, size_t buflen) {
     if (st != AF_SUCCESS) s += "unsupported(" + why + ")";
     else if (real_kind && sh.algo != ALG_EMPTY && !real_fusable(*desc, sh)) s += "packed(" + describe_shape(sh) + ")";
     else s += describe_shape(sh);
+    if (st == AF_SUCCESS && desc->kind == AF_C2C && sh.algo == ALG_STOCKHAM && sh.core.kind == CORE_SINGLE &&
+        !(desc->flags & AF_FLAG_COOLEY_TUKEY) && prime_factor_radices(sh.core.r)) {
+        const size_t at = s.find("stockham");
+        if (at != std::string::npos) s.replace(at, 8, "primefactor");
+    }
     std::snprintf(buf, buflen, "%s", s.c_str());
     return st == AF_SUCCESS && s.size() < buflen ? AF_SUCCESS : (st != AF_SUCCESS ? st : AF_INVALID_ARG);
 }
@@ -899,7 +1010,10 @@ static af_status plan_create_impl(const af_plan_desc *desc, af_plan *plan, bool 
     } else if (sh.algo == ALG_DECOMP) {
         st = build_decomp(P.get(), s);
     } else if (sh.algo != ALG_EMPTY) {
-        st = build_core(P->prec, sh.core, P->core, P->own, s);
+        // plain transforms of one pass run in prime-factor order unless the
+        // caller asks for Cooley-Tukey
+        const bool pf = desc->kind == AF_C2C && sh.algo == ALG_STOCKHAM && !(desc->flags & AF_FLAG_COOLEY_TUKEY);
+        st = build_core(P->prec, sh.core, P->core, P->own, s, pf);
     }
     if (st == AF_SUCCESS && desc->kind == AF_C2C) {
         if (sh.algo == ALG_RADER) st = build_rader(P.get(), s);

@@ -98,5 +98,16 @@ struct AfPass {
     const void *tw_d;
     const void *ft_coarse_d;
     const void *ft_fine_d;
+    // Prime-factor (Good-Thomas) order of a generic pass (Planned.hs:584-617,
+    // for any number of pairwise coprime factors): with every radix of the
+    // stage list a power of a different prime, each stage is a factor of its
+    // own and no stage has twiddles (grp_mask: bit s set for every stage
+    // s > 0); element e of a line enters LDS at Stockham position gt_in[e]
+    // (the skew-grid input map) and output e leaves from position gt_out[e]
+    // (the Chinese-remainder output map).  nullptr / 0: Cooley-Tukey order.
+    const int32_t *gt_in;
+    const int32_t *gt_out;
+    int32_t grp_mask;
+    int32_t _pad1;
     double scale;
 };

@@ -144,7 +144,7 @@ static const int32_t kExtFlags = AFP_POSTTW | AFP_SCALE | AFP_CONV | AFP_SPEC_PL
 
 bool fast_ext_eligible(int prec, const AfPass &d) {
     if (d.flags & ~kExtFlags) return false;
-    if (d.in_map || d.out_map) return false;
+    if (d.in_map || d.out_map || d.gt_in || d.gt_out || d.grp_mask) return false;
     const bool conv = (d.flags & AFP_CONV) != 0;
     if (conv && (d.in_lines_fast || !d.spec)) return false;
     if (!ext_has_shape(prec, d.L, d.B, conv)) return false;

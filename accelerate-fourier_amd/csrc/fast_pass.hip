@@ -282,7 +282,7 @@ static const int32_t kFastFlags = AFP_POSTTW | AFP_SCALE;
 
 bool fast_pass_eligible(int prec, const AfPass &d) {
     if (d.flags & ~kFastFlags) return false;
-    if (d.in_map || d.out_map || d.in_mul || d.out_mul) return false;
+    if (d.in_map || d.out_map || d.in_mul || d.out_mul || d.gt_in || d.gt_out || d.grp_mask) return false;
     if (d.in_len != INT64_MAX || d.out_len != INT64_MAX) return false;
     if (d.L < 256 || d.L > 16384 || (d.L & (d.L - 1))) return false;
     // the W_L table of the working precision feeds the LDS stage table at every length

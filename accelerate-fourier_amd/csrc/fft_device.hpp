@@ -261,10 +261,12 @@ __device__ __forceinline__ void divmod_small(int n, int d, float rcp, int &q, in
 // One Stockham stage of radix R over a tile of B lines of length L.
 // Butterfly i of a line reads elements i + j L/R and writes
 // (i - k) R + k + j p with k = i mod p, p = product of the previous radices.
+// notw: a prime-factor pass whose stages are pairwise coprime (each its own
+// group): no stage has twiddles.
 template <typename T, int R>
 __device__ __forceinline__ void stage(typename Cx<T>::C *lds, const int L, const int ld, const int B,
                                       const int p, const void *__restrict__ tw, const T s,
-                                      const int ps, const bool plain) {
+                                      const int ps, const bool plain, const bool notw) {
     using C = typename Cx<T>::C;
     constexpr int MB = (pass_ept(sizeof(T) == 8) + R - 1) / R;
     const int nbl = L / R;
@@ -296,7 +298,7 @@ __device__ __forceinline__ void stage(typename Cx<T>::C *lds, const int L, const
             const int i = ii[b];
             int q, k;
             divmod_small(i, p, rp, q, k);
-            if (p > 1) {
+            if (p > 1 && !notw) {
 #pragma unroll
                 for (int j = 1; j < R; ++j)
                     v[b][j] = twmul(v[b][j], gld<typename Cx<T>::W>(tw, j * k * tws), s, plain);
@@ -314,18 +316,19 @@ template <typename T, typename D>
 __device__ __forceinline__ void run_stages(typename Cx<T>::C *lds, const D &d, const T s) {
     const void *tw = d.tw;
     const bool plain = (d.flags & AFP_PLAIN_TW) != 0;
+    const bool notw = d.grp_mask != 0;
     for (int st = 0; st < d.nst; ++st) {
         const int R = d.radix[st], p = d.pst[st];
         switch (R) {
-        case 2: stage<T, 2>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
-        case 3: stage<T, 3>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
-        case 4: stage<T, 4>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
-        case 5: stage<T, 5>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
-        case 7: stage<T, 7>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
-        case 8: stage<T, 8>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
-        case 11: stage<T, 11>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
-        case 13: stage<T, 13>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
-        default: stage<T, 16>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain); break;
+        case 2: stage<T, 2>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
+        case 3: stage<T, 3>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
+        case 4: stage<T, 4>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
+        case 5: stage<T, 5>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
+        case 7: stage<T, 7>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
+        case 8: stage<T, 8>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
+        case 11: stage<T, 11>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
+        case 13: stage<T, 13>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
+        default: stage<T, 16>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
         }
     }
 }
@@ -533,7 +536,7 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
                 }
             }
         }
-        lds[pidx(b * ld + e, ps)] = v;
+        lds[pidx(b * ld + (d.gt_in ? gld<int32_t>(d.gt_in, e) : e), ps)] = v;
     }
     __syncthreads();
 
@@ -614,7 +617,7 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
         const int64_t r = tl.row[b], l = tl.line[b];
         const int64_t o = (int64_t)((uint32_t)l * (uint32_t)d.out_ls + (uint32_t)e * (uint32_t)d.out_es);
         if (o >= d.out_len) continue;
-        C v = lds[pidx(b * ld + e, ps)];
+        C v = lds[pidx(b * ld + (d.gt_out ? gld<int32_t>(d.gt_out, e) : e), ps)];
         if (flags & AFP_POSTTW) {
             // the lines of a four-step pass number N / L, so l * e < N: no reduction
             v = cmul(v, ft_twiddle_t(d, (uint64_t)((uint32_t)l * (uint32_t)e), T(0)));

@@ -73,6 +73,13 @@ typedef struct {
 
 /* FP32 stage twiddles and vectors as compensated hi+lo float pairs (default on). */
 #define AF_FLAG_PLAIN_TWIDDLES 0x1u   /* use hi parts only (for error studies) */
+/* Single-pass transforms whose length is a product of three or more pairwise
+ * coprime butterfly radices (2, 4, 8 or 16 and distinct primes 3 ... 13, e.g.
+ * 30, 60, 15015) run in prime-factor (Good-Thomas, Planned.hs:584-617) order by
+ * default: skew-grid input map, Chinese-remainder output map, no twiddles.
+ * This flag keeps the twiddled Cooley-Tukey order (Planned.hs:543-557)
+ * instead; the results agree to rounding. */
+#define AF_FLAG_COOLEY_TUKEY 0x2u
 
 typedef struct af_plan_s *af_plan;
 typedef struct af_batch_s *af_batch;

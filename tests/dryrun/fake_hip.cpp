@@ -24,6 +24,7 @@
 #include <mutex>
 #include <string>
 #include <vector>
+#include <numeric>
 
 #include "af_pass.hpp"
 
@@ -252,6 +253,24 @@ void check_generic_launch(const AfPass &d, int tsz, unsigned block, size_t shmem
     if ((int64_t)shmem < need) problem(k + ": dynamic LDS below what the tile indexes");
     const int64_t stat = 64 * (8 + 8 + 4) + 2 * 64 * csz;
     if ((int64_t)shmem + stat > 160 * 1024) problem(k + ": LDS above 160 KiB with the static arrays");
+    if (d.gt_in || d.gt_out || d.grp_mask) {
+        // prime-factor order: both maps, a group mask within the stages, plain passes only;
+        // the maps are permutations of the line (device memory is host memory here)
+        if (!d.gt_in || !d.gt_out || !d.grp_mask) problem(k + ": incomplete prime-factor descriptor");
+        if (d.grp_mask != (1 << d.nst) - 2) problem(k + ": prime-factor group mask differs from one stage per factor");
+        for (int a = 0; a < d.nst; ++a)
+            for (int b2 = a + 1; b2 < d.nst; ++b2)
+                if (std::gcd(d.radix[a], d.radix[b2]) != 1) problem(k + ": prime-factor radices not pairwise coprime");
+        if (d.flags & (AFP_CONV | AFP_POSTTW | kRarePassFlags)) problem(k + ": prime-factor order on a pass with packing, twiddles or a product");
+        if (in_alloc(d.gt_in, 0, (int64_t)d.L * 4, "gt_in", k) && in_alloc(d.gt_out, 0, (int64_t)d.L * 4, "gt_out", k)) {
+            std::vector<char> seen_in(d.L, 0), seen_out(d.L, 0);
+            for (int e = 0; e < d.L; ++e) {
+                const int32_t a = d.gt_in[e], b = d.gt_out[e];
+                if (a < 0 || a >= d.L || seen_in[a]++) { problem(k + ": gt_in is not a permutation"); break; }
+                if (b < 0 || b >= d.L || seen_out[b]++) { problem(k + ": gt_out is not a permutation"); break; }
+            }
+        }
+    }
     check_pass(d, tsz, k);
 }
 

@@ -135,6 +135,19 @@ def test_host_planner_descriptions():
     assert planned.describe(0) == "c2c.f32:empty"
     assert planned.describe(1) == "c2c.f32:copy1"
     assert planned.describe(8192, kind=_abi.REAL_CONV) == "realconv.f32:stockham4096[16,16,16]"
+    # single passes over pairwise coprime radices run in prime-factor order
+    # (Good-Thomas, Planned.hs:584-617) unless Cooley-Tukey is asked for;
+    # repeated primes (45 = 5 x 9, 4000 = 32 x 125) and convolutions keep it
+    assert planned.describe(30) == "c2c.f32:primefactor30[2,5,3]"
+    assert planned.describe(15) == "c2c.f32:stockham15[5,3]"  # two factors: Cooley-Tukey is faster
+    assert planned.describe(15015) == "c2c.f32:primefactor15015[13,11,7,5,3]"
+    assert planned.describe(1155, precision=_abi.F64) == "c2c.f64:primefactor1155[11,7,5,3]"
+    assert planned.describe(60) == "c2c.f32:primefactor60[4,5,3]"
+    assert planned.describe(30, flags=_abi.FLAG_COOLEY_TUKEY) == "c2c.f32:stockham30[2,5,3]"
+    assert planned.describe(45) == "c2c.f32:stockham45[5,3,3]"
+    assert planned.describe(4000) == "c2c.f32:stockham4000[16,2,5,5,5]"
+    assert planned.describe(60, kind=_abi.CONV_CYCLIC) == "convcyclic.f32:stockham60[4,5,3]"
+    assert planned.describe(61).startswith("c2c.f32:rader61(stockham60[4,5,3])")
     # a prime whose p-1 has a large factor: Rader would nest, so Bluestein; and
     # transformDecompose nests Rader through host-composed decompositions
     # (10006 = 2 x 5003: Good-Thomas; 5003 prime: Rader over 5002 = 82 x 61 ...)

@@ -108,7 +108,8 @@ def run_c2c(lib, n, sign, prec, batch, planar, alg=0):
     return st
 
 
-LENGTHS = sorted(set(list(range(0, 130)) + [255, 256, 257, 360, 1000, 1024, 2048, 4096, 4099, 5000, 8192, 16384,
+LENGTHS = sorted(set(list(range(0, 130)) + [240, 1155, 3003, 9240, 15015, 16016,  # prime-factor order
+                                            255, 256, 257, 360, 1000, 1024, 2048, 4096, 4099, 5000, 8192, 16384,
                                             16385, 20000, 32768, 65521, 65536, 100003, 114689, 2 * 65521,
                                             1 << 20, 1200 * 1200, 997637]))
 

@@ -93,6 +93,45 @@ def test_c2c_mid(n, prec):
     _check_c2c(n, 1, prec, "planar", rows=2)
 
 
+# products of three or more pairwise coprime radices (2^a, a <= 4, and
+# distinct primes up to 13): single passes in prime-factor order; FP64 single
+# passes end at 8192
+PRIME_FACTOR_LENGTHS = [30, 42, 60, 105, 240, 1155, 3003, 4620, 9240, 15015, 16016]
+
+
+@pytest.mark.parametrize("n", PRIME_FACTOR_LENGTHS)
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_prime_factor_order(n, prec):
+    """Good-Thomas (Planned.hs:584-617) inside the single LDS pass: the
+    skew-grid input map and the Chinese-remainder output map applied as LDS
+    positions, no twiddles between the coprime factors.  Both signs, both
+    layouts, against the oracle (whose planner takes Good-Thomas for the odd
+    coprime lengths, Planned.hs:263-270) at the FP64 / FP32 tolerances, and the
+    same lengths in Cooley-Tukey order (AF_FLAG_COOLEY_TUKEY) likewise; the two
+    orders agree to rounding but not bit for bit, so both paths ran."""
+    p = _abi.F64 if prec == "f64" else _abi.F32
+    single = n <= (8192 if prec == "f64" else 16384)
+    assert planned.describe(n, precision=p).split(":")[1].startswith("primefactor" if single else "fourstep")
+    x = synthetic.complex_uniform(300 + n, (3, n))
+    rt = torch.float64 if prec == "f64" else torch.float32
+    ct = torch.complex128 if prec == "f64" else torch.complex64
+    tol = 1e-12 if prec == "f64" else 1e-6
+    for sign in (-1, 1):
+        ref = orc_transform(sign, x)
+        outs = []
+        for cooley_tukey in (False, True):
+            t = planned.Transform(Sign(sign), n, cooley_tukey=cooley_tukey)
+            got_i = host(t(to_dev(x, ct)))
+            yp = t((to_dev(x.real, rt), to_dev(x.imag, rt)))
+            got_p = host(yp[0]) + 1j * host(yp[1])
+            for name, got in (("interleaved", got_i), ("planar", got_p)):
+                err = rel_l2(got, ref)
+                assert err <= tol, (n, prec, sign, cooley_tukey, name, err)
+            outs.append(got_i)
+        if single:
+            assert not np.array_equal(outs[0], outs[1]), "the two orders gave identical bits"
+
+
 @pytest.mark.parametrize("n", LENGTHS_BIG)
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_c2c_big(n, prec):
