@@ -45,7 +45,7 @@ class UnevenBatch:
         _abi.check(_abi.lib().af_batch_describe(self.handle, buf, len(buf)), "af_batch_describe")
         return buf.value.decode()
 
-    def set_schedule(self, chunk_bytes: int = 0, lanes: int = 1) -> None:
+    def set_schedule(self, chunk_bytes: int = 0, lanes: int = 2) -> None:
         """af_batch_set_schedule: rows with intermediates in chunks of at most
         chunk_bytes of them (0: no chunking); lanes = 2 runs the rows without
         intermediates on a parallel branch of the graph."""

@@ -2448,9 +2448,11 @@ af_status batch_upload(af_batch B, const void *in_re, const void *in_im, void *o
 // from re-reading cache-resident intermediates as much as the extra launches
 // and their partly filled last rounds cost.  af_batch_tune measures it again
 // for each batch; chunking also bounds the scratch memory (5.7 GB unchunked,
-// the chunk budget chunked).
+// the chunk budget chunked).  Default lanes: two (the rows without
+// intermediates on a branch of their own, whose small launches then overlap
+// the large ones): 10.32 against 10.61 ms on that batch, in two runs.
 constexpr int64_t kBatchChunkBytes = 0;
-constexpr int kBatchLanes = 1;
+constexpr int kBatchLanes = 2;
 
 }  // namespace
 
@@ -2542,13 +2544,16 @@ af_status af_batch_tune(af_batch B, void *stream, char *report, size_t report_le
     HIPCHK(hipEventCreate(&e1));
     const int64_t start = B->chunk_bytes;
     const int start_lanes = B->lanes;
-    const std::pair<int64_t, int> candidates[] = {{0, 1},
-                                                  {0, 2},
-                                                  {(int64_t)32 << 20, 2},
-                                                  {(int64_t)64 << 20, 2},
-                                                  {(int64_t)128 << 20, 2},
-                                                  {(int64_t)192 << 20, 2},
-                                                  {(int64_t)384 << 20, 2}};
+    // the batch's current schedule first: it is the incumbent the others must beat
+    std::vector<std::pair<int64_t, int>> candidates = {{start, start_lanes}};
+    for (const std::pair<int64_t, int> c : {std::pair<int64_t, int>{0, 1},
+                                            {0, 2},
+                                            {(int64_t)32 << 20, 2},
+                                            {(int64_t)64 << 20, 2},
+                                            {(int64_t)128 << 20, 2},
+                                            {(int64_t)192 << 20, 2},
+                                            {(int64_t)384 << 20, 2}})
+        if (c != candidates[0]) candidates.push_back(c);
     std::string text;
     int64_t best = start;
     int best_lanes = start_lanes;

@@ -343,7 +343,7 @@ af_status af_batch_destroy(af_batch batch);
  * row needing more is a chunk of its own), and lanes = 2 to run the rows
  * without intermediates on a branch of the graph beside the others (their
  * small launches then overlap the large ones) or 1 for one sequence.
- * Recaptures the graph.  The default is (0, 1).
+ * Recaptures the graph.  The default is (0, 2).
  */
 af_status af_batch_set_schedule(af_batch batch, int64_t chunk_bytes, int32_t lanes);
 /*
