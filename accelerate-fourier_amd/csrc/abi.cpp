@@ -1932,11 +1932,12 @@ af_status karatsuba_leaves(std::vector<int64_t> meta, const int64_t *x, const in
     std::vector<int64_t> sorted(3 * npairs);
     for (int64_t i = 0; i < npairs; ++i)
         for (int j = 0; j < 3; ++j) sorted[3 * i + j] = meta[3 * order[i] + j];
-    void *dm = nullptr;
-    HIPCHK(hipMallocAsync(&dm, sorted.size() * 8, s));
+    void *dm = nullptr;  // the pair table, then one int32 per pair for the kernels' hand-over
+    HIPCHK(hipMallocAsync(&dm, sorted.size() * 8 + (size_t)npairs * 4, s));
     af_status st = stage_to_device(dm, sorted.data(), sorted.size() * 8, s);
     if (st == AF_SUCCESS)
-        st = HIPST(launch_karatsuba(x, y, z, (const int64_t *)dm, npairs, (int)maxP, cyclic, s));
+        st = HIPST(launch_karatsuba(x, y, z, (const int64_t *)dm, npairs, (int)maxP, cyclic,
+                                    (int32_t *)((char *)dm + sorted.size() * 8), s));
     (void)hipFreeAsync(dm, s);
     return st;
 }

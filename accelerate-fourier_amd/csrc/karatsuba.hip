@@ -85,24 +85,45 @@ __host__ __device__ inline int karatsuba_lds_words(int P, int s, int B) {
     return 2 * P + 2 * Q + n3 * 2 * s + (B ? n3 / 3 * 4 * s : 0) + kKarWaves * 128;
 }
 
-// meta[3p] = input offset, meta[3p+1] = length, meta[3p+2] = output offset
+// Pair geometry: P = 2^d s >= Lp with s <= 64, bottom levels B = min(d, 3),
+// top levels T = d - B, subtree operand length Q = 2^B s.
+__device__ __forceinline__ void kar_geometry(int64_t Lp, int &d, int &s, int &P, int &B, int &T, int &Q) {
+    d = 0;
+    int64_t s64 = Lp;
+    while (s64 > 64) {
+        ++d;
+        s64 = (Lp + ((int64_t)1 << d) - 1) >> d;
+    }
+    s = (int)s64;
+    P = s << d;
+    B = d < 3 ? d : 3;
+    T = d - B;
+    Q = s << B;
+}
+
+// The pairs the MFMA kernel takes (the other kernel then skips them): every
+// subtree product exact in FP64.  Subtree operands are sums of up to 2^T
+// coefficients, so |X| < 2^(bits(mx) + T), |Y| < 2^(bits(my) + T), and every
+// partial sum of the Q-term schoolbook products stays below
+// 2^(bits(mx) + bits(my) + 2T + bits(Q)) <= 2^53.
+__device__ __forceinline__ bool kar_mfma_takes(u64 mx, u64 my, int T, int Q) {
+    const int bits = (64 - __clzll((long long)mx)) + (64 - __clzll((long long)my)) + 2 * T + (32 - __clz(Q));
+    return bits <= 53;
+}
+
+// meta[3p] = input offset, meta[3p+1] = length, meta[3p+2] = output offset;
+// taken (may be null): pairs k_karatsuba_mfma has done (taken[p] != 0) are skipped
 __global__ void __launch_bounds__(kKarThreads)
 k_karatsuba(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__restrict__ z,
-            const int64_t *__restrict__ meta, int64_t npairs, int cyclic) {
+            const int64_t *__restrict__ meta, int64_t npairs, int cyclic, const int32_t *__restrict__ taken) {
     extern __shared__ __align__(16) u64 sm[];
     const int tid = threadIdx.x, NT = kKarThreads;
     const int lane = tid & 63, wid = tid >> 6;
     for (int64_t pi = blockIdx.x; pi < npairs; pi += gridDim.x) {
         const int64_t off = meta[3 * pi], Lp = meta[3 * pi + 1], zoff = meta[3 * pi + 2];
-        if (Lp <= 0) continue;
-        int d = 0;
-        int64_t s64 = Lp;
-        while (s64 > 64) {
-            ++d;
-            s64 = (Lp + ((int64_t)1 << d) - 1) >> d;
-        }
-        const int s = (int)s64, P = s << d;
-        const int B = d < 3 ? d : 3, T = d - B, Q = s << B;
+        if (Lp <= 0 || (taken && taken[pi])) continue;
+        int d, s, P, B, T, Q;
+        kar_geometry(Lp, d, s, P, B, T, Q);
         int nleaf = 1, nsub = 1;
         for (int i = 0; i < B; ++i) nleaf *= 3;
         for (int i = 0; i < T; ++i) nsub *= 3;
@@ -284,6 +305,212 @@ k_karatsuba(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// Karatsuba with matrix-core subtree products (pairs that kar_mfma_takes accepts)
+// ---------------------------------------------------------------------------
+//
+// The top T levels are walked as in k_karatsuba, but each subtree's product --
+// two operands of length Q = 2^B s, integers small enough that FP64 holds every
+// partial sum exactly -- is one schoolbook product on the FP64 matrix cores
+// instead of B more Karatsuba levels and 3^B leaf products.  With y cut into
+// 16-blocks y_b, output block c of z = x y is
+//   z_c[r] = sum_delta sum_{q<16} T_delta[r][q] y_{c-delta}[q],
+//   T_delta[r][q] = x[16 delta + r - q]  (zero outside the operand),
+// so a tile of 16 output blocks c0 ... c0 + 15 is, for each delta, the 16 x 16
+// Toeplitz block T_delta times the 16 x 16 matrix of y blocks c0 - delta ...
+// c0 + 15 - delta: four v_mfma_f64_16x16x4 per (tile, delta).  Both operands
+// come from zero-padded LDS copies (y skewed to 17 words per block, so the
+// lanes of one MFMA operand fall in different banks) at fixed offsets.  The
+// (tile, delta) units are dealt to the 16 wavefronts in contiguous runs; a
+// wavefront stages each finished tile in LDS and adds it, converted to Int64
+// (exact), into the LDS result once per top-level term with 64-bit LDS atomics
+// on consecutive words.  The atomics commute, so no barrier orders the terms,
+// and over the ring of 64-bit words the bits equal every other exact evaluation.
+typedef double kar_v4d __attribute__((ext_vector_type(4)));
+constexpr int kKarMfmaMaxTerms = 1 << (7 - 3);  // 2^T, T <= 4 for P <= 4608
+constexpr int kKarPadX = 16;                      // zero words before x
+constexpr int kKarPadB = 16;                      // zero blocks before y
+constexpr int kKarStage = 16 * 17;                // one wavefront's tile, skewed
+
+// LDS words of one pair: result 2P, x (Qp + 32), skewed y ((Qp/16 + 32) 17),
+// the wavefronts' tiles, the term table and the reduction slots.
+__host__ __device__ inline int karatsuba_mfma_lds_words(int P, int Q) {
+    const int Qp = (Q + 15) & ~15;
+    return 2 * P + (Qp + 32) + (Qp / 16 + 32) * 17 + kKarWaves * kKarStage + 2 * kKarMfmaMaxTerms + 2 * kKarWaves;
+}
+
+// taken[p] = 1 for the pairs done here, 0 for those left to k_karatsuba
+__global__ void __launch_bounds__(kKarThreads)
+k_karatsuba_mfma(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__restrict__ z,
+                 const int64_t *__restrict__ meta, int64_t npairs, int cyclic, int32_t *__restrict__ taken) {
+    extern __shared__ __align__(16) u64 sm[];
+    const int tid = threadIdx.x, NT = kKarThreads;
+    // the wavefront index as a scalar: the unit runs and their loops are uniform
+    const int lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int r16 = lane & 15, k4 = lane >> 4;  // MFMA operand row / column and K slot
+    for (int64_t pi = blockIdx.x; pi < npairs; pi += gridDim.x) {
+        const int64_t off = meta[3 * pi], Lp = meta[3 * pi + 1], zoff = meta[3 * pi + 2];
+        if (Lp <= 0) continue;
+        int d, s, P, B, T, Q;
+        kar_geometry(Lp, d, s, P, B, T, Q);
+        const int Qp = (Q + 15) & ~15, nb = Qp / 16;
+        const int nxs = Qp + 32, nys = (nb + 32) * 17;
+        u64 *acc = sm;
+        double *xs = (double *)(acc + 2 * P), *ys = xs + nxs, *stage = ys + nys + wid * kKarStage;
+        int64_t *tshift = (int64_t *)(ys + nys + kKarWaves * kKarStage);
+        int64_t *tneg = tshift + kKarMfmaMaxTerms;
+        u64 *red = (u64 *)(tneg + kKarMfmaMaxTerms);
+        u64 xb[kKarMaxBlocks], yb[kKarMaxBlocks];
+        u64 mx = 0, my = 0;
+#pragma unroll
+        for (int b = 0; b < kKarMaxBlocks; ++b) {
+            const int64_t idx = (int64_t)b * Q + tid;
+            const bool in = b < (1 << T) && tid < Q && idx < Lp;
+            xb[b] = in ? x[off + idx] : 0ull;
+            yb[b] = in ? y[off + idx] : 0ull;
+            mx = max(mx, (int64_t)xb[b] < 0 ? 0ull - xb[b] : xb[b]);
+            my = max(my, (int64_t)yb[b] < 0 ? 0ull - yb[b] : yb[b]);
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+            mx = max(mx, (u64)__shfl_xor((unsigned long long)mx, o));
+            my = max(my, (u64)__shfl_xor((unsigned long long)my, o));
+        }
+        if (lane == 0) {
+            red[wid] = mx;
+            red[kKarWaves + wid] = my;
+        }
+        __syncthreads();
+        mx = my = 0;
+        for (int w = 0; w < kKarWaves; ++w) {
+            mx = max(mx, red[w]);
+            my = max(my, red[kKarWaves + w]);
+        }
+        const bool take = kar_mfma_takes(mx, my, T, Q);
+        if (tid == 0) taken[pi] = take ? 1 : 0;
+        if (!take) {
+            __syncthreads();  // every thread has read red; k_karatsuba takes this pair
+            continue;
+        }
+        for (int i = tid; i < 2 * P; i += NT) acc[i] = 0ull;
+        for (int i = tid; i < nxs + nys; i += NT) xs[i] = 0.0;  // the padding stays zero
+        int nsub = 1;
+        for (int i = 0; i < T; ++i) nsub *= 3;
+        // the (tile, delta) units of one subtree product: tiles of 16 output
+        // blocks c0 ..., delta in [c0 - nb + 1, c0 + 15] within [0, nb]
+        const int ntile = (2 * nb + 15) / 16;
+        int nunit = 0;
+        for (int tt = 0; tt < ntile; ++tt) {
+            const int c0 = 16 * tt, dlo = max(0, c0 - nb + 1), dhi = min(nb, c0 + 15);
+            nunit += dhi >= dlo ? dhi - dlo + 1 : 0;
+        }
+        const int u_begin = (int)((int64_t)nunit * wid / kKarWaves), u_end = (int)((int64_t)nunit * (wid + 1) / kKarWaves);
+        for (int sub = 0; sub < nsub; ++sub) {
+            int tbase, tfree;
+            digits3(sub, T, tbase, tfree);
+            __syncthreads();  // the previous subtree's operand reads are done
+            if (tid < Qp) {
+                u64 X = 0, Y = 0;
+                if (tid < Q) {
+#pragma unroll
+                    for (int b = 0; b < kKarMaxBlocks; ++b) {
+                        const bool sel = (b & ~tfree) == tbase;
+                        X += sel ? xb[b] : 0ull;
+                        Y += sel ? yb[b] : 0ull;
+                    }
+                }
+                xs[kKarPadX + tid] = (double)(int64_t)X;
+                ys[(kKarPadB + (tid >> 4)) * 17 + (tid & 15)] = (double)(int64_t)Y;
+            }
+            // the subtree's top-level terms (shift, sign) of prod_i c_{t_i}(u_i)
+            const int nvar = T - __popc(tfree);
+            if (tid < (1 << nvar)) {
+                const int c = tid;
+                int64_t shift = 0;
+                bool neg = false;
+                int v = 0;
+                for (int lev = 1; lev <= T; ++lev) {
+                    const int64_t hl = (int64_t)s << (d - lev);
+                    const int bit = 1 << (T - lev);
+                    const int t = (tbase & bit) ? 2 : ((tfree & bit) ? 1 : 0);
+                    if (t == 1) {
+                        shift += hl;
+                    } else {
+                        const int bc = (c >> v) & 1;
+                        ++v;
+                        if (t == 0) {
+                            if (bc) { shift += hl; neg = !neg; }
+                        } else {
+                            if (bc) shift += 2 * hl;
+                            else { shift += hl; neg = !neg; }
+                        }
+                    }
+                }
+                tshift[c] = shift;
+                tneg[c] = neg ? 1 : 0;
+            }
+            __syncthreads();
+            const int nterm = 1 << nvar;
+            kar_v4d D = {0.0, 0.0, 0.0, 0.0};
+            int cur_tile = -1;
+            // a finished tile: D holds output coefficient 16 (c0 + col) + row,
+            // row = k4 + 4 v, col = r16; staged, then added with consecutive lanes
+            // on consecutive coefficients
+            auto flush = [&](int tile) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) stage[r16 * 17 + k4 + 4 * v] = D[v];
+                wave_sync();
+                const int base = 256 * tile;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int e = lane + 64 * q, kk = base + e;
+                    const u64 val = (u64)(int64_t)stage[(e >> 4) * 17 + (e & 15)];
+                    if (kk < 2 * Q - 1 && val != 0ull) {
+                        for (int c = 0; c < nterm; ++c) {
+                            const u64 w = tneg[c] ? (u64)0 - val : val;
+                            atomicAdd((unsigned long long *)&acc[tshift[c] + kk], (unsigned long long)w);
+                        }
+                    }
+                }
+                wave_sync();
+            };
+            int u = 0;
+            for (int tt = 0; tt < ntile; ++tt) {
+                const int c0 = 16 * tt, dlo = max(0, c0 - nb + 1), dhi = min(nb, c0 + 15);
+                if (dhi < dlo) continue;
+                const int cnt = dhi - dlo + 1;
+                const int lo = max(u, u_begin), hi = min(u + cnt, u_end);
+                if (lo < hi) {
+                    if (cur_tile != tt) {
+                        if (cur_tile >= 0) flush(cur_tile);
+                        D = (kar_v4d){0.0, 0.0, 0.0, 0.0};
+                        cur_tile = tt;
+                    }
+                    for (int dl = dlo + (lo - u); dl < dlo + (hi - u); ++dl) {
+                        // A: T_delta[r16][q] = x[16 delta + r16 - q], q = 4 kap + k4
+                        const double *xa = xs + kKarPadX + 16 * dl + r16 - k4 - 12;
+                        // B: y block c0 + r16 - delta, element q
+                        const double *yv = ys + (c0 + r16 - dl + kKarPadB) * 17 + k4;
+                        D = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[12], yv[0], D, 0, 0, 0);
+                        D = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[8], yv[4], D, 0, 0, 0);
+                        D = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[4], yv[8], D, 0, 0, 0);
+                        D = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[0], yv[12], D, 0, 0, 0);
+                    }
+                }
+                u += cnt;
+            }
+            if (cur_tile >= 0) flush(cur_tile);
+        }
+        __syncthreads();
+        if (cyclic) {
+            for (int64_t k = tid; k < Lp; k += NT)
+                z[zoff + k] = acc[k] + ((k + Lp < 2 * Lp - 1) ? acc[k + Lp] : 0ull);
+        } else {
+            for (int64_t k = tid; k < 2 * Lp - 1; k += NT) z[zoff + k] = acc[k];
+        }
+        __syncthreads();
+    }
+}
+
 // Convolution.Adhoc.complex (Convolution/Adhoc.hs:114-133) in the Gauss form:
 // three real products xr*yr, xi*yi and (xr + xi)*(yr + yi).  The split writes
 // the three operand pairs as consecutive blocks of `count` words; the join
@@ -407,10 +634,11 @@ hipError_t launch_gauss_join(const int64_t *p3, int64_t *zr, int64_t *zi, int64_
 
 void init_karatsuba_attributes() {
     (void)hipFuncSetAttribute((const void *)k_karatsuba, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void *)k_karatsuba_mfma, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
 
 hipError_t launch_karatsuba(const int64_t *x, const int64_t *y, int64_t *z, const int64_t *dev_meta, int64_t npairs,
-                            int max_P, int cyclic, hipStream_t s) {
+                            int max_P, int cyclic, int32_t *dev_taken, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
     // LDS of the largest pair: P = 2^d s with s <= 64
     int words = 0;
@@ -420,8 +648,20 @@ hipError_t launch_karatsuba(const int64_t *x, const int64_t *y, int64_t *z, cons
     const size_t lds = (size_t)words * sizeof(u64);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const int64_t g = npairs < (int64_t)1 << 30 ? npairs : (int64_t)1 << 30;
+    // pairs with small enough values first, on the matrix cores; the other
+    // kernel skips those (dev_taken: one int32 per pair)
+    int mwords = 0;
+    for (int d = 0; (1 << d) <= max_P; ++d)
+        for (int sl = 1; sl <= 64; ++sl)
+            if ((sl << d) <= max_P) mwords = std::max(mwords, karatsuba_mfma_lds_words(sl << d, sl << (d < 3 ? d : 3)));
+    const size_t mlds = (size_t)mwords * sizeof(u64);
+    if (mlds > 160 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_karatsuba_mfma, dim3((unsigned)g), dim3(kKarThreads), mlds, s, (const u64 *)x,
+                       (const u64 *)y, (u64 *)z, dev_meta, npairs, cyclic, dev_taken);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_karatsuba, dim3((unsigned)g), dim3(kKarThreads), lds, s, (const u64 *)x, (const u64 *)y,
-                       (u64 *)z, dev_meta, npairs, cyclic);
+                       (u64 *)z, dev_meta, npairs, cyclic, (const int32_t *)dev_taken);
     return hipGetLastError();
 }
 

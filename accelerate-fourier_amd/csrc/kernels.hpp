@@ -86,9 +86,11 @@ hipError_t launch_dc_spec_mul(int prec, void *x, const void *spec, int64_t L, in
 hipError_t launch_dc_rader_out(int prec, const void *x_re, const void *x_im, const void *c, const int32_t *pw,
                                int64_t p, int64_t rows, void *out_re, void *out_im, hipStream_t s);
 
-// Karatsuba (karatsuba.hip)
+// Karatsuba (karatsuba.hip): the pairs whose values keep every partial sum
+// exact in FP64 on the matrix cores, the rest in 64-bit integers; dev_taken
+// holds one int32 per pair (device scratch, written by the first kernel)
 hipError_t launch_karatsuba(const int64_t *x, const int64_t *y, int64_t *z, const int64_t *dev_meta, int64_t npairs,
-                            int max_P, int cyclic, hipStream_t s);
+                            int max_P, int cyclic, int32_t *dev_taken, hipStream_t s);
 hipError_t launch_gauss_split(const int64_t *xr, const int64_t *xi, const int64_t *yr, const int64_t *yi, int64_t *x3,
                               int64_t *y3, int64_t count, hipStream_t s);
 hipError_t launch_gauss_join(const int64_t *p3, int64_t *zr, int64_t *zi, int64_t count, hipStream_t s);

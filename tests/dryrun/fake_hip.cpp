@@ -442,14 +442,20 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
         };
     }
     if (name.find("af::k_karatsuba") == 0) {
+        // k_karatsuba and k_karatsuba_mfma: the same arguments (the first has one
+        // more, skip_mfma) and LDS layouts of their own
+        const bool mfma = name.find("af::k_karatsuba_mfma") == 0;
         const void *x = *(const void *const *)args[0];
         const void *y = *(const void *const *)args[1];
         void *z = *(void **)args[2];
         const int64_t *meta = *(const int64_t *const *)args[3];
         const int64_t np = *(const int64_t *)args[4];
         const int cyc = *(const int *)args[5];
+        const void *taken = *(const void *const *)args[6];  // one int32 per pair
         return [=] {
             if (!in_alloc(meta, 0, np * 24, "meta", name)) return;
+            if (taken && !in_alloc(taken, 0, np * 4, "taken", name)) return;
+            if (mfma && !taken) problem(name + ": no hand-over array");
             for (int64_t p = 0; p < np; ++p) {
                 const int64_t off = meta[3 * p], L = meta[3 * p + 1], zo = meta[3 * p + 2];
                 if (L <= 0) continue;
@@ -466,7 +472,9 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
                 const int64_t P = sl << d, B = d < 3 ? d : 3;
                 int64_t n3 = 1;
                 for (int i = 0; i < B; ++i) n3 *= 3;
-                const int64_t words = 2 * P + 2 * (sl << B) + n3 * 2 * sl + (B ? n3 / 3 * 4 * sl : 0) + 16 * 128;
+                const int64_t Qp = ((sl << B) + 15) / 16 * 16;
+                const int64_t words = mfma ? 2 * P + (Qp + 32) + (Qp / 16 + 32) * 17 + 16 * 16 * 17 + 2 * 16 + 2 * 16
+                                           : 2 * P + 2 * (sl << B) + n3 * 2 * sl + (B ? n3 / 3 * 4 * sl : 0) + 16 * 128;
                 if (words * 8 > (int64_t)shmem)
                     problem(name + ": a pair needs more LDS than the launch has (L=" + std::to_string(L) + " need " +
                             std::to_string(words * 8) + " have " + std::to_string(shmem) + ")");
