@@ -102,13 +102,19 @@ __device__ __forceinline__ void kar_geometry(int64_t Lp, int &d, int &s, int &P,
 }
 
 // The pairs the MFMA kernel takes (the other kernel then skips them): every
-// subtree product exact in FP64.  Subtree operands are sums of up to 2^T
-// coefficients, so |X| < 2^(bits(mx) + T), |Y| < 2^(bits(my) + T), and every
-// partial sum of the Q-term schoolbook products stays below
-// 2^(bits(mx) + bits(my) + 2T + bits(Q)) <= 2^53.
+// subtree product exact in FP64 and convertible back by the 2^52 + 2^51 bias.
+// Subtree operands are sums of up to 2^T coefficients, so |X| < 2^(bits(mx) + T),
+// |Y| < 2^(bits(my) + T), and every partial sum of the Q-term schoolbook
+// products stays below 2^(bits(mx) + bits(my) + 2T + bits(Q)) <= 2^51.
 __device__ __forceinline__ bool kar_mfma_takes(u64 mx, u64 my, int T, int Q) {
     const int bits = (64 - __clzll((long long)mx)) + (64 - __clzll((long long)my)) + 2 * T + (32 - __clz(Q));
-    return bits <= 53;
+    return bits <= 51;
+}
+
+// An integer-valued double of magnitude below 2^51 as Int64: added to
+// 2^52 + 2^51 its value sits in the low mantissa bits, exactly.
+__device__ __forceinline__ u64 kar_f64_to_i64(double v) {
+    return (u64)__double_as_longlong(v + 6755399441055744.0) - (u64)0x4338000000000000ull;
 }
 
 // meta[3p] = input offset, meta[3p+1] = length, meta[3p+2] = output offset;
@@ -360,16 +366,19 @@ k_karatsuba_mfma(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__re
         int64_t *tshift = (int64_t *)(ys + nys + kKarWaves * kKarStage);
         int64_t *tneg = tshift + kKarMfmaMaxTerms;
         u64 *red = (u64 *)(tneg + kKarMfmaMaxTerms);
-        u64 xb[kKarMaxBlocks], yb[kKarMaxBlocks];
+        // element tid of each operand block, as doubles (exact for the pairs
+        // taken; the others leave before using them)
+        double xb[kKarMaxBlocks], yb[kKarMaxBlocks];
         u64 mx = 0, my = 0;
 #pragma unroll
         for (int b = 0; b < kKarMaxBlocks; ++b) {
             const int64_t idx = (int64_t)b * Q + tid;
             const bool in = b < (1 << T) && tid < Q && idx < Lp;
-            xb[b] = in ? x[off + idx] : 0ull;
-            yb[b] = in ? y[off + idx] : 0ull;
-            mx = max(mx, (int64_t)xb[b] < 0 ? 0ull - xb[b] : xb[b]);
-            my = max(my, (int64_t)yb[b] < 0 ? 0ull - yb[b] : yb[b]);
+            const u64 xv = in ? x[off + idx] : 0ull, yv = in ? y[off + idx] : 0ull;
+            mx = max(mx, (int64_t)xv < 0 ? 0ull - xv : xv);
+            my = max(my, (int64_t)yv < 0 ? 0ull - yv : yv);
+            xb[b] = (double)(int64_t)xv;
+            yb[b] = (double)(int64_t)yv;
         }
         for (int o = 32; o >= 1; o >>= 1) {
             mx = max(mx, (u64)__shfl_xor((unsigned long long)mx, o));
@@ -409,17 +418,17 @@ k_karatsuba_mfma(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__re
             digits3(sub, T, tbase, tfree);
             __syncthreads();  // the previous subtree's operand reads are done
             if (tid < Qp) {
-                u64 X = 0, Y = 0;
-                if (tid < Q) {
+                // the selected blocks' sum: one FMA per block with a uniform 0/1
+                // weight (exact: integers below 2^51)
+                double X = 0.0, Y = 0.0;
 #pragma unroll
-                    for (int b = 0; b < kKarMaxBlocks; ++b) {
-                        const bool sel = (b & ~tfree) == tbase;
-                        X += sel ? xb[b] : 0ull;
-                        Y += sel ? yb[b] : 0ull;
-                    }
+                for (int b = 0; b < kKarMaxBlocks; ++b) {
+                    const double w = (b & ~tfree) == tbase ? 1.0 : 0.0;
+                    X = fma(w, xb[b], X);
+                    Y = fma(w, yb[b], Y);
                 }
-                xs[kKarPadX + tid] = (double)(int64_t)X;
-                ys[(kKarPadB + (tid >> 4)) * 17 + (tid & 15)] = (double)(int64_t)Y;
+                xs[kKarPadX + tid] = X;
+                ys[(kKarPadB + (tid >> 4)) * 17 + (tid & 15)] = Y;
             }
             // the subtree's top-level terms (shift, sign) of prod_i c_{t_i}(u_i)
             const int nvar = T - __popc(tfree);
@@ -460,16 +469,22 @@ k_karatsuba_mfma(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__re
                 for (int v = 0; v < 4; ++v) stage[r16 * 17 + k4 + 4 * v] = D[v];
                 wave_sync();
                 const int base = 256 * tile;
+                u64 val[4], nval[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int e = lane + 64 * q, kk = base + e;
-                    const u64 val = (u64)(int64_t)stage[(e >> 4) * 17 + (e & 15)];
-                    if (kk < 2 * Q - 1 && val != 0ull) {
-                        for (int c = 0; c < nterm; ++c) {
-                            const u64 w = tneg[c] ? (u64)0 - val : val;
-                            atomicAdd((unsigned long long *)&acc[tshift[c] + kk], (unsigned long long)w);
-                        }
-                    }
+                    const int e = lane + 64 * q;
+                    val[q] = base + e < 2 * Q - 1 ? kar_f64_to_i64(stage[(e >> 4) * 17 + (e & 15)]) : 0ull;
+                    nval[q] = (u64)0 - val[q];
+                }
+                for (int c = 0; c < nterm; ++c) {
+                    // the term's shift and sign, uniform: scalar registers
+                    const int64_t sh = __builtin_amdgcn_readfirstlane((int)tshift[c]);
+                    const bool ng = __builtin_amdgcn_readfirstlane((int)tneg[c]) != 0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (val[q] != 0ull)
+                            atomicAdd((unsigned long long *)&acc[sh + base + lane + 64 * q],
+                                      (unsigned long long)(ng ? nval[q] : val[q]));
                 }
                 wave_sync();
             };
