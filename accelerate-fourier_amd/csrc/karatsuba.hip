@@ -366,19 +366,23 @@ k_karatsuba_mfma(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__re
         int64_t *tshift = (int64_t *)(ys + nys + kKarWaves * kKarStage);
         int64_t *tneg = tshift + kKarMfmaMaxTerms;
         u64 *red = (u64 *)(tneg + kKarMfmaMaxTerms);
-        // element tid of each operand block, as doubles (exact for the pairs
-        // taken; the others leave before using them)
-        double xb[kKarMaxBlocks], yb[kKarMaxBlocks];
+        // element e of each block of one operand, as doubles (exact for the pairs
+        // taken; the others leave before using them): threads e < Qp hold x,
+        // threads Qp + e hold y
+        const bool hold_y = tid >= Qp;
+        const int e = hold_y ? tid - Qp : tid;
+        const u64 *src = hold_y ? y : x;
+        double ob[kKarMaxBlocks];
         u64 mx = 0, my = 0;
 #pragma unroll
         for (int b = 0; b < kKarMaxBlocks; ++b) {
-            const int64_t idx = (int64_t)b * Q + tid;
-            const bool in = b < (1 << T) && tid < Q && idx < Lp;
-            const u64 xv = in ? x[off + idx] : 0ull, yv = in ? y[off + idx] : 0ull;
-            mx = max(mx, (int64_t)xv < 0 ? 0ull - xv : xv);
-            my = max(my, (int64_t)yv < 0 ? 0ull - yv : yv);
-            xb[b] = (double)(int64_t)xv;
-            yb[b] = (double)(int64_t)yv;
+            const int64_t idx = (int64_t)b * Q + e;
+            const bool in = b < (1 << T) && e < Q && tid < 2 * Qp && idx < Lp;
+            const u64 v = in ? src[off + idx] : 0ull;
+            const u64 a = (int64_t)v < 0 ? 0ull - v : v;
+            if (hold_y) my = max(my, a);
+            else mx = max(mx, a);
+            ob[b] = (double)(int64_t)v;
         }
         for (int o = 32; o >= 1; o >>= 1) {
             mx = max(mx, (u64)__shfl_xor((unsigned long long)mx, o));
@@ -417,18 +421,14 @@ k_karatsuba_mfma(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__re
             int tbase, tfree;
             digits3(sub, T, tbase, tfree);
             __syncthreads();  // the previous subtree's operand reads are done
-            if (tid < Qp) {
+            if (tid < 2 * Qp) {
                 // the selected blocks' sum: one FMA per block with a uniform 0/1
                 // weight (exact: integers below 2^51)
-                double X = 0.0, Y = 0.0;
+                double X = 0.0;
 #pragma unroll
-                for (int b = 0; b < kKarMaxBlocks; ++b) {
-                    const double w = (b & ~tfree) == tbase ? 1.0 : 0.0;
-                    X = fma(w, xb[b], X);
-                    Y = fma(w, yb[b], Y);
-                }
-                xs[kKarPadX + tid] = X;
-                ys[(kKarPadB + (tid >> 4)) * 17 + (tid & 15)] = Y;
+                for (int b = 0; b < kKarMaxBlocks; ++b) X = fma((b & ~tfree) == tbase ? 1.0 : 0.0, ob[b], X);
+                if (hold_y) ys[(kKarPadB + (e >> 4)) * 17 + (e & 15)] = X;
+                else xs[kKarPadX + e] = X;
             }
             // the subtree's top-level terms (shift, sign) of prod_i c_{t_i}(u_i)
             const int nvar = T - __popc(tfree);
