@@ -261,6 +261,40 @@ def test_karatsuba_ragged_mixed_magnitudes():
         assert np.array_equal(got, ref), (p, L, mags[p])
 
 
+def test_karatsuba_matrix_core_bound():
+    """The matrix-core kernel takes a pair when bits(max|x|) + bits(max|y|) +
+    2T + bits(Q) <= 51 (its subtree products stay exact in FP64 and convert
+    back by the 2^52 + 2^51 bias); the integer kernel takes the others.  Pairs
+    of length 1024 (P = 2^4 64: T = 1 top level, Q = 512) with magnitudes of
+    20 + 19 bits (51: matrix cores), 21 + 19 bits (52) and 20 + 20 bits (52:
+    integer kernel), in one ragged batch with a small pair, match exact Python
+    integer products bit for bit."""
+    L = 1024
+    rng = np.random.default_rng(11)
+    cases = [(2 ** 19 + 5, 2 ** 18 + 3), (2 ** 20 + 5, 2 ** 18 + 3), (2 ** 19 + 5, 2 ** 19 + 3), (2 ** 15, 2 ** 15)]
+    xs, ys = [], []
+    for ax, ay in cases:
+        x = rng.integers(-(ax - 1), ax, L)
+        y = rng.integers(-(ay - 1), ay, L)
+        x[0], y[-1] = ax, -ay  # the largest magnitude of each operand, exactly
+        xs.append(x.astype(np.int64))
+        ys.append(y.astype(np.int64))
+    xs.append(np.array([7, -3], dtype=np.int64))
+    ys.append(np.array([-2, 5], dtype=np.int64))
+    lens = [len(v) for v in xs]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    z = host(convolution.karatsuba_ragged(off, to_dev(np.concatenate(xs), torch.int64),
+                                          to_dev(np.concatenate(ys), torch.int64)))
+    for p, n in enumerate(lens):
+        ref = [0] * (2 * n - 1)
+        xp, yp = [int(v) for v in xs[p]], [int(v) for v in ys[p]]
+        for i in range(n):
+            for j in range(n):
+                ref[i + j] += xp[i] * yp[j]
+        got = z[2 * off[p] - p: 2 * off[p] - p + 2 * n - 1]
+        assert [int(v) for v in got] == ref, (p, n)
+
+
 @pytest.mark.parametrize("L", [4609, 5000, 16384, 100000])
 def test_karatsuba_past_lds_limit(L):
     """Convolution.Adhoc.karatsuba has no length limit (Adhoc.hs:33-64): pairs
@@ -437,7 +471,10 @@ def test_three_pass_2p24(prec, layout, sign):
 
 
 @pytest.mark.parametrize("n,batch,layout,prec", [(65536, 8, "planar", "f32"), (20000, 3, "interleaved", "f32"),
-                                                (100003, 2, "planar", "f32"), (1000, 16, "interleaved", "f64")])
+                                                (100003, 2, "planar", "f32"), (1000, 16, "interleaved", "f64"),
+                                                # prime-factor order: the tuner's other radix orders run the
+                                                # twiddled order instead
+                                                (15015, 4, "interleaved", "f32"), (1155, 8, "planar", "f64")])
 def test_autotuned_transform(n, batch, layout, prec):
     """After af_plan_tune the transform (and its adjoint) with the tuned shapes
     still matches numpy's FP64 FFT; the wisdom text round-trips."""
