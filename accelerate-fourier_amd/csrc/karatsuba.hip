@@ -338,11 +338,13 @@ constexpr int kKarPadX = 16;                      // zero words before x
 constexpr int kKarPadB = 16;                      // zero blocks before y
 constexpr int kKarStage = 16 * 17;                // one wavefront's tile, skewed
 
-// LDS words of one pair: result 2P, x (Qp + 32), skewed y ((Qp/16 + 32) 17),
-// the wavefronts' tiles, the term table and the reduction slots.
+// LDS words of one pair: result 2P, two sets (one subtree computed while the
+// next is formed) of x (Qp + 32), skewed y ((Qp/16 + 32) 17) and the term table,
+// the wavefronts' tiles and the reduction slots.
 __host__ __device__ inline int karatsuba_mfma_lds_words(int P, int Q) {
     const int Qp = (Q + 15) & ~15;
-    return 2 * P + (Qp + 32) + (Qp / 16 + 32) * 17 + kKarWaves * kKarStage + 2 * kKarMfmaMaxTerms + 2 * kKarWaves;
+    return 2 * P + 2 * ((Qp + 32) + (Qp / 16 + 32) * 17) + kKarWaves * kKarStage + 4 * kKarMfmaMaxTerms +
+           2 * kKarWaves;
 }
 
 // taken[p] = 1 for the pairs done here, 0 for those left to k_karatsuba
@@ -362,10 +364,11 @@ k_karatsuba_mfma(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__re
         const int Qp = (Q + 15) & ~15, nb = Qp / 16;
         const int nxs = Qp + 32, nys = (nb + 32) * 17;
         u64 *acc = sm;
-        double *xs = (double *)(acc + 2 * P), *ys = xs + nxs, *stage = ys + nys + wid * kKarStage;
-        int64_t *tshift = (int64_t *)(ys + nys + kKarWaves * kKarStage);
-        int64_t *tneg = tshift + kKarMfmaMaxTerms;
-        u64 *red = (u64 *)(tneg + kKarMfmaMaxTerms);
+        double *xs = (double *)(acc + 2 * P);  // set k: x at xs + k (nxs + nys), y after it
+        double *stage = xs + 2 * (nxs + nys) + wid * kKarStage;
+        int64_t *tshift = (int64_t *)(xs + 2 * (nxs + nys) + kKarWaves * kKarStage);  // set k at k 16
+        int64_t *tneg = tshift + 2 * kKarMfmaMaxTerms;
+        u64 *red = (u64 *)(tneg + 2 * kKarMfmaMaxTerms);
         // element e of each block of one operand, as doubles (exact for the pairs
         // taken; the others leave before using them): threads e < Qp hold x,
         // threads Qp + e hold y
@@ -405,7 +408,8 @@ k_karatsuba_mfma(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__re
             continue;
         }
         for (int i = tid; i < 2 * P; i += NT) acc[i] = 0ull;
-        for (int i = tid; i < nxs + nys; i += NT) xs[i] = 0.0;  // the padding stays zero
+        for (int i = tid; i < 2 * (nxs + nys); i += NT) xs[i] = 0.0;  // the padding stays zero
+        __syncthreads();  // zeroed before any operand is formed
         int nsub = 1;
         for (int i = 0; i < T; ++i) nsub *= 3;
         // the (tile, delta) units of one subtree product: tiles of 16 output
@@ -417,18 +421,19 @@ k_karatsuba_mfma(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__re
             nunit += dhi >= dlo ? dhi - dlo + 1 : 0;
         }
         const int u_begin = (int)((int64_t)nunit * wid / kKarWaves), u_end = (int)((int64_t)nunit * (wid + 1) / kKarWaves);
-        for (int sub = 0; sub < nsub; ++sub) {
+        // operands and terms of subtree `sb` into set k
+        auto form = [&](int sb, int k) {
             int tbase, tfree;
-            digits3(sub, T, tbase, tfree);
-            __syncthreads();  // the previous subtree's operand reads are done
+            digits3(sb, T, tbase, tfree);
+            double *xk = xs + k * (nxs + nys), *yk = xk + nxs;
             if (tid < 2 * Qp) {
                 // the selected blocks' sum: one FMA per block with a uniform 0/1
                 // weight (exact: integers below 2^51)
                 double X = 0.0;
 #pragma unroll
                 for (int b = 0; b < kKarMaxBlocks; ++b) X = fma((b & ~tfree) == tbase ? 1.0 : 0.0, ob[b], X);
-                if (hold_y) ys[(kKarPadB + (e >> 4)) * 17 + (e & 15)] = X;
-                else xs[kKarPadX + e] = X;
+                if (hold_y) yk[(kKarPadB + (e >> 4)) * 17 + (e & 15)] = X;
+                else xk[kKarPadX + e] = X;
             }
             // the subtree's top-level terms (shift, sign) of prod_i c_{t_i}(u_i)
             const int nvar = T - __popc(tfree);
@@ -454,10 +459,21 @@ k_karatsuba_mfma(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__re
                         }
                     }
                 }
-                tshift[c] = shift;
-                tneg[c] = neg ? 1 : 0;
+                tshift[k * kKarMfmaMaxTerms + c] = shift;
+                tneg[k * kKarMfmaMaxTerms + c] = neg ? 1 : 0;
             }
-            __syncthreads();
+        };
+        form(0, 0);
+        __syncthreads();
+        // subtree sb from set sb & 1 while the next one is formed into the
+        // other set: one barrier per subtree
+        for (int sb = 0; sb < nsub; ++sb) {
+            const int k = sb & 1;
+            int tbase, tfree;
+            digits3(sb, T, tbase, tfree);
+            const int nvar = T - __popc(tfree);
+            const double *xk = xs + k * (nxs + nys), *yk = xk + nxs;
+            const int64_t *tsh = tshift + k * kKarMfmaMaxTerms, *tng = tneg + k * kKarMfmaMaxTerms;
             const int nterm = 1 << nvar;
             kar_v4d D = {0.0, 0.0, 0.0, 0.0};
             int cur_tile = -1;
@@ -478,8 +494,8 @@ k_karatsuba_mfma(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__re
                 }
                 for (int c = 0; c < nterm; ++c) {
                     // the term's shift and sign, uniform: scalar registers
-                    const int64_t sh = __builtin_amdgcn_readfirstlane((int)tshift[c]);
-                    const bool ng = __builtin_amdgcn_readfirstlane((int)tneg[c]) != 0;
+                    const int64_t sh = __builtin_amdgcn_readfirstlane((int)tsh[c]);
+                    const bool ng = __builtin_amdgcn_readfirstlane((int)tng[c]) != 0;
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
                         if (val[q] != 0ull)
@@ -502,9 +518,9 @@ k_karatsuba_mfma(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__re
                     }
                     for (int dl = dlo + (lo - u); dl < dlo + (hi - u); ++dl) {
                         // A: T_delta[r16][q] = x[16 delta + r16 - q], q = 4 kap + k4
-                        const double *xa = xs + kKarPadX + 16 * dl + r16 - k4 - 12;
+                        const double *xa = xk + kKarPadX + 16 * dl + r16 - k4 - 12;
                         // B: y block c0 + r16 - delta, element q
-                        const double *yv = ys + (c0 + r16 - dl + kKarPadB) * 17 + k4;
+                        const double *yv = yk + (c0 + r16 - dl + kKarPadB) * 17 + k4;
                         D = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[12], yv[0], D, 0, 0, 0);
                         D = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[8], yv[4], D, 0, 0, 0);
                         D = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[4], yv[8], D, 0, 0, 0);
@@ -514,8 +530,9 @@ k_karatsuba_mfma(const u64 *__restrict__ x, const u64 *__restrict__ y, u64 *__re
                 u += cnt;
             }
             if (cur_tile >= 0) flush(cur_tile);
+            if (sb + 1 < nsub) form(sb + 1, k ^ 1);
+            __syncthreads();
         }
-        __syncthreads();
         if (cyclic) {
             for (int64_t k = tid; k < Lp; k += NT)
                 z[zoff + k] = acc[k] + ((k + Lp < 2 * Lp - 1) ? acc[k + Lp] : 0ull);

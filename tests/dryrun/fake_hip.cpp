@@ -473,7 +473,7 @@ std::function<void()> check_launch(const std::string &name, dim3 grid, dim3 bloc
                 int64_t n3 = 1;
                 for (int i = 0; i < B; ++i) n3 *= 3;
                 const int64_t Qp = ((sl << B) + 15) / 16 * 16;
-                const int64_t words = mfma ? 2 * P + (Qp + 32) + (Qp / 16 + 32) * 17 + 16 * 16 * 17 + 2 * 16 + 2 * 16
+                const int64_t words = mfma ? 2 * P + 2 * ((Qp + 32) + (Qp / 16 + 32) * 17) + 16 * 16 * 17 + 4 * 16 + 2 * 16
                                            : 2 * P + 2 * (sl << B) + n3 * 2 * sl + (B ? n3 / 3 * 4 * sl : 0) + 16 * 128;
                 if (words * 8 > (int64_t)shmem)
                     problem(name + ": a pair needs more LDS than the launch has (L=" + std::to_string(L) + " need " +
