@@ -488,12 +488,22 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
     __syncthreads();
 
     // ---- load phase ----
+    // A thread holds at most EPT elements of the tile (B L <= EPT * blockDim.x
+    // for every launch the planner makes; a remainder loop covers any other):
+    // its loads are issued a batch of elements at a time, before the batch's
+    // LDS writes, instead of one load latency per element.
+    constexpr int EPT = pass_ept(sizeof(T) == 8);
+    // elements whose loads are in flight together: as many as fit the 128
+    // registers of a 1024-thread workgroup beside the rest of the kernel
+    constexpr int kLoadBatch = sizeof(T) == 8 ? 4 : 8, kStoreBatch = sizeof(T) == 8 ? 2 : 4;
+    static_assert(EPT % kLoadBatch == 0 && EPT % kStoreBatch == 0, "batches divide EPT");
     const int total = B * L;
     const float rB = 1.0f / (float)B, rL = 1.0f / (float)L;
-    for (int idx = tid; idx < total; idx += NT) {
+    auto load_elem = [&](const int idx, int &li) -> C {
         int b, e;
         if (d.in_lines_fast) divmod_small(idx, B, rB, e, b);
         else divmod_small(idx, L, rL, b, e);
+        li = pidx(b * ld + (d.gt_in ? gld<int32_t>(d.gt_in, e) : e), ps);
         C v = mk(T(0), T(0));
         if (tl.valid[b]) {
             const int64_t r = tl.row[b], l = tl.line[b];
@@ -536,7 +546,27 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
                 }
             }
         }
-        lds[pidx(b * ld + (d.gt_in ? gld<int32_t>(d.gt_in, e) : e), ps)] = v;
+        return v;
+    };
+#pragma unroll 1
+    for (int k0 = 0; k0 < EPT; k0 += kLoadBatch) {
+        C vb[kLoadBatch];
+        int lb[kLoadBatch];
+#pragma unroll
+        for (int k = 0; k < kLoadBatch; ++k) {
+            lb[k] = -1;
+            if (tid + (k0 + k) * NT < total) vb[k] = load_elem(tid + (k0 + k) * NT, lb[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kLoadBatch; ++k)
+            if (lb[k] >= 0) lds[lb[k]] = vb[k];
+    }
+    {
+        for (int idx = tid + EPT * NT; idx < total; idx += NT) {
+            int li;
+            const C v = load_elem(idx, li);
+            lds[li] = v;
+        }
     }
     __syncthreads();
 
@@ -546,21 +576,29 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
 
     // ---- convolution: spectrum product and inverse stages ----
     if (flags & AFP_CONV) {
-        for (int idx = tid; idx < total; idx += NT) {
+        auto spec_elem = [&](const int idx) {
             int b, e;
             divmod_small(idx, L, rL, b, e);
-            if (!tl.valid[b]) continue;
+            if (!tl.valid[b]) return;
             const int64_t r = tl.row[b], l = tl.line[b];
             const int li = pidx(b * ld + e, ps);
+            const int64_t si = r * d.spec_bs + (int64_t)((uint32_t)l * (uint32_t)d.spec_ls + (uint32_t)e);
             C v = lds[li];
             if (l == 0 && e == 0) {
                 if (flags & AFP_RADER) s_dc[b] = v;
                 if (flags & AFP_DC_CAPTURE) gst<C>(d.dc_out, r, v);
             }
-            const int64_t si = r * d.spec_bs + (int64_t)((uint32_t)l * (uint32_t)d.spec_ls + (uint32_t)e);
             if (flags & AFP_SPEC_PLAIN) lds[li] = cmul(v, gld<C>(d.spec, si));
             else lds[li] = vmul(v, gld<W>(d.spec, si), plain);
-        }
+        };
+        // unrolled in batches: the spectrum loads (global) do not wait for the
+        // LDS writes
+#pragma unroll 1
+        for (int k0 = 0; k0 < EPT; k0 += kLoadBatch)
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; ++k)
+                if (tid + (k0 + k) * NT < total) spec_elem(tid + (k0 + k) * NT);
+        for (int idx = tid + EPT * NT; idx < total; idx += NT) spec_elem(idx);
         __syncthreads();
         run_stages<T>(lds, d, -s1);
     }
@@ -609,14 +647,16 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
         return;
     }
     const bool rader_any = (flags & (AFP_RADER | AFP_RADER_OUT)) != 0;
-    for (int idx = tid; idx < total; idx += NT) {
+    // the value and address of output element idx (address -1: not written)
+    auto store_elem = [&](const int idx, int64_t &ga) -> C {
         int b, e;
         if (d.out_lines_fast) divmod_small(idx, B, rB, e, b);
         else divmod_small(idx, L, rL, b, e);
-        if (!tl.valid[b]) continue;
+        ga = -1;
+        if (!tl.valid[b]) return mk(T(0), T(0));
         const int64_t r = tl.row[b], l = tl.line[b];
         const int64_t o = (int64_t)((uint32_t)l * (uint32_t)d.out_ls + (uint32_t)e * (uint32_t)d.out_es);
-        if (o >= d.out_len) continue;
+        if (o >= d.out_len) return mk(T(0), T(0));
         C v = lds[pidx(b * ld + (d.gt_out ? gld<int32_t>(d.gt_out, e) : e), ps)];
         if (flags & AFP_POSTTW) {
             // the lines of a four-step pass number N / L, so l * e < N: no reduction
@@ -624,15 +664,36 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
         }
         if (d.out_mul) v = cmul(v, gld<C>(d.out_mul, o));
         if (flags & AFP_SCALE) v = cscale(v, T(d.scale));
-        const int64_t ob = r * d.out_bs;
         if (rader_any) {
             C x0;
             if (flags & AFP_RADER) x0 = s_x0[b];
             else x0 = ld_c<T>(d.x0_re, d.x0_im, r * d.x0_bs);
             v = cadd(v, x0);
         }
-        const int64_t a = d.out_map ? (int64_t)gld<int32_t>(d.out_map, o) : o;
-        st_c<T>(d.out_re, d.out_im, ob + a, v);
+        ga = r * d.out_bs + (d.out_map ? (int64_t)gld<int32_t>(d.out_map, o) : o);
+        return v;
+    };
+    // a batch of elements' table loads before the first global store of the
+    // batch (the compiler cannot move a load above a store that may alias it)
+#pragma unroll 1
+    for (int k0 = 0; k0 < EPT; k0 += kStoreBatch) {
+        C vb[kStoreBatch];
+        int64_t gb[kStoreBatch];
+#pragma unroll
+        for (int k = 0; k < kStoreBatch; ++k) {
+            gb[k] = -1;
+            if (tid + (k0 + k) * NT < total) vb[k] = store_elem(tid + (k0 + k) * NT, gb[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kStoreBatch; ++k)
+            if (gb[k] >= 0) st_c<T>(d.out_re, d.out_im, gb[k], vb[k]);
+    }
+    {
+        for (int idx = tid + EPT * NT; idx < total; idx += NT) {
+            int64_t ga;
+            const C v = store_elem(idx, ga);
+            if (ga >= 0) st_c<T>(d.out_re, d.out_im, ga, v);
+        }
     }
     if (rader_any && tid < B && tl.valid[tid] && tl.line[tid] == 0) {
         // X[0] = x0 + sum of the other inputs = x0 + DC of the permuted sequence
