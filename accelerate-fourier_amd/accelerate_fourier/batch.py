@@ -58,8 +58,9 @@ class UnevenBatch:
 
     def tune(self) -> str:
         """af_batch_tune: times chunk sizes of the batch's graph on the current
-        stream and keeps the fastest; returns the report."""
-        buf = ctypes.create_string_buffer(8192)
+        stream and keeps the fastest, then the radix order of each generic
+        grouped launch; returns the report."""
+        buf = ctypes.create_string_buffer(1 << 16)
         _abi.check(_abi.lib().af_batch_tune(self.handle, _arrays.stream_handle(), buf, len(buf)), "af_batch_tune")
         return buf.value.decode()
 

@@ -2172,6 +2172,9 @@ struct af_batch_s {
         // extended fast kernel instead of the generic one
         bool ext = false, conv = false;
         int L = 0, B = 0, sign = 0;
+        // generic groups: the radix order of its descriptors (reorder_radices),
+        // chosen by af_batch_tune; 0 = as planned
+        int order = 0;
         std::vector<int64_t> rows;  // transform indices
         int64_t desc_off = 0, prefix_off = 0, total_tiles = 0;
     };
@@ -2417,24 +2420,31 @@ af_status batch_build(af_batch B, int64_t chunk_bytes, int lanes) {
     return AF_SUCCESS;
 }
 
+// The host descriptors of one grouped launch for the given buffers, with the
+// group's radix order.
+af_status batch_group_descs(af_batch B, const af_batch_s::Group &g, const void *in_re, const void *in_im,
+                            void *out_re, void *out_im) {
+    std::vector<AfPass> ps;
+    int64_t j = g.desc_off;
+    for (int64_t i : g.rows) {
+        ps.clear();
+        AFCHK(batch_passes(B, i, in_re, in_im, out_re, out_im, (char *)B->scratch, ps));
+        AfPass d = ps[g.phase];
+        if (!g.ext) reorder_radices(d, g.order);  // the stage order changes no tile, thread or LDS count
+        // the captured launch fixed this descriptor's tiles and shape
+        const int64_t k = j - g.desc_off;
+        if (d.ntiles != B->h_prefix[g.prefix_off + k + 1] - B->h_prefix[g.prefix_off + k] ||
+            (g.ext && (d.L != g.L || d.B != g.B)) || (!g.ext && (d.nthreads > g.nthreads || d.lds_bytes > g.lds)))
+            return AF_INTERNAL;
+        B->h_descs[j++] = d;
+    }
+    return AF_SUCCESS;
+}
+
 // The descriptors of every grouped launch for the given buffers, uploaded on s
 // (the caller holds B->mu and has waited for the previous launch).
 af_status batch_upload(af_batch B, const void *in_re, const void *in_im, void *out_re, void *out_im, hipStream_t s) {
-    std::vector<AfPass> ps;
-    for (const auto &g : B->groups) {
-        int64_t j = g.desc_off;
-        for (int64_t i : g.rows) {
-            ps.clear();
-            AFCHK(batch_passes(B, i, in_re, in_im, out_re, out_im, (char *)B->scratch, ps));
-            const AfPass &d = ps[g.phase];
-            // the captured launch fixed this descriptor's tiles and shape
-            const int64_t k = j - g.desc_off;
-            if (d.ntiles != B->h_prefix[g.prefix_off + k + 1] - B->h_prefix[g.prefix_off + k] ||
-                (g.ext && (d.L != g.L || d.B != g.B)) || (!g.ext && (d.nthreads > g.nthreads || d.lds_bytes > g.lds)))
-                return AF_INTERNAL;
-            B->h_descs[j++] = d;
-        }
-    }
+    for (const auto &g : B->groups) AFCHK(batch_group_descs(B, g, in_re, in_im, out_re, out_im));
     if (!B->h_descs.empty())
         HIPCHK(hipMemcpyAsync(B->d_descs, B->h_descs.data(), B->h_descs.size() * sizeof(AfPass), hipMemcpyHostToDevice,
                               s));
@@ -2587,18 +2597,59 @@ af_status af_batch_tune(af_batch B, void *stream, char *report, size_t report_le
             best_lanes = cand.second;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     { const af_status h__ = HIPST(hipStreamSynchronize(s)); if (st == AF_SUCCESS) st = h__; }
     // the chosen budget (or, after a failure, the one the batch had)
     const af_status st2 =
         st == AF_SUCCESS ? batch_build(B, best, best_lanes) : batch_build(B, start, start_lanes);
     B->recorded = false;
     if (st == AF_SUCCESS) st = st2;
-    if (st == AF_SUCCESS && report && report_len) {
+    if (st == AF_SUCCESS)
         text += "chosen chunk_bytes=" + std::to_string((long long)best) + " lanes=" + std::to_string(best_lanes) + "\n";
-        std::snprintf(report, report_len, "%s", text.c_str());
+    // then the radix order of every generic grouped launch, each timed alone:
+    // as planned, reversed, odd radices first (the incumbent must be beaten by 3%)
+    if (st == AF_SUCCESS) st = batch_upload(B, in.p, nullptr, out.p, nullptr, s);
+    for (size_t k = 0; k < B->groups.size() && st == AF_SUCCESS; ++k) {
+        af_batch_s::Group &g = B->groups[k];
+        if (g.ext) continue;
+        int maxst = 0;
+        for (int64_t j = g.desc_off; j < g.desc_off + (int64_t)g.rows.size(); ++j)
+            maxst = std::max(maxst, B->h_descs[j].nst);
+        if (maxst < 2) continue;
+        double tb = 1e30;
+        int bo = 0;
+        for (int o = 0; o < 3 && st == AF_SUCCESS; ++o) {
+            g.order = o;
+            st = batch_group_descs(B, g, in.p, nullptr, out.p, nullptr);
+            if (st == AF_SUCCESS)
+                st = HIPST(hipMemcpyAsync(B->d_descs + g.desc_off, B->h_descs.data() + g.desc_off,
+                                          g.rows.size() * sizeof(AfPass), hipMemcpyHostToDevice, s));
+            double t = 1e30;
+            for (int r = 0; r < 6 && st == AF_SUCCESS; ++r) {
+                st = HIPST(hipEventRecord(e0, s));
+                if (st == AF_SUCCESS)
+                    st = HIPST(launch_pass_grouped(B->prec, B->d_descs + g.desc_off, B->d_prefix + g.prefix_off,
+                                                   (int)g.rows.size(), g.total_tiles, g.nthreads, g.lds, g.full, s));
+                if (st == AF_SUCCESS) st = HIPST(hipEventRecord(e1, s));
+                if (st == AF_SUCCESS) st = HIPST(hipEventSynchronize(e1));
+                float ms = 0.f;
+                if (st == AF_SUCCESS) st = HIPST(hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) t = std::min(t, (double)ms);
+            }
+            if (st == AF_SUCCESS && (o == 0 || t < tb * 0.97)) {
+                tb = t;
+                bo = o;
+            }
+        }
+        g.order = bo;
+        char line[128];
+        std::snprintf(line, sizeof line, "launch %zu order=%d %.1f us\n", k, bo, tb * 1e3);
+        text += line;
     }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    { const af_status h__ = HIPST(hipStreamSynchronize(s)); if (st == AF_SUCCESS) st = h__; }
+    B->uploaded = false;  // the next call uploads its buffers' descriptors, with the chosen orders
+    if (st == AF_SUCCESS && report && report_len) std::snprintf(report, report_len, "%s", text.c_str());
     return st;
 }
 

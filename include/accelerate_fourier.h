@@ -351,8 +351,13 @@ af_status af_batch_set_schedule(af_batch batch, int64_t chunk_bytes, int32_t lan
  * for several schedules (one lane unchunked; two lanes unchunked and with
  * chunks of 32 MiB ... 384 MiB of intermediates) on zero-filled interleaved
  * buffers it allocates itself, keeps the fastest (it must beat the incumbent
- * by 3%) and recaptures the graph with it.  Synchronizes `stream`.  `report`
- * (may be NULL) receives one line per candidate and the choice.
+ * by 3%) and recaptures the graph with it.  Then it times every grouped
+ * launch of the generic kernel alone with its stages as planned, reversed and
+ * odd radices first, and keeps the fastest order per launch (again by 3%);
+ * the order lives in the launch's descriptors, so the graph stays as
+ * captured.  af_batch_set_schedule resets the orders to the planned ones.
+ * Synchronizes `stream`.  `report` (may be NULL) receives one line per
+ * schedule, the choice, and one line per timed launch.
  */
 af_status af_batch_tune(af_batch batch, void *stream, char *report, size_t report_len);
 /*

@@ -221,6 +221,8 @@ def test_uneven_batch_4096_rows(dry, prec):
         assert dry.af_batch_tune(h, None, rep, len(rep)) == 0
         text = rep.value.decode()
         assert text.count("chunk_bytes=") == 8 and "chosen chunk_bytes=0 lanes=2" in text, text
+        orders = [ln for ln in text.split("\n") if ln.startswith("launch ")]
+        assert orders and all(" order=0 " in ln for ln in orders), text
         assert dry.af_batch_describe(h, desc, len(desc)) == 0
         assert " chunks=" in desc.value.decode().split("\n")[0]
         h2 = b(tot * e // 2)

@@ -863,6 +863,7 @@ def test_uneven_batch_chunks_and_tuner():
         del got
     rep = b.tune()
     assert rep.count("chunk_bytes=") == 8 and "chosen chunk_bytes=" in rep, rep
+    assert " order=" in rep, rep
     print(rep)
     got = b(x32)
     torch.cuda.synchronize()
