@@ -258,6 +258,19 @@ __device__ __forceinline__ void divmod_small(int n, int d, float rcp, int &q, in
     }
 }
 
+// A value every lane of the wavefront holds alike (a descriptor field), moved
+// to scalar registers: branches on it become scalar branches and the compiler
+// no longer treats it as varying per lane (the descriptor is read through a
+// generic or LDS pointer, whose loads it must assume differ between lanes).
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uni(int64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)(uint64_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+template <typename P> __device__ __forceinline__ P *uni(P *p) { return (P *)(uintptr_t)uni((int64_t)(uintptr_t)p); }
+__device__ __forceinline__ double uni(double v) { return __longlong_as_double(uni((int64_t)__double_as_longlong(v))); }
+
 // One Stockham stage of radix R over a tile of B lines of length L.
 // Butterfly i of a line reads elements i + j L/R and writes
 // (i - k) R + k + j p with k = i mod p, p = product of the previous radices.
@@ -314,21 +327,28 @@ __device__ __forceinline__ void stage(typename Cx<T>::C *lds, const int L, const
 
 template <typename T, typename D>
 __device__ __forceinline__ void run_stages(typename Cx<T>::C *lds, const D &d, const T s) {
-    const void *tw = d.tw;
-    const bool plain = (d.flags & AFP_PLAIN_TW) != 0;
-    const bool notw = d.grp_mask != 0;
-    for (int st = 0; st < d.nst; ++st) {
-        const int R = d.radix[st], p = d.pst[st];
+    const void *tw0 = uni(d.tw);
+    const bool plain = (uni(d.flags) & AFP_PLAIN_TW) != 0;
+    const bool notw = uni(d.grp_mask) != 0;
+    const int L0 = uni(d.L), ld0 = uni(d.ld), B0 = uni(d.B), ps = uni(d.padshift), nst = uni(d.nst);
+    for (int st = 0; st < nst; ++st) {
+        // the geometry re-read in scalar registers in every stage: values
+        // derived from it (per radix) stay inside the stage instead of being
+        // hoisted out of the loop for every radix at once
+        int L = L0, ld = ld0, B = B0;
+        const void *tw = tw0;
+        asm volatile("" : "+s"(L), "+s"(ld), "+s"(B), "+s"(tw));
+        const int R = uni(d.radix[st]), p = uni(d.pst[st]);
         switch (R) {
-        case 2: stage<T, 2>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
-        case 3: stage<T, 3>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
-        case 4: stage<T, 4>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
-        case 5: stage<T, 5>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
-        case 7: stage<T, 7>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
-        case 8: stage<T, 8>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
-        case 11: stage<T, 11>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
-        case 13: stage<T, 13>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
-        default: stage<T, 16>(lds, d.L, d.ld, d.B, p, tw, s, d.padshift, plain, notw); break;
+        case 2: stage<T, 2>(lds, L, ld, B, p, tw, s, ps, plain, notw); break;
+        case 3: stage<T, 3>(lds, L, ld, B, p, tw, s, ps, plain, notw); break;
+        case 4: stage<T, 4>(lds, L, ld, B, p, tw, s, ps, plain, notw); break;
+        case 5: stage<T, 5>(lds, L, ld, B, p, tw, s, ps, plain, notw); break;
+        case 7: stage<T, 7>(lds, L, ld, B, p, tw, s, ps, plain, notw); break;
+        case 8: stage<T, 8>(lds, L, ld, B, p, tw, s, ps, plain, notw); break;
+        case 11: stage<T, 11>(lds, L, ld, B, p, tw, s, ps, plain, notw); break;
+        case 13: stage<T, 13>(lds, L, ld, B, p, tw, s, ps, plain, notw); break;
+        default: stage<T, 16>(lds, L, ld, B, p, tw, s, ps, plain, notw); break;
         }
     }
 }
@@ -351,6 +371,13 @@ __device__ __forceinline__ void st_c(void *re, void *im, int64_t a, typename Cx<
         gst<typename Cx<T>::C>(re, a, v);
     }
 }
+
+// The four-step twiddle fields of a descriptor, held in scalar registers.
+struct FtTables {
+    const void *ft_coarse;
+    const void *ft_fine;
+    int32_t ftShift, ftSign;
+};
 
 // Four-step twiddle W_N^(sign * a), a < N, as the product coarse[a >> shift] *
 // fine[a & mask] of two table entries.  FP64: plain double product.  FP32: the
@@ -470,8 +497,8 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
     using C = typename Cx<T>::C;
     using W = typename Cx<T>::W;
     const int tid = threadIdx.x, NT = blockDim.x;
-    const int B = d.B, L = d.L, ld = d.ld, ps = d.padshift;
-    const int32_t flags = FULL ? d.flags : (d.flags & ~kRarePassFlags);
+    const int B = uni(d.B), L = uni(d.L), ld = uni(d.ld), ps = uni(d.padshift);
+    const int32_t flags = FULL ? uni(d.flags) : (uni(d.flags) & ~kRarePassFlags);
     const bool plain = (flags & AFP_PLAIN_TW) != 0;
     const int64_t nlines_total = d.nrows * d.nlines;
 
@@ -499,15 +526,20 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
     static_assert(EPT % kLoadBatch == 0 && EPT % kStoreBatch == 0, "batches divide EPT");
     const int total = B * L;
     const float rB = 1.0f / (float)B, rL = 1.0f / (float)L;
+    const bool in_lf = uni(d.in_lines_fast) != 0;
+    const int32_t *const gt_in = uni(d.gt_in), *const in_map = uni(d.in_map);
+    const void *const in_re = uni(d.in_re), *const in_im = uni(d.in_im), *const in_mul = uni(d.in_mul);
+    const int64_t in_bs = uni(d.in_bs), in_len = uni(d.in_len);
+    const uint32_t in_ls = (uint32_t)uni((int32_t)d.in_ls), in_es = (uint32_t)uni((int32_t)d.in_es);
     auto load_elem = [&](const int idx, int &li) -> C {
         int b, e;
-        if (d.in_lines_fast) divmod_small(idx, B, rB, e, b);
+        if (in_lf) divmod_small(idx, B, rB, e, b);
         else divmod_small(idx, L, rL, b, e);
-        li = pidx(b * ld + (d.gt_in ? gld<int32_t>(d.gt_in, e) : e), ps);
+        li = pidx(b * ld + (gt_in ? gld<int32_t>(gt_in, e) : e), ps);
         C v = mk(T(0), T(0));
         if (tl.valid[b]) {
             const int64_t r = tl.row[b], l = tl.line[b];
-            const int64_t rb = r * d.in_bs;
+            const int64_t rb = r * in_bs;
             if (flags & AFP_REAL_ENTANGLE) {
                 // fromSpectrum (Real.hs:96-107): z = (S_e + S_{e+L}) + i W_{2L}^{+e} (S_e - S_{e+L})
                 const C p0 = ld_c<T>(d.in_re, d.in_im, rb + e);
@@ -538,11 +570,11 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
                 v = cadd(cadd(q0, q1), mk(-t.y, t.x));
             } else {
                 // positions inside a row stay below 2^31 (lengths <= 2^30)
-                const int64_t pos = (int64_t)((uint32_t)l * (uint32_t)d.in_ls + (uint32_t)e * (uint32_t)d.in_es);
-                if (pos < d.in_len) {
-                    const int64_t a = d.in_map ? (int64_t)gld<int32_t>(d.in_map, pos) : pos;
-                    v = ld_c<T>(d.in_re, d.in_im, rb + a);
-                    if (d.in_mul) v = cmul(v, gld<C>(d.in_mul, pos));
+                const int64_t pos = (int64_t)((uint32_t)l * in_ls + (uint32_t)e * in_es);
+                if (pos < in_len) {
+                    const int64_t a = in_map ? (int64_t)gld<int32_t>(in_map, pos) : pos;
+                    v = ld_c<T>(in_re, in_im, rb + a);
+                    if (in_mul) v = cmul(v, gld<C>(in_mul, pos));
                 }
             }
         }
@@ -576,20 +608,23 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
 
     // ---- convolution: spectrum product and inverse stages ----
     if (flags & AFP_CONV) {
+        const void *const spec = uni(d.spec);
+        const int64_t spec_bs = uni(d.spec_bs);
+        const uint32_t spec_ls = (uint32_t)uni((int32_t)d.spec_ls);
         auto spec_elem = [&](const int idx) {
             int b, e;
             divmod_small(idx, L, rL, b, e);
             if (!tl.valid[b]) return;
             const int64_t r = tl.row[b], l = tl.line[b];
             const int li = pidx(b * ld + e, ps);
-            const int64_t si = r * d.spec_bs + (int64_t)((uint32_t)l * (uint32_t)d.spec_ls + (uint32_t)e);
+            const int64_t si = r * spec_bs + (int64_t)((uint32_t)l * spec_ls + (uint32_t)e);
             C v = lds[li];
             if (l == 0 && e == 0) {
                 if (flags & AFP_RADER) s_dc[b] = v;
                 if (flags & AFP_DC_CAPTURE) gst<C>(d.dc_out, r, v);
             }
-            if (flags & AFP_SPEC_PLAIN) lds[li] = cmul(v, gld<C>(d.spec, si));
-            else lds[li] = vmul(v, gld<W>(d.spec, si), plain);
+            if (flags & AFP_SPEC_PLAIN) lds[li] = cmul(v, gld<C>(spec, si));
+            else lds[li] = vmul(v, gld<W>(spec, si), plain);
         };
         // unrolled in batches: the spectrum loads (global) do not wait for the
         // LDS writes
@@ -647,30 +682,42 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
         return;
     }
     const bool rader_any = (flags & (AFP_RADER | AFP_RADER_OUT)) != 0;
+    const bool out_lf = uni(d.out_lines_fast) != 0;
+    const int32_t *const gt_out = uni(d.gt_out), *const out_map = uni(d.out_map);
+    const void *const out_mul = uni(d.out_mul);
+    void *const out_re = uni(d.out_re), *const out_im = uni(d.out_im);
+    const int64_t out_bs = uni(d.out_bs), out_len = uni(d.out_len);
+    const uint32_t out_ls = (uint32_t)uni((int32_t)d.out_ls), out_es = (uint32_t)uni((int32_t)d.out_es);
+    const T scale = T(uni(d.scale));
+    FtTables ft;
+    ft.ft_coarse = uni(d.ft_coarse);
+    ft.ft_fine = uni(d.ft_fine);
+    ft.ftShift = uni(d.ftShift);
+    ft.ftSign = uni(d.ftSign);
     // the value and address of output element idx (address -1: not written)
     auto store_elem = [&](const int idx, int64_t &ga) -> C {
         int b, e;
-        if (d.out_lines_fast) divmod_small(idx, B, rB, e, b);
+        if (out_lf) divmod_small(idx, B, rB, e, b);
         else divmod_small(idx, L, rL, b, e);
         ga = -1;
         if (!tl.valid[b]) return mk(T(0), T(0));
         const int64_t r = tl.row[b], l = tl.line[b];
-        const int64_t o = (int64_t)((uint32_t)l * (uint32_t)d.out_ls + (uint32_t)e * (uint32_t)d.out_es);
-        if (o >= d.out_len) return mk(T(0), T(0));
-        C v = lds[pidx(b * ld + (d.gt_out ? gld<int32_t>(d.gt_out, e) : e), ps)];
+        const int64_t o = (int64_t)((uint32_t)l * out_ls + (uint32_t)e * out_es);
+        if (o >= out_len) return mk(T(0), T(0));
+        C v = lds[pidx(b * ld + (gt_out ? gld<int32_t>(gt_out, e) : e), ps)];
         if (flags & AFP_POSTTW) {
             // the lines of a four-step pass number N / L, so l * e < N: no reduction
-            v = cmul(v, ft_twiddle_t(d, (uint64_t)((uint32_t)l * (uint32_t)e), T(0)));
+            v = cmul(v, ft_twiddle_t(ft, (uint64_t)((uint32_t)l * (uint32_t)e), T(0)));
         }
-        if (d.out_mul) v = cmul(v, gld<C>(d.out_mul, o));
-        if (flags & AFP_SCALE) v = cscale(v, T(d.scale));
+        if (out_mul) v = cmul(v, gld<C>(out_mul, o));
+        if (flags & AFP_SCALE) v = cscale(v, scale);
         if (rader_any) {
             C x0;
             if (flags & AFP_RADER) x0 = s_x0[b];
             else x0 = ld_c<T>(d.x0_re, d.x0_im, r * d.x0_bs);
             v = cadd(v, x0);
         }
-        ga = r * d.out_bs + (d.out_map ? (int64_t)gld<int32_t>(d.out_map, o) : o);
+        ga = r * out_bs + (out_map ? (int64_t)gld<int32_t>(out_map, o) : o);
         return v;
     };
     // a batch of elements' table loads before the first global store of the
@@ -686,13 +733,13 @@ __device__ __forceinline__ void pass_tile(const D &d, const int64_t tile, typena
         }
 #pragma unroll
         for (int k = 0; k < kStoreBatch; ++k)
-            if (gb[k] >= 0) st_c<T>(d.out_re, d.out_im, gb[k], vb[k]);
+            if (gb[k] >= 0) st_c<T>(out_re, out_im, gb[k], vb[k]);
     }
     {
         for (int idx = tid + EPT * NT; idx < total; idx += NT) {
             int64_t ga;
             const C v = store_elem(idx, ga);
-            if (ga >= 0) st_c<T>(d.out_re, d.out_im, ga, v);
+            if (ga >= 0) st_c<T>(out_re, out_im, ga, v);
         }
     }
     if (rader_any && tid < B && tl.valid[tid] && tl.line[tid] == 0) {
