@@ -105,15 +105,11 @@ def test_no_store_data_clobber():
 
 
 # The FP64 generic pass kernels keep a few registers' worth of spills at the
-# 128-register budget of 1024-thread workgroups, and the FP32 single-descriptor
-# ones a value or two around the batched element loads (a spill and a reload
-# per thread, against a load phase that no longer waits one memory latency per
-# element); no other kernel may use scratch (private memory traffic per lane,
-# paid in every workgroup).
+# 128-register budget of 1024-thread workgroups; no other kernel may use
+# scratch (private memory traffic per lane, paid in every workgroup).
 _SCRATCH_ALLOWED = {"_ZN2af6k_passIdLb0EEEv6AfPass": 64, "_ZN2af6k_passIdLb1EEEv6AfPass": 64,
                     "_ZN2af14k_pass_groupedIdLb0EEEvPK6AfPassPKli": 64,
-                    "_ZN2af14k_pass_groupedIdLb1EEEvPK6AfPassPKli": 64,
-                    "_ZN2af6k_passIfLb0EEEv6AfPass": 24, "_ZN2af6k_passIfLb1EEEv6AfPass": 24}
+                    "_ZN2af14k_pass_groupedIdLb1EEEvPK6AfPassPKli": 64}
 
 
 def test_no_kernel_spills():
