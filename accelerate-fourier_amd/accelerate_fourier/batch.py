@@ -48,11 +48,14 @@ class UnevenBatch:
     def set_schedule(self, chunk_bytes: int = 0, lanes: int = 2) -> None:
         """af_batch_set_schedule: rows with intermediates in chunks of at most
         chunk_bytes of them (0: no chunking); lanes = 2 runs the rows without
-        intermediates on a parallel branch of the graph."""
+        intermediates on a parallel branch of the graph, lanes = 3 (unchunked
+        only) also the mixed-radix and Rader rows with intermediates on a third."""
         if int(chunk_bytes) < 0:
             raise ValueError("chunk_bytes must be >= 0")
-        if int(lanes) not in (1, 2):
-            raise ValueError("lanes must be 1 or 2")
+        if int(lanes) not in (1, 2, 3):
+            raise ValueError("lanes must be 1, 2 or 3")
+        if int(lanes) == 3 and int(chunk_bytes) != 0:
+            raise ValueError("three lanes need chunk_bytes = 0")
         _abi.check(_abi.lib().af_batch_set_schedule(self.handle, int(chunk_bytes), int(lanes)),
                    "af_batch_set_schedule")
 

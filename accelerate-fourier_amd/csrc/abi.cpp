@@ -2160,9 +2160,14 @@ struct af_batch_s {
     int nchunks = 0;
     // lanes = 2: the rows without intermediates (chunk 0 when there are any)
     // run on a branch of the graph beside the others, so that their small
-    // launches overlap the large ones; 1: every launch in one sequence
+    // launches overlap the large ones; 1: every launch in one sequence;
+    // 3 (unchunked only): as 2, and the rows with intermediates that some pass
+    // runs on the generic kernel (mixed radix, Rader) on a third branch beside
+    // those all of whose passes run on the power-of-two kernels (Bluestein
+    // pads), each set a chunk of its own with its own scratch
     int lanes = 1;
     bool light_chunk = false;
+    int mixed_chunk = -1;  // lanes = 3: the chunk on the third branch
     DevBuf scratch_buf, descs_buf, prefix_buf;
     void *scratch = nullptr;
     struct Group {
@@ -2266,19 +2271,41 @@ af_status batch_build(af_batch B, int64_t chunk_bytes, int lanes) {
     std::vector<std::vector<int64_t>> chunks;
     if (!light.empty()) chunks.push_back(light);
     B->light_chunk = !light.empty();
+    B->mixed_chunk = -1;
     size_t region = 0, used = 0;
-    for (int64_t i : heavy) {
-        const size_t need = B->scratch_need[i];
-        const bool fresh = chunks.empty() || chunks.back().empty() || B->scratch_need[chunks.back()[0]] == 0 ||
-                           (chunk_bytes > 0 && used + need > (size_t)chunk_bytes);
-        if (fresh) {
-            chunks.emplace_back();
-            used = 0;
+    if (lanes == 3 && chunk_bytes == 0) {
+        // two chunks of rows with intermediates that may run at once: disjoint scratch
+        std::vector<int64_t> mixed, pow2;
+        for (int64_t i : heavy) {
+            bool all_fast = true;
+            for (const AfPass &d : B->shapes[i]) all_fast = all_fast && fast_ext_eligible(B->prec, d);
+            (all_fast ? pow2 : mixed).push_back(i);
         }
-        B->scratch_off[i] = used;
-        used += need;
-        region = std::max(region, used);
-        chunks.back().push_back(i);
+        for (const std::vector<int64_t> *set : {&mixed, &pow2}) {
+            if (set->empty()) continue;
+            if (set == &mixed) B->mixed_chunk = (int)chunks.size();
+            chunks.emplace_back();
+            for (int64_t i : *set) {
+                B->scratch_off[i] = used;
+                used += B->scratch_need[i];
+                chunks.back().push_back(i);
+            }
+        }
+        region = used;
+    } else {
+        for (int64_t i : heavy) {
+            const size_t need = B->scratch_need[i];
+            const bool fresh = chunks.empty() || chunks.back().empty() || B->scratch_need[chunks.back()[0]] == 0 ||
+                               (chunk_bytes > 0 && used + need > (size_t)chunk_bytes);
+            if (fresh) {
+                chunks.emplace_back();
+                used = 0;
+            }
+            B->scratch_off[i] = used;
+            used += need;
+            region = std::max(region, used);
+            chunks.back().push_back(i);
+        }
     }
     for (int64_t i : light) B->scratch_off[i] = 0;
     B->nchunks = (int)chunks.size();
@@ -2356,26 +2383,33 @@ af_status batch_build(af_batch B, int64_t chunk_bytes, int lanes) {
     B->d_prefix = (int64_t *)B->prefix_buf.p;
     if (nprefix) HIPCHK(hipMemcpy(B->d_prefix, B->h_prefix.data(), nprefix * 8, hipMemcpyHostToDevice));
     // capture the grouped launches once; with two lanes the light chunk's
-    // launches go to a second stream forked from and joined back to the first
-    // (the capture turns the fork and join into graph dependencies)
+    // launches go to a second stream forked from and joined back to the first,
+    // with three lanes the mixed chunk's to a third (the capture turns the
+    // forks and joins into graph dependencies)
     if (!B->groups.empty()) {
-        const bool two = lanes == 2 && B->light_chunk && B->nchunks > 1;
-        hipStream_t s = nullptr, s2 = nullptr;
-        hipEvent_t fork = nullptr, join = nullptr;
+        const bool two = lanes >= 2 && B->light_chunk && B->nchunks > 1;
+        const bool third = B->mixed_chunk >= 0 && B->nchunks > 1;
+        hipStream_t s = nullptr, s2 = nullptr, s3 = nullptr;
+        hipEvent_t fork = nullptr, join = nullptr, join3 = nullptr;
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         af_status st = AF_SUCCESS;
+        if (two || third) st = HIPST(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
         if (two) {
-            st = HIPST(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-            if (st == AF_SUCCESS) st = HIPST(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+            if (st == AF_SUCCESS) st = HIPST(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
             if (st == AF_SUCCESS) st = HIPST(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+        }
+        if (third) {
+            if (st == AF_SUCCESS) st = HIPST(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking));
+            if (st == AF_SUCCESS) st = HIPST(hipEventCreateWithFlags(&join3, hipEventDisableTiming));
         }
         if (st == AF_SUCCESS) st = HIPST(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         const bool capturing = st == AF_SUCCESS;
-        if (two && st == AF_SUCCESS) st = HIPST(hipEventRecord(fork, s));
+        if ((two || third) && st == AF_SUCCESS) st = HIPST(hipEventRecord(fork, s));
         if (two && st == AF_SUCCESS) st = HIPST(hipStreamWaitEvent(s2, fork, 0));
+        if (third && st == AF_SUCCESS) st = HIPST(hipStreamWaitEvent(s3, fork, 0));
         for (const auto &g : B->groups) {
             if (st != AF_SUCCESS) break;
-            hipStream_t gs = two && g.chunk == 0 ? s2 : s;
+            hipStream_t gs = two && g.chunk == 0 ? s2 : third && g.chunk == B->mixed_chunk ? s3 : s;
             if (g.ext)
                 st = HIPST(launch_fast_ext_grouped(B->prec, g.L, g.B, g.conv, g.sign, B->d_descs + g.desc_off,
                                                    B->d_prefix + g.prefix_off, (int)g.rows.size(), g.total_tiles, gs));
@@ -2385,6 +2419,8 @@ af_status batch_build(af_batch B, int64_t chunk_bytes, int lanes) {
         }
         if (two && st == AF_SUCCESS) st = HIPST(hipEventRecord(join, s2));
         if (two && st == AF_SUCCESS) st = HIPST(hipStreamWaitEvent(s, join, 0));
+        if (third && st == AF_SUCCESS) st = HIPST(hipEventRecord(join3, s3));
+        if (third && st == AF_SUCCESS) st = HIPST(hipStreamWaitEvent(s, join3, 0));
         hipGraph_t graph = nullptr;
         if (capturing) {
             const af_status h__ = HIPST(hipStreamEndCapture(s, &graph));
@@ -2392,7 +2428,9 @@ af_status batch_build(af_batch B, int64_t chunk_bytes, int lanes) {
         }
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
+        if (join3) (void)hipEventDestroy(join3);
         if (s2) (void)hipStreamDestroy(s2);
+        if (s3) (void)hipStreamDestroy(s3);
         (void)hipStreamDestroy(s);
         if (st != AF_SUCCESS) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -2528,7 +2566,7 @@ af_status af_batch_create(int64_t count, const int64_t *lengths, int32_t sign, i
 }
 
 af_status af_batch_set_schedule(af_batch B, int64_t chunk_bytes, int32_t lanes) {
-    if (!B || chunk_bytes < 0 || (lanes != 1 && lanes != 2)) return AF_INVALID_ARG;
+    if (!B || chunk_bytes < 0 || lanes < 1 || lanes > 3 || (lanes == 3 && chunk_bytes != 0)) return AF_INVALID_ARG;
     std::lock_guard<std::mutex> lk(B->mu);
     if (B->recorded) HIPCHK(hipEventSynchronize(B->done));
     B->recorded = false;
@@ -2559,6 +2597,7 @@ af_status af_batch_tune(af_batch B, void *stream, char *report, size_t report_le
     std::vector<std::pair<int64_t, int>> candidates = {{start, start_lanes}};
     for (const std::pair<int64_t, int> c : {std::pair<int64_t, int>{0, 1},
                                             {0, 2},
+                                            {0, 3},
                                             {(int64_t)32 << 20, 2},
                                             {(int64_t)64 << 20, 2},
                                             {(int64_t)128 << 20, 2},

@@ -342,13 +342,15 @@ af_status af_batch_destroy(af_batch batch);
  * intermediates in chunks of at most chunk_bytes of them (0: no chunking; a
  * row needing more is a chunk of its own), and lanes = 2 to run the rows
  * without intermediates on a branch of the graph beside the others (their
- * small launches then overlap the large ones) or 1 for one sequence.
- * Recaptures the graph.  The default is (0, 2).
+ * small launches then overlap the large ones) or 1 for one sequence; lanes = 3
+ * (with chunk_bytes = 0 only) also moves the rows with intermediates that use
+ * the generic kernel (mixed radix, Rader) to a third branch beside the
+ * power-of-two Bluestein rows.  Recaptures the graph.  The default is (0, 2).
  */
 af_status af_batch_set_schedule(af_batch batch, int64_t chunk_bytes, int32_t lanes);
 /*
  * Autotuner of a batch (no reference counterpart): times the batch's graph
- * for several schedules (one lane unchunked; two lanes unchunked and with
+ * for several schedules (one lane unchunked; three lanes unchunked; two lanes unchunked and with
  * chunks of 32 MiB ... 384 MiB of intermediates) on zero-filled interleaved
  * buffers it allocates itself, keeps the fastest (it must beat the incumbent
  * by 3%) and recaptures the graph with it.  Then it times every grouped

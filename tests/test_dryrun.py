@@ -220,7 +220,7 @@ def test_uneven_batch_4096_rows(dry, prec):
         rep = ctypes.create_string_buffer(8192)
         assert dry.af_batch_tune(h, None, rep, len(rep)) == 0
         text = rep.value.decode()
-        assert text.count("chunk_bytes=") == 8 and "chosen chunk_bytes=0 lanes=2" in text, text
+        assert text.count("chunk_bytes=") == 9 and "chosen chunk_bytes=0 lanes=2" in text, text
         orders = [ln for ln in text.split("\n") if ln.startswith("launch ")]
         assert orders and all(" order=0 " in ln for ln in orders), text
         assert dry.af_batch_describe(h, desc, len(desc)) == 0

@@ -846,7 +846,7 @@ def test_uneven_batch_chunks_and_tuner():
     b = UnevenBatch(lens, forward, "f32")
     ref = b(x32)
     torch.cuda.synchronize()
-    for cb, lanes in ((32 << 20, 1), (160 << 20, 2), (0, 2)):
+    for cb, lanes in ((32 << 20, 1), (160 << 20, 2), (0, 3), (0, 2)):
         b.set_schedule(cb, lanes)
         head = b.describe().split("\n")[0]
         assert f"chunk_bytes={cb}" in head and f"lanes={lanes}" in head, head
@@ -862,7 +862,7 @@ def test_uneven_batch_chunks_and_tuner():
               f"identical rows {int((rel == 0).sum())} of {len(rel)}")
         del got
     rep = b.tune()
-    assert rep.count("chunk_bytes=") == 8 and "chosen chunk_bytes=" in rep, rep
+    assert rep.count("chunk_bytes=") == 9 and "chosen chunk_bytes=" in rep, rep
     assert " order=" in rep, rep
     print(rep)
     got = b(x32)
@@ -873,7 +873,9 @@ def test_uneven_batch_chunks_and_tuner():
     with pytest.raises(ValueError):
         b.set_schedule(-1)
     with pytest.raises(ValueError):
-        b.set_schedule(0, 3)
+        b.set_schedule(0, 4)
+    with pytest.raises(ValueError):
+        b.set_schedule(32 << 20, 3)
 
 
 def test_uneven_batch_two_streams_and_threads():
