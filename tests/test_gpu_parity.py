@@ -295,6 +295,61 @@ def test_karatsuba_matrix_core_bound():
         assert [int(v) for v in got] == ref, (p, n)
 
 
+def test_random_lengths_real_and_karatsuba():
+    """Seeded random lengths: toSpectrum / fromSpectrum and the real
+    convolution with its adjoint at 24 even lengths up to 60000 in both
+    precisions (every kernel path: fused power-of-two, generic, packed beyond
+    one pass), and one ragged Karatsuba batch of 48 pairs with lengths up to
+    6000 (matrix-core, integer and split-above-the-LDS-limit pairs, both
+    coefficient ranges) bit for bit against the oracle.  FP32 results meet the
+    oracle's FP64 values at 1e-6.  FP64 results meet numpy's FFT at 1e-12 and
+    the oracle at the reference's 1e-10 (Fourier.hs:34-35): for half lengths
+    that the reference plans with Bluestein (3412, 3413, ...) its chirp phase
+    k^2 / n is not reduced modulo 2n before the sine and cosine, and the oracle,
+    which restates it, errs by 5e-13 .. 1e-12 against numpy where this path
+    stays near 1e-15."""
+    import oracle
+    rng = np.random.default_rng(23)
+    lens = sorted(set(2 * int(h) for h in rng.integers(1, 30000, 24)))
+
+    def conv_np(a, b, conj_b=False):
+        fa, fb = np.fft.rfft(a), np.fft.rfft(b)
+        return np.fft.irfft(fa * (np.conj(fb) if conj_b else fb), n=a.shape[-1]) * a.shape[-1]
+
+    for i, n in enumerate(lens):
+        x = synthetic.normal(300 + i, 2 * n).reshape(2, n)
+        k = synthetic.normal(400 + i, 2 * n).reshape(2, n) / np.sqrt(n)
+        w = synthetic.normal(500 + i, 2 * n).reshape(2, n)
+        Sx, Sk, Sw = oracle.to_spectrum(x), oracle.to_spectrum(k), oracle.to_spectrum(w)
+        orc = [Sx, oracle.from_spectrum(Sx), oracle.from_spectrum(Sx * Sk), oracle.from_spectrum(Sw * np.conj(Sk)),
+               oracle.from_spectrum(Sw * np.conj(Sx))]
+        npy = [np.fft.fft(x), None, conv_np(x, k), conv_np(w, k, True), conv_np(w, x, True)]
+        for rt in (torch.float64, torch.float32):
+            S = real.to_spectrum(planned.transform(forward, n // 2), to_dev(x, rt))
+            y = real.from_spectrum(planned.transform(inverse, n // 2), to_dev(Sx, S.dtype))
+            yc = real.convolve(to_dev(x, rt), to_dev(k, rt))
+            dx, dk = real.convolve_vjp(to_dev(x, rt), to_dev(k, rt), to_dev(w, rt))
+            for j, got in enumerate(host(t) for t in (S, y, yc, dx, dk)):
+                if rt == torch.float32:
+                    assert rel_l2(got, orc[j]) <= 1e-6, (n, j, "f32")
+                else:
+                    assert rel_l2(got, orc[j]) <= 1e-10, (n, j, "f64 vs oracle")
+                    ref = npy[j] if npy[j] is not None else np.fft.ifft(Sx, axis=-1).real * n
+                    assert rel_l2(got, ref) <= 1e-12, (n, j, "f64 vs numpy")
+    klens = [int(v) for v in rng.integers(1, 6001, 48)]
+    mags = [2 ** 15 if p % 3 else 2 ** 40 for p in range(len(klens))]
+    xs = [synthetic.integers(600 + p, L, -m, m) for p, (L, m) in enumerate(zip(klens, mags))]
+    ys = [synthetic.integers(700 + p, L, -m, m) for p, (L, m) in enumerate(zip(klens, mags))]
+    off = np.concatenate([[0], np.cumsum(klens)]).astype(np.int64)
+    z = host(convolution.karatsuba_ragged(off, to_dev(np.concatenate(xs), torch.int64),
+                                          to_dev(np.concatenate(ys), torch.int64)))
+    for p, L in enumerate(klens):
+        ref = oracle.karatsuba(xs[p][None, :], ys[p][None, :])[0]
+        assert np.array_equal(z[2 * off[p] - p: 2 * off[p] - p + 2 * L - 1], ref), (p, L, mags[p])
+    print(f"real: {len(lens)} even lengths {lens[0]}..{lens[-1]}; Karatsuba: {len(klens)} pairs, "
+          f"{sum(L > 4608 for L in klens)} above the LDS limit")
+
+
 @pytest.mark.parametrize("L", [4609, 5000, 16384, 100000])
 def test_karatsuba_past_lds_limit(L):
     """Convolution.Adhoc.karatsuba has no length limit (Adhoc.hs:33-64): pairs
